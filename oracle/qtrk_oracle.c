@@ -1,0 +1,942 @@
+/*
+ * qtrk_oracle.c -- CPU oracle (test infrastructure; see qtrk_oracle.h).
+ *
+ * Restates, operation by operation and in the same floating-point types, the
+ * reference CPU tracker.  Built with -ffp-contract=off (the reference is
+ * built with MSVC /fp:precise, which does not fuse multiply-adds).
+ */
+#define _GNU_SOURCE
+#include "qtrk_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+#ifdef ORACLE_FP64
+#define FC(x) ((fl)(x))
+#define FSQRT sqrt
+#define FEXP exp
+#define FCOS cos
+#define FSIN sin
+#define FPOW pow
+#define FABS fabs
+#define FACOS acos
+#define FERF erf
+#else
+#define FC(x) ((fl)(x##f))
+#define FSQRT sqrtf
+#define FEXP expf
+#define FCOS cosf
+#define FSIN sinf
+#define FPOW powf
+#define FABS fabsf
+#define FACOS acosf
+#define FERF erff
+#endif
+
+#define MIN_RADPROFILE_SMP_COUNT 4 /* cputrack/QueuedTracker.h:18 */
+
+enum { M_XCOR1D = 1, M_QI = 2, M_GAUSS2D = 4, M_ZLUTALIGN = 8, M_LOCALIZEZ = 16,
+       M_NORMPROF = 64, M_CLEAR4 = 128, M_FOURIERLUT = 256, M_ZWEIGHTED = 512 };
+
+/* ---------------------------------------------------------------- config */
+
+/* cputrack/utils.cpp:644-652 */
+int oq_nearest_pow2(int v)
+{
+	int r = 1;
+	while (r < v) r *= 2;
+	if (fabsf((float)(r - v)) < fabsf((float)(r / 2 - v))) return r;
+	return r / 2;
+}
+
+/* cputrack/qtrk_c_api.h:60-81 */
+void oq_default_settings(OConfig* c)
+{
+	memset(c, 0, sizeof(*c));
+	c->width = c->height = 150;
+	c->numThreads = -1;
+	c->xc1_profileLength = 128;
+	c->xc1_profileWidth = 32;
+	c->xc1_iterations = 2;
+	c->zlut_minradius = 1.0f;
+	c->zlut_angular_coverage = 0.7f;
+	c->zlut_radial_coverage = 3.0f;
+	c->zlut_roi_coverage = 1.0f;
+	c->qi_iterations = 4;
+	c->qi_minradius = 1;
+	c->qi_angular_coverage = 0.7f;
+	c->qi_radial_coverage = 3.0f;
+	c->qi_roi_coverage = 1.0f;
+	c->qi_angstep_factor = 1.0f;
+	c->cuda_device = -2;
+	c->com_bgcorrection = 0.0f;
+	c->gauss2D_iterations = 6;
+	c->gauss2D_sigma = 4;
+	c->downsample = 0;
+}
+
+/* QTrkComputedConfig::Update, cputrack/QueuedTracker.cpp:6-21 (float arithmetic,
+ * float->int truncation, pi written as 3.141593f). */
+void oq_config_update(OConfig* c)
+{
+	int roi = c->width / 2;
+	c->zlut_maxradius = roi * c->zlut_roi_coverage;
+	float zlut_perimeter = 2 * 3.141593f * c->zlut_maxradius;
+	c->zlut_angularsteps = (int)(zlut_perimeter * c->zlut_angular_coverage);
+	c->zlut_radialsteps = (int)((c->zlut_maxradius - c->zlut_minradius) * c->zlut_radial_coverage);
+	c->qi_maxradius = roi * c->qi_roi_coverage;
+	float qi_perimeter = 2 * 3.141593f * c->qi_maxradius;
+	c->qi_angstepspq = (int)(qi_perimeter * c->qi_angular_coverage / 4);
+	c->qi_radialsteps = (int)((c->qi_maxradius - c->qi_minradius) * c->qi_radial_coverage);
+	c->qi_radialsteps = oq_nearest_pow2(c->qi_radialsteps);
+}
+
+/* ------------------------------------------------------------- sampling */
+
+/* Bilinear sample, cputrack/utils.h:36-58.  The reference's `int rx=x`
+ * truncates toward zero and treats rx<0 || rx+1>=w as outside; for w,h>=2 that
+ * is exactly x <= -1 || x >= w-1 (so x in (-1,0) is inside and extrapolates).
+ * Written as a range test so NaN and out-of-int-range coordinates are
+ * "outside", as x86's truncating conversion makes them in the reference. */
+fl oq_interpolate(const fl* img, int w, int h, fl x, fl y, int* outside)
+{
+	if (w < 2 || h < 2 || !(x > (fl)-1 && y > (fl)-1 && x < (fl)(w - 1) && y < (fl)(h - 1))) {
+		if (outside) *outside = 1;
+		return 0;
+	}
+	int rx = (int)x, ry = (int)y;
+	if (outside) *outside = 0;
+	fl v00 = img[w * ry + rx];
+	fl v10 = img[w * ry + rx + 1];
+	fl v01 = img[w * (ry + 1) + rx];
+	fl v11 = img[w * (ry + 1) + rx + 1];
+	fl fx = x - (fl)rx;
+	fl v0 = v00 + (v10 - v00) * fx;
+	fl v1 = v01 + (v11 - v01) * fx;
+	return v0 + (v1 - v0) * (y - (fl)ry);
+}
+
+/* ComputeRadialBinWindow, cputrack/utils.cpp:665-677 */
+void oq_radial_bin_window(int n, fl* out)
+{
+	for (int i = 0; i < n; i++) {
+		fl x = i / (fl)n;
+		fl t2 = FC(0.05);
+		fl t1 = FC(0.01);
+		fl omx = 1 - x;
+		fl fall = 1 - FEXP(-(omx * omx) / t2);
+		fl rise = 1 - FEXP(-(x * x) / t1);
+		out[i] = FSQRT(fall * rise * x * 2);
+	}
+}
+
+/* ------------------------------------------------------------------ FFT */
+
+static inline ocpx cmul(ocpx a, ocpx b)
+{
+	ocpx r = { a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re };
+	return r;
+}
+static inline ocpx cadd(ocpx a, ocpx b) { ocpx r = { a.re + b.re, a.im + b.im }; return r; }
+static inline ocpx csub(ocpx a, ocpx b) { ocpx r = { a.re - b.re, a.im - b.im }; return r; }
+
+/* Plan: twiddles exp(+-2*pi*i*k/n) and the "4s, then 2s, then odd" factor
+ * list of kissfft (cputrack/kissfft.h:11-47). */
+int oq_fft_init(OFFT* p, int n, int inverse)
+{
+	memset(p, 0, sizeof(*p));
+	p->n = n;
+	p->inverse = inverse;
+	p->tw = (ocpx*)malloc(sizeof(ocpx) * (size_t)n);
+	if (!p->tw) return -1;
+	fl phinc = (inverse ? 2 : -2) * FACOS((fl)-1) / n;
+	for (int i = 0; i < n; i++) {
+		fl a = i * phinc;
+		if (a == 0) { p->tw[i].re = 1; p->tw[i].im = a; }
+		else { p->tw[i].re = FCOS(a); p->tw[i].im = FSIN(a); }
+	}
+	int m = n, f = 4;
+	do {
+		while (m % f) {
+			switch (f) { case 4: f = 2; break; case 2: f = 3; break; default: f += 2; break; }
+			if (f * f > m) f = m;
+		}
+		m /= f;
+		p->radix[p->nstages] = f;
+		p->rem[p->nstages] = m;
+		p->nstages++;
+	} while (m > 1 && p->nstages < 32);
+	return 0;
+}
+
+void oq_fft_free(OFFT* p)
+{
+	free(p->tw);
+	p->tw = 0;
+}
+
+static void bfly2(const OFFT* p, ocpx* F, size_t fs, int m)
+{
+	for (int k = 0; k < m; k++) {
+		ocpx t = cmul(F[m + k], p->tw[k * fs]);
+		F[m + k] = csub(F[k], t);
+		F[k] = cadd(F[k], t);
+	}
+}
+
+static void bfly4(const OFFT* p, ocpx* F, size_t fs, size_t m)
+{
+	ocpx s[6];
+	int nii = p->inverse * -2 + 1;
+	for (size_t k = 0; k < m; k++) {
+		s[0] = cmul(F[k + m], p->tw[k * fs]);
+		s[1] = cmul(F[k + 2 * m], p->tw[k * fs * 2]);
+		s[2] = cmul(F[k + 3 * m], p->tw[k * fs * 3]);
+		s[5] = csub(F[k], s[1]);
+		F[k] = cadd(F[k], s[1]);
+		s[3] = cadd(s[0], s[2]);
+		s[4] = csub(s[0], s[2]);
+		ocpx r = { s[4].im * nii, -s[4].re * nii };
+		s[4] = r;
+		F[k + 2 * m] = csub(F[k], s[3]);
+		F[k] = cadd(F[k], s[3]);
+		F[k + m] = cadd(s[5], s[4]);
+		F[k + 3 * m] = csub(s[5], s[4]);
+	}
+}
+
+/* Generic radix-p DFT butterfly, cputrack/kissfft.h:256-290. */
+static void bfly_generic(const OFFT* p, ocpx* F, size_t fs, int m, int r)
+{
+	ocpx sc[64];
+	for (int u = 0; u < m; u++) {
+		int k = u;
+		for (int q1 = 0; q1 < r; q1++) { sc[q1] = F[k]; k += m; }
+		k = u;
+		for (int q1 = 0; q1 < r; q1++) {
+			int tw = 0;
+			F[k] = sc[0];
+			for (int q = 1; q < r; q++) {
+				tw += (int)fs * k;
+				if (tw >= p->n) tw -= p->n;
+				F[k] = cadd(F[k], cmul(sc[q], p->tw[tw]));
+			}
+			k += m;
+		}
+	}
+}
+
+/* Recursive decimation in time, cputrack/kissfft.h:78-111. */
+static void kf_work(const OFFT* p, int stage, ocpx* Fout, const ocpx* f, size_t fstride)
+{
+	int r = p->radix[stage], m = p->rem[stage];
+	ocpx* beg = Fout;
+	ocpx* end = Fout + r * m;
+	if (m == 1) {
+		do { *Fout = *f; f += fstride; } while (++Fout != end);
+	} else {
+		do { kf_work(p, stage + 1, Fout, f, fstride * r); f += fstride; } while ((Fout += m) != end);
+	}
+	Fout = beg;
+	if (r == 2) bfly2(p, Fout, fstride, m);
+	else if (r == 4) bfly4(p, Fout, fstride, m);
+	else bfly_generic(p, Fout, fstride, m, r);
+}
+
+void oq_fft_transform(const OFFT* p, const ocpx* in, ocpx* out) { kf_work(p, 0, out, in, 1); }
+
+void oq_fft(int n, int inverse, const fl* in_ri, fl* out_ri)
+{
+	OFFT p;
+	oq_fft_init(&p, n, inverse);
+	oq_fft_transform(&p, (const ocpx*)in_ri, (ocpx*)out_ri);
+	oq_fft_free(&p);
+}
+
+/* Input index read into each output slot at the leaves of kf_work. */
+static void leaf_rec(const OFFT* p, int stage, int out, int in, int fstride, int* perm)
+{
+	int r = p->radix[stage], m = p->rem[stage];
+	for (int q = 0; q < r; q++) {
+		if (m == 1) perm[out + q] = in + q * fstride;
+		else leaf_rec(p, stage + 1, out + q * m, in + q * fstride, fstride * r, perm);
+	}
+}
+
+int oq_fft_leaf_order(int n, int* perm)
+{
+	OFFT p;
+	if (oq_fft_init(&p, n, 0)) return -1;
+	leaf_rec(&p, 0, 0, 0, 1, perm);
+	oq_fft_free(&p);
+	return p.nstages;
+}
+
+/* -------------------------------------------------------- peak fitting */
+
+static const fl QIW[7] = { FC(0.14), FC(0.5), FC(0.85), FC(1.0), FC(0.85), FC(0.5), FC(0.14) };
+static const double ZW[7] = { FC(0.15), FC(0.5), FC(0.85), FC(1.0), FC(0.85), FC(0.5), FC(0.15) };
+
+/* LsqSqQuadFit<T>::calculate with weights, cputrack/LsqQuadraticFit.h:18-24, 64-102 */
+#define LSQFIT_BODY(T)                                                                        \
+	T Sx = 0, Sy = 0, Sx2 = 0, Sx3 = 0, Sxy = 0, Sx4 = 0, Sx2y = 0, Sw = 0;                   \
+	for (int i = 0; i < n; i++) {                                                             \
+		T x = X[i], y = Y[i];                                                                 \
+		Sx += x * W[i];                                                                       \
+		Sy += y * W[i];                                                                       \
+		T sq = x * x;                                                                         \
+		Sx2 += x * x * W[i];                                                                  \
+		Sx3 += sq * x * W[i];                                                                 \
+		Sx4 += sq * sq * W[i];                                                                \
+		Sxy += x * y * W[i];                                                                  \
+		Sx2y += sq * y * W[i];                                                                \
+		Sw += W[i];                                                                           \
+	}                                                                                         \
+	T s40 = Sx4, s30 = Sx3, s20 = Sx2, s10 = Sx, s21 = Sx2y, s11 = Sxy, s01 = Sy, s00 = Sw;   \
+	T d = s40 * (s20 * s00 - s10 * s10) - s30 * (s30 * s00 - s10 * s20) + s20 * (s30 * s10 - s20 * s20); \
+	T a = (s21 * (s20 * s00 - s10 * s10) - s11 * (s30 * s00 - s10 * s20) + s01 * (s30 * s10 - s20 * s20)) / d; \
+	T b = (s40 * (s11 * s00 - s01 * s10) - s30 * (s21 * s00 - s01 * s20) + s20 * (s21 * s10 - s11 * s20)) / d; \
+	T c = (s40 * (s20 * s01 - s10 * s11) - s30 * (s30 * s01 - s10 * s21) + s20 * (s30 * s11 - s20 * s21)) / d; \
+	abcd[0] = a; abcd[1] = b; abcd[2] = c; abcd[3] = d;
+
+void oq_lsqfit_f(int n, const fl* X, const fl* Y, const fl* W, fl* abcd) { LSQFIT_BODY(fl) }
+void oq_lsqfit_d(int n, const double* X, const double* Y, const double* W, double* abcd) { LSQFIT_BODY(double) }
+
+/* ComputeMaxInterp<float,7>::Compute with QI_LSQFIT_WEIGHTS,
+ * cputrack/LsqQuadraticFit.h:132-176, cputrack/QueuedTracker.h:106-107 */
+fl oq_max_interp_qi(const fl* data, int len, int* imax)
+{
+	int iMax = 0;
+	fl vMax = data[0];
+	for (int k = 1; k < len; k++)
+		if (data[k] > vMax) { vMax = data[k]; iMax = k; }
+	if (imax) *imax = iMax;
+	int s = iMax - 3 > 0 ? iMax - 3 : 0;
+	int e = iMax + 4 < len ? iMax + 4 : len;
+	int np = e - s;
+	if (np < 3) return (fl)iMax;
+	fl xs[7], abcd[4];
+	for (int i = s; i < e; i++) xs[i - s] = (fl)(i - iMax);
+	oq_lsqfit_f(np, xs, &data[s], QIW, abcd);
+	if (FABS(abcd[0]) < FC(1e-9)) return (fl)iMax;
+	return (fl)iMax + (-abcd[1] / (2 * abcd[0]));
+}
+
+/* ComputeMaxInterp<double,7>::Compute with ZLUT_LSQFIT_WEIGHTS,
+ * cputrack/cpu_tracker.cpp:860, cputrack/QueuedTracker.h:115-116 */
+double oq_max_interp_zlut(const double* data, int len, int* imax)
+{
+	int iMax = 0;
+	double vMax = data[0];
+	for (int k = 1; k < len; k++)
+		if (data[k] > vMax) { vMax = data[k]; iMax = k; }
+	if (imax) *imax = iMax;
+	int s = iMax - 3 > 0 ? iMax - 3 : 0;
+	int e = iMax + 4 < len ? iMax + 4 : len;
+	int np = e - s;
+	if (np < 3) return (double)iMax;
+	double xs[7], abcd[4];
+	for (int i = s; i < e; i++) xs[i - s] = (double)(i - iMax);
+	oq_lsqfit_d(np, xs, &data[s], ZW, abcd);
+	if (fabs(abcd[0]) < (double)FC(1e-9)) return (double)iMax;
+	return (double)iMax + (-abcd[1] / (2 * abcd[0]));
+}
+
+/* -------------------------------------------------------------- COM */
+
+/* CPUTracker::ComputeMeanAndCOM, cputrack/cpu_tracker.cpp:681-729: double sums
+ * of float values (v*v rounded to float first), float mean/stdev, then the
+ * background-corrected moments with x as the outer loop. */
+void oq_com(OWorker* wk, fl bgcorr, fl* comx, fl* comy)
+{
+	int w = wk->w, h = wk->h;
+	const fl* img = wk->img;
+	double sum = 0, sum2 = 0, mX = 0, mY = 0;
+	for (int y = 0; y < h; y++)
+		for (int x = 0; x < w; x++) {
+			fl v = img[y * w + x];
+			sum += v;
+			sum2 += (fl)(v * v);
+		}
+	fl invN = FC(1.0) / (w * h);
+	wk->mean = (fl)(sum * invN);
+	double var = sum2 * invN - (fl)(wk->mean * wk->mean);
+	wk->stdev = FSQRT((fl)var);
+	sum = 0;
+	for (int x = 0; x < w; x++)
+		for (int y = 0; y < h; y++) {
+			fl v = img[y * w + x];
+			fl t = FABS(v - wk->mean) - bgcorr * wk->stdev;
+			v = ((fl)0 < t) ? t : (fl)0;
+			sum += v;
+			mX += x * (double)v;
+			mY += y * (double)v;
+		}
+	*comx = (fl)(mX / sum);
+	*comy = (fl)(mY / sum);
+}
+
+/* --------------------------------------------------------------- QI */
+
+static int check_bounds(int w, int h, fl cx, fl cy, fl r)
+{
+	return cx + r >= w || cx - r < 0 || cy + r >= h || cy - r < 0;
+}
+
+/* CPUTracker::ComputeQuadrantProfile, cputrack/cpu_tracker.cpp:631-677 */
+void oq_quadrant_profile(OWorker* wk, fl* dst, int nr, int angularSteps, int q, fl minr, fl maxr,
+                         fl cx, fl cy, const fl* weights)
+{
+	static const int qmat[8] = { 1, 1, -1, 1, -1, -1, 1, -1 };
+	int mx = qmat[2 * q], my = qmat[2 * q + 1];
+	if (angularSteps < MIN_RADPROFILE_SMP_COUNT) angularSteps = MIN_RADPROFILE_SMP_COUNT;
+	fl rstep = (maxr - minr) / nr;
+	fl angstepf = (fl)wk->ndirs / angularSteps;
+	int istep = (int)angstepf;
+	for (int i = 0; i < nr; i++) {
+		fl sum = 0;
+		fl r = minr + rstep * i;
+		int n = 0;
+		for (int a = 0; a < angularSteps; a++) {
+			int j = istep * a;
+			fl x = cx + mx * wk->qdirs[2 * j] * r;
+			fl y = cy + my * wk->qdirs[2 * j + 1] * r;
+			int out;
+			fl v = oq_interpolate(wk->img, wk->w, wk->h, x, y, &out);
+			if (!out) { sum += v; n++; }
+		}
+		dst[i] = n >= MIN_RADPROFILE_SMP_COUNT ? sum / n : wk->mean;
+		if (weights) dst[i] *= weights[i];
+	}
+}
+
+/* CPUTracker::QI_ComputeOffset, cputrack/cpu_tracker.cpp:375-406 */
+fl oq_qi_offset(OWorker* wk, ocpx* profile, int nr, int* imax)
+{
+	int N = nr * 2;
+	ocpx* rev = (ocpx*)alloca(sizeof(ocpx) * N);
+	ocpx* fo = (ocpx*)alloca(sizeof(ocpx) * N);
+	ocpx* fo2 = (ocpx*)alloca(sizeof(ocpx) * N);
+	fl* ac = (fl*)alloca(sizeof(fl) * N);
+	for (int x = 0; x < N; x++) rev[x] = profile[N - 1 - x];
+	oq_fft_transform(&wk->fwd, profile, fo);
+	oq_fft_transform(&wk->fwd, rev, fo2);
+	for (int x = 0; x < N; x++) {
+		ocpx cj = { fo2[x].re, -fo2[x].im };
+		fo[x] = cmul(fo[x], cj);
+	}
+	oq_fft_transform(&wk->inv, fo, fo2);
+	for (int x = 0; x < N; x++) ac[x] = fo2[(x + nr) % N].re;
+	fl maxPos = oq_max_interp_qi(ac, N, imax);
+	return (maxPos - nr) * (FC(3.14159265359) / 4);
+}
+
+static void ensure_fft(OFFT* p, int n, int inverse)
+{
+	if (p->tw && p->n == n) return;
+	if (p->tw) oq_fft_free(p);
+	oq_fft_init(p, n, inverse);
+}
+
+/* CPUTracker::ComputeQI, cputrack/cpu_tracker.cpp:285-360 */
+void oq_qi(OWorker* wk, fl cx, fl cy, int iterations, int nr, int A, float factor, fl minr, fl maxr,
+           const fl* weights, fl* outx, fl* outy, int* boundary, OTrace* tr)
+{
+	if (A != wk->ndirs) {
+		free(wk->qdirs);
+		wk->qdirs = (fl*)malloc(sizeof(fl) * 2 * (size_t)(A > 0 ? A : 1));
+		wk->ndirs = A;
+		for (int j = 0; j < A; j++) {
+#ifdef ORACLE_FP64
+			fl ang = 0.5 * 3.141593 * (j + 0.5) / (double)A;
+#else
+			float ang = (float)(0.5 * (double)3.141593f * (double)(j + 0.5f) / (double)(float)A);
+#endif
+			wk->qdirs[2 * j] = FCOS(ang);
+			wk->qdirs[2 * j + 1] = FSIN(ang);
+		}
+	}
+	ensure_fft(&wk->fwd, nr * 2, 0);
+	ensure_fft(&wk->inv, nr * 2, 1);
+	fl* buf = (fl*)alloca(sizeof(fl) * nr * 4);
+	fl *q0 = buf, *q1 = buf + nr, *q2 = buf + 2 * nr, *q3 = buf + 3 * nr;
+	ocpx* c0 = (ocpx*)alloca(sizeof(ocpx) * nr * 2);
+	ocpx* c1 = c0 + nr;
+	fl ppl = (maxr - minr) / nr;
+	int bh = 0;
+	fl angsteps = A / (fl)FPOW((fl)factor, (fl)iterations);
+	for (int k = 0; k < iterations; k++) {
+		bh = check_bounds(wk->w, wk->h, cx, cy, maxr);
+		for (int q = 0; q < 4; q++)
+			oq_quadrant_profile(wk, buf + q * nr, nr, (int)angsteps, q, minr, maxr, cx, cy, weights);
+		for (int r = 0; r < nr; r++) {
+			c0[nr - r - 1].re = q1[r] + q2[r]; c0[nr - r - 1].im = 0;
+			c1[r].re = q0[r] + q3[r]; c1[r].im = 0;
+		}
+		int ix, iy;
+		fl offX = oq_qi_offset(wk, c0, nr, &ix);
+		for (int r = 0; r < nr; r++) {
+			c1[r].re = q0[r] + q1[r]; c1[r].im = 0;
+			c0[nr - r - 1].re = q2[r] + q3[r]; c0[nr - r - 1].im = 0;
+		}
+		fl offY = oq_qi_offset(wk, c0, nr, &iy);
+		cx += offX * ppl;
+		cy += offY * ppl;
+		angsteps *= (fl)factor;
+		if (tr && k < 16) { tr->qi_x[k] = cx; tr->qi_y[k] = cy; tr->qi_imax_x[k] = ix; tr->qi_imax_y[k] = iy; }
+	}
+	*outx = cx;
+	*outy = cy;
+	*boundary = bh;
+}
+
+/* ------------------------------------------------------- radial profile */
+
+/* NormalizeRadialProfile, cputrack/utils.cpp:230-261 */
+void oq_normalize_profile(fl* prof, int n)
+{
+	double sum = 0;
+	for (int i = 0; i < n; i++) sum += prof[i];
+	fl mean = (fl)(sum / n);
+	double rms2 = 0;
+	for (int i = 0; i < n; i++) {
+		prof[i] -= mean;
+		rms2 += (fl)(prof[i] * prof[i]);
+	}
+	double inv = FC(1.0) / sqrt(rms2 / n);
+	for (int i = 0; i < n; i++) prof[i] = (fl)(prof[i] * inv);
+}
+
+/* ::ComputeRadialProfile, cputrack/utils.cpp:275-320 */
+void oq_radial_profile(const fl* img, int w, int h, fl* dst, int nbins, int nang, fl minr, fl maxr,
+                       fl cx, fl cy, fl mean, int normalize)
+{
+	fl* dirs = (fl*)alloca(sizeof(fl) * 2 * (nang > 0 ? nang : 1));
+	for (int j = 0; j < nang; j++) {
+		fl ang = 2 * FC(3.141593) * j / (fl)nang;
+		dirs[2 * j] = FCOS(ang);
+		dirs[2 * j + 1] = FSIN(ang);
+	}
+	fl rstep = (maxr - minr) / nbins;
+	for (int i = 0; i < nbins; i++) {
+		double sum = 0;
+		int n = 0;
+		fl r = minr + rstep * i;
+		for (int a = 0; a < nang; a++) {
+			fl x = cx + dirs[2 * a] * r;
+			fl y = cy + dirs[2 * a + 1] * r;
+			int out;
+			fl v = oq_interpolate(img, w, h, x, y, &out);
+			if (!out) { sum += v; n++; }
+		}
+		dst[i] = n > MIN_RADPROFILE_SMP_COUNT ? (fl)(sum / n) : mean;
+	}
+	if (normalize) oq_normalize_profile(dst, nbins);
+}
+
+/* CPUTracker::LUTProfileCompare, cputrack/cpu_tracker.cpp:791-868 (the
+ * compiled #else branch :829-839, quadratic-fit mode :847-861) */
+fl oq_lut_compare(const fl* prof, const float* lut, int planes, int res, const float* weights,
+                  double* scores, int* best)
+{
+	for (int k = 0; k < planes; k++) {
+		double s = 0;
+		for (int r = 0; r < res; r++) {
+			double d = (fl)(prof[r] - (fl)lut[k * res + r]);
+			if (weights) d *= weights[r];
+			d = -d * d;
+			s += d;
+		}
+		scores[k] = s;
+	}
+	return (fl)oq_max_interp_zlut(scores, planes, best);
+}
+
+/* ------------------------------------------------------------ job */
+
+int oq_worker_init(OWorker* wk, const OTracker* t)
+{
+	memset(wk, 0, sizeof(*wk));
+	wk->w = t->cfg.width;
+	wk->h = t->cfg.height;
+	wk->img = (fl*)calloc((size_t)wk->w * wk->h, sizeof(fl));
+	return wk->img ? 0 : -1;
+}
+
+void oq_worker_free(OWorker* wk)
+{
+	free(wk->img);
+	free(wk->qdirs);
+	oq_fft_free(&wk->fwd); oq_fft_free(&wk->inv);
+	oq_fft_free(&wk->xfwd); oq_fft_free(&wk->xinv);
+	oq_fft_free(&wk->qafwd); oq_fft_free(&wk->qainv);
+	memset(wk, 0, sizeof(*wk));
+}
+
+/* CPUTracker::SetImage / SetImageFloat, cputrack/cpu_tracker.h:123-136,
+ * cputrack/cpu_tracker.cpp:91-96 */
+void oq_set_image(OWorker* wk, const void* data, int pitch, int pdt)
+{
+	const unsigned char* bp = (const unsigned char*)data;
+	for (int y = 0; y < wk->h; y++) {
+		for (int x = 0; x < wk->w; x++) {
+			fl v;
+			if (pdt == 0) v = (fl)bp[x];
+			else if (pdt == 1) v = (fl)((const uint16_t*)bp)[x];
+			else v = (fl)((const float*)bp)[x];
+			wk->img[y * wk->w + x] = v;
+		}
+		bp += pitch;
+	}
+	wk->mean = 0;
+}
+
+/* QueuedCPUTracker::ProcessJob, cputrack/QueuedCPUTracker.cpp:273-361 (COM,
+ * QI and Z-LUT modes; bias correction is applied by the caller). */
+void oq_process_job(const OTracker* t, OWorker* wk, const void* roi, int pitch, int pdt,
+                    const OJob* job, OResult* out, OTrace* tr)
+{
+	const OConfig* c = &t->cfg;
+	int mode = t->mode;
+	oq_set_image(wk, roi, pitch, pdt);
+	if (mode & M_CLEAR4) { wk->img[0] = wk->img[1] = wk->img[2] = wk->img[3] = 0; }
+	if (t->calib_offset || t->calib_gain) {
+		size_t idx = (size_t)c->width * c->height * job->zlutIndex;
+		int npx = wk->w * wk->h;
+		const float* off = t->calib_offset ? t->calib_offset + idx : 0;
+		const float* gn = t->calib_gain ? t->calib_gain + idx : 0;
+		for (int i = 0; i < npx; i++) {
+			if (off && !gn) wk->img[i] = wk->img[i] + off[i] * 1.0f;
+			if (gn && !off) wk->img[i] = wk->img[i] * gn[i] * 1.0f;
+			if (gn && off) wk->img[i] = (wk->img[i] + off[i] * 1.0f) * gn[i] * 1.0f;
+		}
+	}
+	memset(out, 0, sizeof(*out));
+	out->job = *job;
+	fl comx, comy;
+	oq_com(wk, (fl)c->com_bgcorrection, &comx, &comy);
+	out->imageMean = (float)wk->mean;
+	if (isnan(comx) || isnan(comy)) { comx = (fl)(c->width / 2); comy = (fl)(c->height / 2); }
+	if (tr) { tr->comx = comx; tr->comy = comy; tr->mean = wk->mean; tr->stdev = wk->stdev; }
+	int boundary = 0;
+	fl px = 0, py = 0, pz = 0;
+	if (mode & M_QI) {
+		out->comx = (float)comx; out->comy = (float)comy;
+		oq_qi(wk, comx, comy, c->qi_iterations, c->qi_radialsteps, c->qi_angstepspq, c->qi_angstep_factor,
+		      (fl)c->qi_minradius, (fl)c->qi_maxradius, t->qi_window, &px, &py, &boundary, tr);
+	} else if (mode & M_GAUSS2D) {
+		out->comx = (float)comx; out->comy = (float)comy;
+		fl g[4];
+		oq_gauss2d_mle(wk->img, wk->w, wk->h, wk->mean, comx, comy, c->gauss2D_iterations, (fl)c->gauss2D_sigma, g);
+		px = g[0]; py = g[1];
+	} else {
+		out->comx = (float)comx; out->comy = (float)comy;
+		px = comx; py = comy;
+	}
+	if (mode & M_LOCALIZEZ) {
+		int res = c->zlut_radialsteps;
+		fl* prof = tr && tr->profile ? tr->profile : (fl*)alloca(sizeof(fl) * res);
+		oq_radial_profile(wk->img, wk->w, wk->h, prof, res, c->zlut_angularsteps, (fl)c->zlut_minradius,
+		                  (fl)c->zlut_maxradius, px, py, wk->mean, (mode & M_NORMPROF) != 0);
+		boundary = check_bounds(wk->w, wk->h, px, py, (fl)c->zlut_maxradius);
+		if (t->zluts && job->zlutIndex >= 0 && job->zlutIndex < t->zlut_count) {
+			double* sc = tr && tr->scores ? tr->scores : (double*)alloca(sizeof(double) * t->zlut_planes);
+			int best = 0;
+			const float* lut = t->zluts + (size_t)job->zlutIndex * t->zlut_planes * res;
+			pz = oq_lut_compare(prof, lut, t->zlut_planes, res, t->zcmp, sc, &best);
+			if (tr) tr->best_plane = best;
+		}
+	}
+	out->x = (float)px; out->y = (float)py; out->z = (float)pz;
+	out->error = boundary ? 1 : 0;
+}
+
+static void tracker_setup(OTracker* t, const OConfig* cfg, int mode, const float* zluts, int zc, int zp,
+                          const float* zcmp)
+{
+	memset(t, 0, sizeof(*t));
+	t->cfg = *cfg;
+	t->mode = mode;
+	t->zluts = zluts;
+	t->zlut_count = zc;
+	t->zlut_planes = zp;
+	t->zcmp = zcmp;
+	t->qi_window = (fl*)malloc(sizeof(fl) * (size_t)(cfg->qi_radialsteps > 0 ? cfg->qi_radialsteps : 1));
+	oq_radial_bin_window(cfg->qi_radialsteps, t->qi_window);
+}
+
+static int pdt_bytes(int pdt) { return pdt == 0 ? 1 : pdt == 1 ? 2 : 4; }
+
+void oq_localize_batch(const OConfig* cfg, int mode, const float* zluts, int zcount, int zplanes,
+                       const float* zcmp, const void* rois, int pdt, const OJob* jobs, int n, OResult* out)
+{
+	OTracker t;
+	tracker_setup(&t, cfg, mode, zluts, zcount, zplanes, zcmp);
+	OWorker wk;
+	oq_worker_init(&wk, &t);
+	size_t roibytes = (size_t)cfg->width * cfg->height * pdt_bytes(pdt);
+	for (int i = 0; i < n; i++)
+		oq_process_job(&t, &wk, (const char*)rois + roibytes * i, cfg->width * pdt_bytes(pdt), pdt, &jobs[i], &out[i], 0);
+	oq_worker_free(&wk);
+	free(t.qi_window);
+}
+
+/* ----------------------------------------------- queued CPU tracker */
+
+/* Restatement of QueuedCPUTracker's job queue (cputrack/QueuedCPUTracker.cpp:37-94,
+ * 213-256, 435-469): the scheduler copies each ROI into a pooled job and
+ * blocks while jobs+inprogress >= max(2,threads)*50; workers pop the front,
+ * sleep 1 ms when idle, and push results to the back. */
+typedef struct {
+	OTracker t;
+	pthread_mutex_t jm, rm;
+	unsigned char* slab;       /* job pool storage: maxq slots of one ROI */
+	OJob* jobjob;
+	int* freelist; int nfree;
+	int* queue; int qhead, qcount, qcap;
+	int inprogress;
+	OResult* results; int nresults;
+	int maxq, quit;
+	size_t roibytes; int pdt;
+} OQueue;
+
+typedef struct { OQueue* q; } OWorkerArg;
+
+static void sleep_ms(int ms)
+{
+	struct timespec ts = { ms / 1000, (long)(ms % 1000) * 1000000L };
+	nanosleep(&ts, 0);
+}
+
+static void* queue_worker(void* arg)
+{
+	OQueue* q = ((OWorkerArg*)arg)->q;
+	OWorker wk;
+	oq_worker_init(&wk, &q->t);
+	while (1) {
+		int slot = -1;
+		pthread_mutex_lock(&q->jm);
+		if (q->quit) { pthread_mutex_unlock(&q->jm); break; }
+		if (q->qcount > 0) {
+			slot = q->queue[q->qhead];
+			q->qhead = (q->qhead + 1) % q->qcap;
+			q->qcount--;
+			q->inprogress++;
+		}
+		pthread_mutex_unlock(&q->jm);
+		if (slot < 0) { sleep_ms(1); continue; }
+		OResult r;
+		oq_process_job(&q->t, &wk, q->slab + q->roibytes * slot, q->t.cfg.width * pdt_bytes(q->pdt), q->pdt,
+		               &q->jobjob[slot], &r, 0);
+		pthread_mutex_lock(&q->rm);
+		q->results[q->nresults++] = r;
+		pthread_mutex_unlock(&q->rm);
+		pthread_mutex_lock(&q->jm);
+		q->freelist[q->nfree++] = slot;
+		q->inprogress--;
+		pthread_mutex_unlock(&q->jm);
+	}
+	oq_worker_free(&wk);
+	return 0;
+}
+
+static double now_s(void)
+{
+	struct timespec ts;
+	clock_gettime(CLOCK_MONOTONIC, &ts);
+	return ts.tv_sec + ts.tv_nsec * 1e-9;
+}
+
+/* Runs n jobs through the queue with `nthreads` workers; returns the wall time
+ * from the first schedule until all n results exist (the reference's
+ * WaitForFinish clock, cputrack-test/SharedTests.h:300-314).  Results are
+ * returned in fetch order (the reference fetches from the back: LIFO). */
+double oq_queued_run(const OConfig* cfg, int mode, const float* zluts, int zcount, int zplanes,
+                     const float* zcmp, const void* rois, int pdt, const OJob* jobs, int n,
+                     int nthreads, OResult* out)
+{
+	OQueue q;
+	memset(&q, 0, sizeof(q));
+	tracker_setup(&q.t, cfg, mode, zluts, zcount, zplanes, zcmp);
+	if (nthreads <= 0) nthreads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+	q.maxq = (nthreads > 2 ? nthreads : 2) * 50;
+	q.pdt = pdt;
+	q.roibytes = (size_t)cfg->width * cfg->height * pdt_bytes(pdt);
+	q.qcap = q.maxq + nthreads + 1;
+	q.slab = (unsigned char*)malloc(q.roibytes * q.qcap);
+	q.jobjob = (OJob*)malloc(sizeof(OJob) * q.qcap);
+	q.freelist = (int*)malloc(sizeof(int) * q.qcap);
+	q.queue = (int*)malloc(sizeof(int) * q.qcap);
+	q.results = (OResult*)malloc(sizeof(OResult) * (size_t)(n > 0 ? n : 1));
+	for (int i = 0; i < q.qcap; i++) q.freelist[q.nfree++] = i;
+	pthread_mutex_init(&q.jm, 0);
+	pthread_mutex_init(&q.rm, 0);
+	pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * nthreads);
+	OWorkerArg wa = { &q };
+	for (int i = 0; i < nthreads; i++) pthread_create(&th[i], 0, queue_worker, &wa);
+
+	double t0 = now_s();
+	for (int i = 0; i < n; i++) {
+		while (1) {
+			pthread_mutex_lock(&q.jm);
+			int ql = q.qcount + q.inprogress;
+			pthread_mutex_unlock(&q.jm);
+			if (ql < q.maxq) break;
+			sleep_ms(5);
+		}
+		pthread_mutex_lock(&q.jm);
+		int slot = q.freelist[--q.nfree];
+		pthread_mutex_unlock(&q.jm);
+		memcpy(q.slab + q.roibytes * slot, (const char*)rois + q.roibytes * i, q.roibytes);
+		q.jobjob[slot] = jobs[i];
+		pthread_mutex_lock(&q.jm);
+		q.queue[(q.qhead + q.qcount) % q.qcap] = slot;
+		q.qcount++;
+		pthread_mutex_unlock(&q.jm);
+	}
+	while (1) {
+		pthread_mutex_lock(&q.rm);
+		int rc = q.nresults;
+		pthread_mutex_unlock(&q.rm);
+		if (rc == n) break;
+		sleep_ms(1);
+	}
+	double t1 = now_s();
+	pthread_mutex_lock(&q.jm);
+	q.quit = 1;
+	pthread_mutex_unlock(&q.jm);
+	for (int i = 0; i < nthreads; i++) pthread_join(th[i], 0);
+	for (int i = 0; i < n; i++) out[i] = q.results[n - 1 - i];
+	free(th); free(q.slab); free(q.jobjob); free(q.freelist); free(q.queue); free(q.results);
+	free(q.t.qi_window);
+	pthread_mutex_destroy(&q.jm);
+	pthread_mutex_destroy(&q.rm);
+	return t1 - t0;
+}
+
+/* ------------------------------------------------- image generation */
+
+/* ComputeBSplineWeights, cputrack/CubicBSpline.h:15-24 */
+void oq_bspline_weights(fl* w, fl t)
+{
+	fl t2 = t * t;
+	fl t3 = t2 * t;
+	fl omt = 1 - t;
+	w[0] = FC(1.0) / 6 * omt * omt * omt;
+	w[1] = FC(1.0) / 6 * (3 * t3 - 6 * t2 + 4);
+	w[2] = FC(1.0) / 6 * (-3 * t3 + 3 * t2 + 3 * t + 1);
+	w[3] = FC(1.0) / 6 * t3;
+}
+
+/* GenerateImageFromLUT, cputrack/utils.cpp:325-390; zlut is [planes][steps]. */
+void oq_generate_image_from_lut(fl* img, int w, int h, const float* zlut, int planes, int steps,
+                                fl minr, fl maxr, fl px, fl py, fl pz, int spline, int ovs)
+{
+	fl radialcov = steps / (maxr - minr);
+	fl* zi = (fl*)alloca(sizeof(fl) * steps);
+	if (spline) {
+		int iz = (int)pz;
+		if (iz > planes - 3) iz = planes - 3;
+		if (iz < 1) iz = 1;
+		fl wt[4];
+		fl fz = pz - iz;
+		oq_bspline_weights(wt, fz);
+		for (int r = 0; r < steps; r++) {
+			fl v = 0;
+			for (int i = 0; i < 4; i++) v += wt[i] * (fl)zlut[(i - 1 + iz) * steps + r];
+			zi[r] = v;
+		}
+	} else {
+		int iz = (int)pz;
+		if (iz < 0) for (int r = 0; r < steps; r++) zi[r] = zlut[r];
+		else if (iz >= planes - 1) for (int r = 0; r < steps; r++) zi[r] = zlut[(planes - 1) * steps + r];
+		else for (int r = 0; r < steps; r++) {
+			fl a = zlut[iz * steps + r], b = zlut[(iz + 1) * steps + r];
+			zi[r] = a + (b - a) * (pz - iz);
+		}
+	}
+	fl step = FC(1.0) / ovs;
+	for (int y = 0; y < h; y++)
+		for (int x = 0; x < w; x++) {
+			fl s = 0;
+			for (int ox = 0; ox < ovs; ox++)
+				for (int oy = 0; oy < ovs; oy++) {
+					fl X = x + (ox + FC(0.5)) * step - px - FC(0.5);
+					fl Y = y + (oy + FC(0.5)) * step - py - FC(0.5);
+					fl pr = FSQRT(X * X + Y * Y);
+					fl r = (pr - minr) * radialcov;
+					if (r > steps - 2) r = (fl)(steps - 2);
+					if (r < 0) r = 0;
+					int i = (int)r;
+					s += zi[i] + (zi[i + 1] - zi[i]) * (r - i);
+				}
+			img[y * w + x] = s / (ovs * ovs);
+		}
+}
+
+/* erf as the reference build binds it: a float argument resolves to the C
+ * library's erff (the A&S template in cputrack/utils.h:395-417 loses overload
+ * resolution to the non-template float overload). */
+fl oq_erf(fl x) { return FERF(x); }
+
+/* GenerateGaussianSpotImage, cputrack/utils.cpp:393-402 */
+void oq_gaussian_spot(fl* img, int w, int h, fl px, fl py, fl sigma, fl I0, fl bg)
+{
+	fl ed = FC(1.0) / FSQRT(2 * sigma * sigma);
+	for (int y = 0; y < h; y++)
+		for (int x = 0; x < w; x++) {
+			fl dx = FC(0.5) * oq_erf((x - px + FC(0.5)) * ed) - FC(0.5) * oq_erf((x - px - FC(0.5)) * ed);
+			fl dy = FC(0.5) * oq_erf((y - py + FC(0.5)) * ed) - FC(0.5) * oq_erf((y - py - FC(0.5)) * ed);
+			img[y * w + x] = bg + I0 * dx * dy;
+		}
+}
+
+/* CPUTracker::Compute2DGaussianMLE, cputrack/cpu_tracker.cpp:514-589:
+ * independent (diagonal) Newton updates of x, y, I0 and bg. out4 = x,y,I0,bg. */
+void oq_gauss2d_mle(const fl* img, int w, int h, fl mean, fl x0, fl y0, int iters, fl sigma, fl* out4)
+{
+	fl px = x0, py = y0;
+	fl I0 = mean * FC(0.5) * w * h;
+	fl bg = mean * FC(0.5);
+	const fl s2 = FC(1.0) / (FSQRT((fl)2) * sigma);
+	const fl s2pi = FC(1.0) / (FSQRT(2 * FC(3.14159265359)) * sigma);
+	const fl s2pi3 = FC(1.0) / (FSQRT(2 * FC(3.14159265359)) * sigma * sigma * sigma);
+	for (int it = 0; it < iters; it++) {
+		double dx = 0, dy = 0, dI = 0, dB = 0, ddx = 0, ddy = 0, ddI = 0, ddB = 0;
+		for (int y = 0; y < h; y++)
+			for (int x = 0; x < w; x++) {
+				fl Xe0 = (x - px + FC(0.5)) * s2;
+				fl Ye0 = (y - py + FC(0.5)) * s2;
+				fl Xe1 = (x - px - FC(0.5)) * s2;
+				fl Ye1 = (y - py - FC(0.5)) * s2;
+				fl DX = FC(0.5) * oq_erf(Xe0) - FC(0.5) * oq_erf(Xe1);
+				fl DY = FC(0.5) * oq_erf(Ye0) - FC(0.5) * oq_erf(Ye1);
+				fl mu = bg + I0 * DX * DY;
+				fl dmx = I0 * s2pi * (FEXP(-Xe1 * Xe1) - FEXP(-Xe0 * Xe0)) * DY;
+				fl dmy = I0 * s2pi * (FEXP(-Ye1 * Ye1) - FEXP(-Ye0 * Ye0)) * DX;
+				fl dmI = DX * DY;
+				fl dmB = 1;
+				fl smp = img[y * w + x];
+				fl f = smp / mu - 1;
+				dx += dmx * f;
+				dy += dmy * f;
+				dI += dmI * f;
+				dB += dmB * f;
+				fl d2x = I0 * s2pi3 * ((x - px - FC(0.5)) * FEXP(-Xe1 * Xe1) - (x - px + FC(0.5)) * FEXP(-Xe0 * Xe0)) * DY;
+				fl d2y = I0 * s2pi3 * ((y - py - FC(0.5)) * FEXP(-Ye1 * Ye1) - (y - py + FC(0.5)) * FEXP(-Ye0 * Ye0)) * DX;
+				ddx += d2x * f - dmx * dmx * smp / (mu * mu);
+				ddy += d2y * f - dmy * dmy * smp / (mu * mu);
+				ddI += -dmI * dmI * smp / (mu * mu);
+				ddB += -smp / (mu * mu);
+			}
+		px = (fl)(px - dx / ddx);
+		py = (fl)(py - dy / ddy);
+		I0 = (fl)(I0 - dI / ddI);
+		bg = (fl)(bg - dB / ddB);
+	}
+	out4[0] = px; out4[1] = py; out4[2] = I0; out4[3] = bg;
+}

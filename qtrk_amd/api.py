@@ -1,0 +1,438 @@
+"""ctypes binding of libqtrk.so, mirroring the reference's C# binding.
+
+Struct layouts follow QTrkDotNet/QTrkDLL.cs:38-215 (packed, sequential) and
+the function table follows QTrkDotNet/QTrkDLL.cs:240-424; ``QTrkInstance``
+mirrors QTrkDotNet/QTrkInstance.cs:31-189.  Every call checks the library's
+last-error slot and raises ``QTrkError`` when a call failed, so a failing GPU
+path is never silent.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqtrk.so")
+
+# LocalizeModeEnum (cputrack/qtrk_c_api.h:5-20)
+LT_OnlyCOM = 0
+LT_XCor1D = 1
+LT_QI = 2
+LT_Gaussian2D = 4
+LT_ZLUTAlign = 8
+LT_LocalizeZ = 16
+LT_NormalizeProfile = 64
+LT_ClearFirstFourPixels = 128
+LT_FourierLUT = 256
+LT_LocalizeZWeighted = 512
+
+# QTRK_PixelDataType (cputrack/qtrk_c_api.h:24-29)
+QTrkU8, QTrkU16, QTrkFloat = 0, 1, 2
+PDT_OF_DTYPE = {np.dtype(np.uint8): QTrkU8, np.dtype(np.uint16): QTrkU16, np.dtype(np.float32): QTrkFloat}
+
+BUILDLUT_IMAGELUT, BUILDLUT_FOURIER, BUILDLUT_NORMALIZE, BUILDLUT_BIASCORRECT = 1, 2, 4, 8
+
+
+class Vector2f(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float)]
+
+
+class Vector3f(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class LocalizationJob(C.Structure):
+    _pack_ = 1
+    _fields_ = [("frame", C.c_uint), ("timestamp", C.c_uint), ("zlutIndex", C.c_int), ("initialPos", Vector3f)]
+
+
+class LocalizationResult(C.Structure):
+    _pack_ = 1
+    _fields_ = [("job", LocalizationJob), ("pos", Vector3f), ("firstGuess", Vector2f),
+                ("error", C.c_uint), ("imageMean", C.c_float)]
+
+
+_SETTINGS_FIELDS = [
+    ("width", C.c_int), ("height", C.c_int), ("numThreads", C.c_int), ("cuda_device", C.c_int),
+    ("com_bgcorrection", C.c_float),
+    ("zlut_minradius", C.c_float), ("zlut_radial_coverage", C.c_float),
+    ("zlut_angular_coverage", C.c_float), ("zlut_roi_coverage", C.c_float),
+    ("qi_iterations", C.c_int), ("qi_minradius", C.c_float), ("qi_radial_coverage", C.c_float),
+    ("qi_angular_coverage", C.c_float), ("qi_roi_coverage", C.c_float), ("qi_angstep_factor", C.c_float),
+    ("xc1_profileLength", C.c_int), ("xc1_profileWidth", C.c_int), ("xc1_iterations", C.c_int),
+    ("gauss2D_iterations", C.c_int), ("gauss2D_sigma", C.c_float), ("downsample", C.c_int),
+]
+
+
+class QTrkSettings(C.Structure):
+    _pack_ = 1
+    _fields_ = _SETTINGS_FIELDS
+
+
+class QTrkComputedConfig(C.Structure):
+    _pack_ = 1
+    _fields_ = _SETTINGS_FIELDS + [
+        ("zlut_radialsteps", C.c_int), ("zlut_angularsteps", C.c_int), ("zlut_maxradius", C.c_float),
+        ("qi_radialsteps", C.c_int), ("qi_angstepspq", C.c_int), ("qi_maxradius", C.c_float),
+    ]
+
+
+class ROIPosition(C.Structure):
+    _pack_ = 1
+    _fields_ = [("x", C.c_int), ("y", C.c_int)]
+
+
+class ImageData(C.Structure):
+    _pack_ = 4
+    _fields_ = [("data", C.POINTER(C.c_float)), ("w", C.c_int), ("h", C.c_int)]
+
+
+# numpy views of the packed records
+JOB_DTYPE = np.dtype([("frame", "<u4"), ("timestamp", "<u4"), ("zlutIndex", "<i4"),
+                      ("ix", "<f4"), ("iy", "<f4"), ("iz", "<f4")])
+RESULT_DTYPE = np.dtype([("frame", "<u4"), ("timestamp", "<u4"), ("zlutIndex", "<i4"),
+                         ("ix", "<f4"), ("iy", "<f4"), ("iz", "<f4"),
+                         ("x", "<f4"), ("y", "<f4"), ("z", "<f4"),
+                         ("comx", "<f4"), ("comy", "<f4"), ("error", "<u4"), ("imageMean", "<f4")])
+assert JOB_DTYPE.itemsize == 24 and RESULT_DTYPE.itemsize == 52
+
+
+class QTrkError(RuntimeError):
+    pass
+
+
+_P = C.c_void_p
+_FP = C.POINTER(C.c_float)
+_IP = C.POINTER(C.c_int)
+
+# name: (restype, argtypes) -- QTrkDotNet/QTrkDLL.cs:240-367 plus include/qtrk_amd.h
+SIGNATURES = {
+    "TestDLLCallConv": (None, [C.c_int]),
+    "QTrkGetDefaultConfig": (None, [C.POINTER(QTrkSettings)]),
+    "QTrkGetComputedConfig": (None, [C.POINTER(QTrkSettings), C.POINTER(QTrkComputedConfig)]),
+    "QTrkCreateInstance": (_P, [C.POINTER(QTrkSettings)]),
+    "QTrkFreeInstance": (None, [_P]),
+    "QTrkSetLocalizationMode": (None, [_P, C.c_int]),
+    "QTrkSetPixelCalibrationImages": (None, [_P, _P, _P]),
+    "QTrkSetPixelCalibrationFactors": (None, [C.c_float, C.c_float]),
+    "QTrkScheduleLocalization": (None, [_P, _P, C.c_int, C.c_int, C.POINTER(LocalizationJob)]),
+    "QTrkClearResults": (None, [_P]),
+    "QTrkFlush": (None, [_P]),
+    "QTrkScheduleFrame": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int,
+                                    C.POINTER(LocalizationJob)]),
+    "QTrkEnableRadialZLUTCompareProfile": (None, [C.c_bool]),
+    "QTrkGetRadialZLUTCompareProfile": (None, [_P]),
+    "QTrkSetRadialZLUT": (None, [_P, _P, C.c_int, C.c_int]),
+    "QTrkGetRadialZLUT": (None, [_P, _P]),
+    "QTrkGetRadialZLUTSize": (None, [_P, _IP, _IP, _IP]),
+    "QTrkSetRadialWeights": (None, [_P, _P]),
+    "QTrkBeginLUT": (None, [_P, C.c_uint]),
+    "QTrkBuildLUT": (None, [_P, _P, C.c_int, C.c_int, C.c_int, _P]),
+    "QTrkBuildLUTFromFrame": (None, [_P, C.POINTER(ImageData), C.c_int, _P, C.c_int]),
+    "QTrkFinalizeLUT": (None, [_P]),
+    "QTrkGetResultCount": (C.c_int, [_P]),
+    "QTrkFetchResults": (C.c_int, [_P, _P, C.c_int]),
+    "QTrkGetQueueLength": (C.c_int, [_P, _IP]),
+    "QTrkIsIdle": (C.c_bool, [_P]),
+    "QTrkGetProfileReport": (None, [_P, C.c_char_p, C.c_int]),
+    "QTrkGetWarnings": (None, [_P, C.c_char_p, C.c_int]),
+    "QTrkFreeROIPositions": (None, [_P]),
+    "QTrkFindBeads": (_P, [C.POINTER(ImageData), C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _IP,
+                           C.POINTER(ImageData)]),
+    "ComputeRadialProfile": (None, [_FP, C.c_int, C.c_int, C.c_float, C.c_float, Vector2f,
+                                    C.POINTER(ImageData), C.c_float, C.c_bool]),
+    "NormalizeRadialProfile": (None, [_FP, C.c_int]),
+    "NormalizeImage": (None, [C.POINTER(ImageData)]),
+    "GenerateImageFromLUT": (None, [C.POINTER(ImageData), C.POINTER(ImageData), C.c_float, C.c_float,
+                                    Vector3f, C.c_bool, C.c_int]),
+    "ApplyPoissonNoise": (None, [C.POINTER(ImageData), C.c_float, C.c_float]),
+    "ApplyGaussianNoise": (None, [C.POINTER(ImageData), C.c_float]),
+    "GenerateGaussianSpotImage": (None, [C.POINTER(ImageData), Vector2f, C.c_float, C.c_float, C.c_float]),
+    "GetPreciseTime": (C.c_double, []),
+    # include/qtrk_amd.h
+    "QTrkGetLastError": (C.c_char_p, []),
+    "QTrkClearLastError": (None, []),
+    "QTrkGetDeviceInfo": (C.c_int, [C.c_char_p, C.c_int]),
+    "QTrkBatchCreate": (_P, [_P, C.c_int, C.c_int]),
+    "QTrkBatchFree": (None, [_P]),
+    "QTrkBatchUpload": (C.c_int, [_P, _P, _P, C.c_int]),
+    "QTrkBatchDeviceROIs": (_P, [_P]),
+    "QTrkBatchDeviceJobs": (_P, [_P]),
+    "QTrkBatchDeviceResults": (_P, [_P]),
+    "QTrkBatchRun": (C.c_int, [_P, C.c_int, C.c_int]),
+    "QTrkBatchSync": (C.c_float, [_P]),
+    "QTrkBatchDownload": (C.c_int, [_P, _P, C.c_int]),
+    "QTrkGenerateImagesFromLUT": (None, [_FP, C.c_int, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.c_float, _FP, _IP, C.c_bool, C.c_int]),
+}
+
+_lib = None
+
+
+def lib(path=None):
+    """Loads libqtrk.so (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or os.environ.get("QTRK_LIB", LIB_PATH)
+    if not os.path.exists(p):
+        raise QTrkError(f"{p} not found: run __graft_entry__.build() (hipcc) first")
+    L = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    if path is None:
+        _lib = L
+    return L
+
+
+def check():
+    """Raises QTrkError if the last library call on this thread failed."""
+    L = lib()
+    msg = L.QTrkGetLastError()
+    if msg:
+        L.QTrkClearLastError()
+        raise QTrkError(msg.decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_FP)
+
+
+def default_settings(**kw):
+    s = QTrkSettings()
+    lib().QTrkGetDefaultConfig(C.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+def computed_config(settings):
+    cc = QTrkComputedConfig()
+    lib().QTrkGetComputedConfig(C.byref(settings), C.byref(cc))
+    return cc
+
+
+def make_jobs(n, frame=None, zlut_index=None, timestamp=0):
+    j = np.zeros(n, dtype=JOB_DTYPE)
+    j["frame"] = np.arange(n) if frame is None else frame
+    j["timestamp"] = timestamp
+    j["zlutIndex"] = 0 if zlut_index is None else zlut_index
+    return j
+
+
+class QTrkInstance:
+    """A tracker instance (QTrkDotNet/QTrkInstance.cs:31-189)."""
+
+    def __init__(self, settings):
+        L = lib()
+        L.QTrkClearLastError()
+        self.settings = settings
+        self.h = L.QTrkCreateInstance(C.byref(settings))
+        check()
+        if not self.h:
+            raise QTrkError("QTrkCreateInstance returned NULL")
+        self.cfg = computed_config(settings)
+
+    def _call(self, name, *args):
+        r = getattr(lib(), name)(self.h, *args)
+        check()
+        return r
+
+    def free(self):
+        if self.h:
+            lib().QTrkFreeInstance(self.h)
+            self.h = None
+            check()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
+
+    def set_localization_mode(self, mode):
+        self._call("QTrkSetLocalizationMode", int(mode))
+
+    def set_radial_zlut(self, lut):
+        """lut: float32 [count][planes][zlut_radialsteps], or (count, planes) to allocate a zeroed one."""
+        if isinstance(lut, tuple):
+            self._call("QTrkSetRadialZLUT", None, int(lut[0]), int(lut[1]))
+            return
+        lut = np.ascontiguousarray(lut, dtype=np.float32)
+        assert lut.ndim == 3 and lut.shape[2] == self.cfg.zlut_radialsteps, lut.shape
+        self._call("QTrkSetRadialZLUT", _ptr(lut), lut.shape[0], lut.shape[1])
+
+    def get_radial_zlut_size(self):
+        c, p, r = C.c_int(), C.c_int(), C.c_int()
+        self._call("QTrkGetRadialZLUTSize", C.byref(c), C.byref(p), C.byref(r))
+        return c.value, p.value, r.value
+
+    def get_radial_zlut(self):
+        c, p, r = self.get_radial_zlut_size()
+        out = np.zeros((c, p, r), np.float32)
+        if out.size:
+            self._call("QTrkGetRadialZLUT", _ptr(out))
+        return out
+
+    def set_radial_weights(self, w):
+        if w is None:
+            self._call("QTrkSetRadialWeights", None)
+        else:
+            w = np.ascontiguousarray(w, np.float32)
+            self._w = w
+            self._call("QTrkSetRadialWeights", _ptr(w))
+
+    def set_pixel_calibration_images(self, offset, gain):
+        o = None if offset is None else np.ascontiguousarray(offset, np.float32)
+        g = None if gain is None else np.ascontiguousarray(gain, np.float32)
+        self._call("QTrkSetPixelCalibrationImages", None if o is None else _ptr(o), None if g is None else _ptr(g))
+
+    def schedule_localization(self, roi, job):
+        """roi: 2-D uint8/uint16/float32 array (any row pitch); job: LocalizationJob or JOB_DTYPE record."""
+        roi = np.asarray(roi)
+        pdt = PDT_OF_DTYPE[roi.dtype]
+        if roi.strides[1] != roi.itemsize:
+            roi = np.ascontiguousarray(roi)
+        if not isinstance(job, LocalizationJob):
+            j = LocalizationJob()
+            C.memmove(C.byref(j), np.asarray(job, JOB_DTYPE).tobytes(), 24)
+            job = j
+        self._call("QTrkScheduleLocalization", _ptr(roi), roi.strides[0], pdt, C.byref(job))
+
+    def schedule_frame(self, frame, positions, job):
+        frame = np.asarray(frame)
+        pdt = PDT_OF_DTYPE[frame.dtype]
+        pos = np.ascontiguousarray(positions, dtype=np.int32)
+        j = LocalizationJob()
+        C.memmove(C.byref(j), np.asarray(job, JOB_DTYPE).tobytes(), 24)
+        return self._call("QTrkScheduleFrame", _ptr(frame), frame.strides[0], frame.shape[1], frame.shape[0],
+                          _ptr(pos), len(pos), pdt, C.byref(j))
+
+    def flush(self):
+        self._call("QTrkFlush")
+
+    def clear_results(self):
+        self._call("QTrkClearResults")
+
+    def result_count(self):
+        return self._call("QTrkGetResultCount")
+
+    def fetch_results(self, max_results=None):
+        n = self.result_count() if max_results is None else max_results
+        out = np.zeros(max(n, 0), RESULT_DTYPE)
+        got = self._call("QTrkFetchResults", _ptr(out), n) if n > 0 else 0
+        return out[:got]
+
+    def queue_length(self):
+        m = C.c_int()
+        n = self._call("QTrkGetQueueLength", C.byref(m))
+        return n, m.value
+
+    def is_idle(self):
+        return bool(self._call("QTrkIsIdle"))
+
+    def begin_lut(self, flags):
+        self._call("QTrkBeginLUT", int(flags))
+
+    def build_lut(self, images, plane, known_pos=None):
+        """images: [count][h][w] stack, image i -> bead i."""
+        images = np.ascontiguousarray(images)
+        pdt = PDT_OF_DTYPE[images.dtype]
+        kp = None
+        if known_pos is not None:
+            kp = np.ascontiguousarray(known_pos, np.float32)
+        self._call("QTrkBuildLUT", _ptr(images), images.strides[1], pdt, int(plane),
+                   None if kp is None else _ptr(kp))
+
+    def finalize_lut(self):
+        self._call("QTrkFinalizeLUT")
+
+    def profile_report(self):
+        buf = C.create_string_buffer(4096)
+        self._call("QTrkGetProfileReport", buf, 4096)
+        return buf.value.decode()
+
+    def warnings(self):
+        buf = C.create_string_buffer(8192)
+        self._call("QTrkGetWarnings", buf, 8192)
+        return buf.value.decode()
+
+    def wait_for(self, n, timeout=120.0):
+        """Flush, then poll until n results exist (cputrack-test/SharedTests.h:300-314)."""
+        import time
+        self.flush()
+        t0 = time.time()
+        while self.result_count() < n:
+            if time.time() - t0 > timeout:
+                raise QTrkError(f"timeout waiting for {n} results")
+            time.sleep(0.0005)
+
+    def localize_all(self, rois, jobs):
+        """Schedules every ROI, waits, and returns results sorted back into job order."""
+        for i in range(len(rois)):
+            self.schedule_localization(rois[i], jobs[i])
+        self.wait_for(len(rois))
+        res = self.fetch_results(len(rois))
+        key = {(int(j["frame"]), int(j["zlutIndex"])): i for i, j in enumerate(jobs)}
+        order = np.array([key[(int(r["frame"]), int(r["zlutIndex"]))] for r in res])
+        out = np.zeros(len(rois), RESULT_DTYPE)
+        out[order] = res
+        return out
+
+
+class DeviceBatch:
+    """ROIs resident in GPU memory, localized with a tracker's settings (include/qtrk_amd.h)."""
+
+    def __init__(self, trk, max_rois, pdt):
+        self.trk = trk
+        self.h = lib().QTrkBatchCreate(trk.h, int(max_rois), int(pdt))
+        check()
+        self.max = max_rois
+
+    def upload(self, rois, jobs):
+        rois = np.ascontiguousarray(rois)
+        jobs = np.ascontiguousarray(jobs, JOB_DTYPE)
+        if lib().QTrkBatchUpload(self.h, _ptr(rois), _ptr(jobs), len(jobs)) != 0:
+            check()
+
+    def run(self, n, repeats=1):
+        if lib().QTrkBatchRun(self.h, int(n), int(repeats)) != 0:
+            check()
+
+    def sync(self):
+        ms = lib().QTrkBatchSync(self.h)
+        if ms < 0:
+            check()
+        return ms
+
+    def download(self, n):
+        out = np.zeros(n, RESULT_DTYPE)
+        if lib().QTrkBatchDownload(self.h, _ptr(out), int(n)) != 0:
+            check()
+        return out
+
+    def free(self):
+        if self.h:
+            lib().QTrkBatchFree(self.h)
+            self.h = None
+
+
+def generate_images_from_lut(luts, pos, lut_index, width, height, minr, maxr, spline=True, ovs=4):
+    """Batch GenerateImageFromLUT on the host (include/qtrk_amd.h)."""
+    luts = np.ascontiguousarray(luts, np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    li = np.ascontiguousarray(lut_index, np.int32)
+    n = len(pos)
+    out = np.zeros((n, height, width), np.float32)
+    lib().QTrkGenerateImagesFromLUT(_fptr(out), n, width, height, _fptr(luts), luts.shape[0], luts.shape[1],
+                                    luts.shape[2], minr, maxr, _fptr(pos), li.ctypes.data_as(_IP), spline, ovs)
+    check()
+    return out

@@ -1,0 +1,668 @@
+// hip_tracker.cpp -- host runtime of the MI355X tracker.
+#include "hip_tracker.hpp"
+#include "errors.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace qtrk {
+
+void DeviceBuffer::ensure(size_t n)
+{
+	if (n <= bytes && p) return;
+	release();
+	if (n == 0) return;
+	HIP_CHECK(hipMalloc(&p, n));
+	bytes = n;
+}
+void DeviceBuffer::release()
+{
+	if (p) hipFree(p);
+	p = nullptr;
+	bytes = 0;
+}
+void PinnedBuffer::ensure(size_t n)
+{
+	if (n <= bytes && p) return;
+	release();
+	if (n == 0) return;
+	HIP_CHECK(hipHostMalloc(&p, n, hipHostMallocDefault));
+	bytes = n;
+}
+void PinnedBuffer::release()
+{
+	if (p) hipHostFree(p);
+	p = nullptr;
+	bytes = 0;
+}
+
+static void upload(DeviceBuffer& b, const void* src, size_t bytes)
+{
+	b.ensure(bytes);
+	if (bytes) HIP_CHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+// kissfft factorisation of n (cputrack/kissfft.h:29-46): 4s first, then 2s, ...
+static int fft_factor(int n, int* radix, int* rem)
+{
+	int m = n, f = 4, ns = 0;
+	do {
+		while (m % f) {
+			switch (f) { case 4: f = 2; break; case 2: f = 3; break; default: f += 2; break; }
+			if (f * f > m) f = m;
+		}
+		m /= f;
+		if (ns >= kMaxFFTStages) fail("FFT length has too many factors");
+		radix[ns] = f;
+		rem[ns] = m;
+		ns++;
+	} while (m > 1);
+	return ns;
+}
+
+// Input index that lands in each output slot at kissfft's leaves.
+static void fft_leaves(const int* radix, const int* rem, int stage, int out, int in, int fstride, std::vector<int>& perm)
+{
+	for (int q = 0; q < radix[stage]; q++) {
+		if (rem[stage] == 1) perm[out + q] = in + q * fstride;
+		else fft_leaves(radix, rem, stage + 1, out + q * rem[stage], in + q * fstride, fstride * radix[stage], perm);
+	}
+}
+
+QueuedHIPTracker::QueuedHIPTracker(const QTrkComputedConfig& cc)
+{
+	cfg = cc;
+	int ndev = 0;
+	HIP_CHECK(hipGetDeviceCount(&ndev));
+	if (ndev <= 0) fail("no HIP device");
+	dev = cfg.cuda_device >= 0 ? cfg.cuda_device : 0;
+	if (dev >= ndev) fail("cuda_device out of range");
+	HIP_CHECK(hipSetDevice(dev));
+	hipDeviceProp_t prop;
+	HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+	lds_limit = prop.sharedMemPerBlock;
+	if (cfg.width < 2 || cfg.height < 2) fail("ROI width and height must be >= 2");
+	if (cfg.downsample != 0) fail("downsample != 0 is not supported");
+	HIP_CHECK(hipStreamCreateWithFlags(&main_stream, hipStreamNonBlocking));
+	BuildTables();
+	AllocSlots();
+}
+
+QueuedHIPTracker::~QueuedHIPTracker()
+{
+	hipSetDevice(dev);
+	{
+		std::unique_lock<std::mutex> lk(mtx);
+		for (Slot* s : slots)
+			if (s->state == Slot::Launched) hipEventSynchronize(s->ev_done);
+	}
+	FreeSlots();
+	if (main_stream) hipStreamDestroy(main_stream);
+}
+
+void QueuedHIPTracker::BuildTables()
+{
+	const int A = cfg.qi_angstepspq, nr = cfg.qi_radialsteps, iters = cfg.qi_iterations;
+	if (A < 1) fail("qi_angstepspq must be >= 1");
+	if (nr < 1) fail("qi_radialsteps must be >= 1");
+	if (cfg.zlut_radialsteps < 0 || cfg.zlut_angularsteps < 0) fail("invalid Z-LUT sampling sizes");
+	if (iters < 0) fail("qi_iterations must be >= 0");
+
+	// QI directions (cputrack/cpu_tracker.cpp:294-300): 0.5*pi*(j+0.5)/A with a
+	// double 0.5, rounded to float once.
+	std::vector<float> qd(2 * (size_t)A);
+	for (int j = 0; j < A; j++) {
+		float ang = (float)(0.5 * (double)3.141593f * (double)(j + 0.5f) / (double)(float)A);
+		qd[2 * j] = cosf(ang);
+		qd[2 * j + 1] = sinf(ang);
+	}
+	upload(d_qi_dirs, qd.data(), qd.size() * 4);
+
+	// Angular samples and table stride per iteration (cputrack/cpu_tracker.cpp:314,
+	// 356, 642-657): the float count is truncated, clamped to >= 4, and the
+	// stride is trunc((float)A / count).
+	std::vector<int> steps(std::max(iters, 1)), stride(std::max(iters, 1));
+	float angsteps = A / powf(cfg.qi_angstep_factor, (float)iters);
+	for (int k = 0; k < iters; k++) {
+		int s = (int)angsteps;
+		if (s < MIN_RADPROFILE_SMP_COUNT) s = MIN_RADPROFILE_SMP_COUNT;
+		float angstepf = (float)A / s;
+		steps[k] = s;
+		stride[k] = (int)angstepf;
+		angsteps *= cfg.qi_angstep_factor;
+	}
+	upload(d_qi_steps, steps.data(), steps.size() * 4);
+	upload(d_qi_stride, stride.data(), stride.size() * 4);
+
+	// Radial bin window, cputrack/utils.cpp:665-677
+	std::vector<float> wnd(nr);
+	for (int i = 0; i < nr; i++) {
+		float x = i / (float)nr;
+		float omx = 1 - x;
+		float fall = 1.0f - expf(-(omx * omx) / 0.05f);
+		float rise = 1.0f - expf(-(x * x) / 0.01f);
+		wnd[i] = sqrtf(fall * rise * x * 2);
+	}
+	upload(d_qi_window, wnd.data(), wnd.size() * 4);
+
+	// FFT of length 2*nr: stages, leaf order and twiddles (cputrack/kissfft.h:11-16)
+	fft_n = 2 * nr;
+	int rem[kMaxFFTStages];
+	fft_nst = fft_factor(fft_n, fft_radix, rem);
+	int fs = 1;
+	for (int s = 0; s < fft_nst; s++) {
+		if (fft_radix[s] != 2 && fft_radix[s] != 4) fail("QI FFT length must be a power of two");
+		fft_m[s] = rem[s];
+		fft_fs[s] = fs;
+		fs *= fft_radix[s];
+	}
+	std::vector<int> perm(fft_n);
+	fft_leaves(fft_radix, rem, 0, 0, 0, 1, perm);
+	upload(d_perm, perm.data(), perm.size() * 4);
+	std::vector<float> tf(2 * (size_t)fft_n), ti(2 * (size_t)fft_n);
+	const float phf = -2 * acosf(-1.0f) / fft_n, phi = 2 * acosf(-1.0f) / fft_n;
+	for (int i = 0; i < fft_n; i++) {
+		tf[2 * i] = cosf(i * phf);
+		tf[2 * i + 1] = sinf(i * phf);
+		ti[2 * i] = cosf(i * phi);
+		ti[2 * i + 1] = sinf(i * phi);
+	}
+	upload(d_tw_fwd, tf.data(), tf.size() * 4);
+	upload(d_tw_inv, ti.data(), ti.size() * 4);
+
+	// Radial-profile directions, cputrack/utils.cpp:278-282
+	const int ZA = cfg.zlut_angularsteps;
+	std::vector<float> zd(2 * (size_t)std::max(ZA, 1));
+	for (int j = 0; j < ZA; j++) {
+		float ang = 2 * 3.141593f * j / (float)ZA;
+		zd[2 * j] = cosf(ang);
+		zd[2 * j + 1] = sinf(ang);
+	}
+	upload(d_z_dirs, zd.data(), zd.size() * 4);
+
+	LocalizeParams P{};
+	FillParams(P, 2, 1, nullptr, nullptr, 0, nullptr, nullptr);
+	P.zplanes = 0;
+	size_t need = localize_lds_bytes(P);
+	if (need + 8 * 1024 > lds_limit)
+		fail("ROI too large for the per-workgroup LDS (" + std::to_string(need) + " bytes needed)");
+}
+
+void QueuedHIPTracker::AllocSlots()
+{
+	for (int i = 0; i < numSlots; i++) {
+		Slot* s = new Slot();
+		HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+		HIP_CHECK(hipEventCreate(&s->ev_start));
+		HIP_CHECK(hipEventCreate(&s->ev_k0));
+		HIP_CHECK(hipEventCreate(&s->ev_k1));
+		HIP_CHECK(hipEventCreate(&s->ev_done));
+		size_t rb = (size_t)cfg.width * cfg.height * 2;
+		s->h_rois.ensure(rb * batchSize);
+		s->h_jobs.ensure(sizeof(DevJob) * batchSize);
+		s->h_results.ensure(sizeof(DevResult) * batchSize);
+		s->d_rois.ensure(rb * batchSize);
+		s->d_jobs.ensure(sizeof(DevJob) * batchSize);
+		s->d_results.ensure(sizeof(DevResult) * batchSize);
+		slots.push_back(s);
+	}
+}
+
+void QueuedHIPTracker::FreeSlots()
+{
+	for (Slot* s : slots) {
+		if (s->stream) hipStreamDestroy(s->stream);
+		hipEventDestroy(s->ev_start);
+		hipEventDestroy(s->ev_k0);
+		hipEventDestroy(s->ev_k1);
+		hipEventDestroy(s->ev_done);
+		delete s;
+	}
+	slots.clear();
+	filling = nullptr;
+}
+
+void QueuedHIPTracker::Warn(const std::string& w)
+{
+	if (warnings.size() < 8192) warnings += w + "\n";
+}
+
+void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void* src, const int64_t* roi_offset,
+                                  int pitch, const DevJob* jobs, DevResult* res)
+{
+	P = LocalizeParams{};
+	P.src = (const uint8_t*)src;
+	P.roi_offset = roi_offset;
+	P.pitch = pitch;
+	P.pdt = pdt;
+	P.n = n;
+	P.jobs = jobs;
+	P.results = res;
+	P.width = cfg.width;
+	P.height = cfg.height;
+	P.mode = localizeMode;
+	P.com_bgcorrection = cfg.com_bgcorrection;
+	P.qi_iters = cfg.qi_iterations;
+	P.qi_nr = cfg.qi_radialsteps;
+	P.qi_A = cfg.qi_angstepspq;
+	P.qi_minr = cfg.qi_minradius;
+	P.qi_maxr = cfg.qi_maxradius;
+	P.qi_iter_steps = d_qi_steps.as<int>();
+	P.qi_iter_stride = d_qi_stride.as<int>();
+	P.qi_dirs = d_qi_dirs.as<float2>();
+	P.qi_window = d_qi_window.as<float>();
+	P.fft_n = fft_n;
+	P.fft_nst = fft_nst;
+	for (int s = 0; s < kMaxFFTStages; s++) {
+		P.fft_radix[s] = fft_radix[s];
+		P.fft_m[s] = fft_m[s];
+		P.fft_fs[s] = fft_fs[s];
+	}
+	P.fft_perm = d_perm.as<int>();
+	P.tw_fwd = d_tw_fwd.as<float2>();
+	P.tw_inv = d_tw_inv.as<float2>();
+	P.z_nb = cfg.zlut_radialsteps;
+	P.z_A = cfg.zlut_angularsteps;
+	P.z_minr = cfg.zlut_minradius;
+	P.z_maxr = cfg.zlut_maxradius;
+	P.z_dirs = d_z_dirs.as<float2>();
+	P.zluts = zcount > 0 ? d_zlut.as<float>() : nullptr;
+	P.zcount = zcount;
+	P.zplanes = zplanes;
+	P.zcmp = has_zcmp ? d_zcmp.as<float>() : nullptr;
+	P.cmp_out = cmp_enabled && zcount > 0 ? d_cmp.as<float>() : nullptr;
+	P.calib_offset = has_off ? d_calib_off.as<float>() : nullptr;
+	P.calib_gain = has_gain ? d_calib_gain.as<float>() : nullptr;
+}
+
+// ------------------------------------------------------------- queueing
+
+QueuedHIPTracker::Slot* QueuedHIPTracker::AcquireSlot(std::unique_lock<std::mutex>& lk)
+{
+	while (true) {
+		Harvest();
+		for (Slot* s : slots)
+			if (s->state == Slot::Free) {
+				s->state = Slot::Filling;
+				s->n = 0;
+				return s;
+			}
+		// Queue full: block like the reference (cputrack/QueuedCPUTracker.cpp:437-440)
+		// until the oldest launched batch completes.
+		Slot* oldest = nullptr;
+		for (Slot* s : slots)
+			if (s->state == Slot::Launched && (!oldest || s->seq < oldest->seq)) oldest = s;
+		if (!oldest) fail("no batch slot available");
+		hipEvent_t ev = oldest->ev_done;
+		lk.unlock();
+		hipError_t e = hipEventSynchronize(ev);
+		lk.lock();
+		HIP_CHECK(e);
+	}
+}
+
+void QueuedHIPTracker::ScheduleLocalization(void* data, int pitch, QTRK_PixelDataType pdt, const LocalizationJob* jobInfo)
+{
+	if ((int)pdt < 0 || (int)pdt > 2) fail("invalid pixel data type");
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	if (filling && filling->n > 0 && filling->pdt != (int)pdt) {
+		Launch(filling);
+		filling = nullptr;
+	}
+	if (!filling) filling = AcquireSlot(lk);
+	Slot* s = filling;
+	s->pdt = (int)pdt;
+	const size_t row = (size_t)cfg.width * PDT_BytesPerPixel(pdt);
+	const size_t rb = row * cfg.height;
+	if (s->h_rois.bytes < rb * batchSize) {
+		s->h_rois.ensure(rb * batchSize);
+		s->d_rois.ensure(rb * batchSize);
+	}
+	uint8_t* dst = s->h_rois.as<uint8_t>() + rb * s->n;
+	const uint8_t* srcp = (const uint8_t*)data;
+	for (int y = 0; y < cfg.height; y++) memcpy(dst + row * y, srcp + (size_t)pitch * y, row);
+	memcpy(s->h_jobs.as<DevJob>() + s->n, jobInfo, sizeof(DevJob));
+	s->n++;
+	if (s->n >= batchSize) {
+		Launch(s);
+		filling = nullptr;
+	}
+}
+
+void QueuedHIPTracker::Launch(Slot* s)
+{
+	const size_t rb = (size_t)cfg.width * cfg.height * PDT_BytesPerPixel((QTRK_PixelDataType)s->pdt);
+	HIP_CHECK(hipEventRecord(s->ev_start, s->stream));
+	HIP_CHECK(hipMemcpyAsync(s->d_rois.p, s->h_rois.p, rb * s->n, hipMemcpyHostToDevice, s->stream));
+	HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
+	LocalizeParams P;
+	FillParams(P, s->pdt, s->n, s->d_rois.p, nullptr, cfg.width * PDT_BytesPerPixel((QTRK_PixelDataType)s->pdt),
+	           s->d_jobs.as<DevJob>(), s->d_results.as<DevResult>());
+	HIP_CHECK(hipEventRecord(s->ev_k0, s->stream));
+	HIP_CHECK(launch_localize(P, s->stream));
+	HIP_CHECK(hipEventRecord(s->ev_k1, s->stream));
+	HIP_CHECK(hipMemcpyAsync(s->h_results.p, s->d_results.p, sizeof(DevResult) * s->n, hipMemcpyDeviceToHost, s->stream));
+	HIP_CHECK(hipEventRecord(s->ev_done, s->stream));
+	s->seq = ++seqCounter;
+	s->state = Slot::Launched;
+}
+
+void QueuedHIPTracker::Harvest()
+{
+	std::vector<Slot*> done;
+	for (Slot* s : slots)
+		if (s->state == Slot::Launched && hipEventQuery(s->ev_done) == hipSuccess) done.push_back(s);
+	std::sort(done.begin(), done.end(), [](Slot* a, Slot* b) { return a->seq < b->seq; });
+	for (Slot* s : done) {
+		float kms = 0, all = 0;
+		hipEventElapsedTime(&kms, s->ev_k0, s->ev_k1);
+		hipEventElapsedTime(&all, s->ev_start, s->ev_done);
+		t_kernel_ms += kms;
+		t_copy_ms += all - kms;
+		batchesDone++;
+		roisDone += s->n;
+		const DevResult* r = s->h_results.as<DevResult>();
+		const bool bias = HasZLUTBiasCorrection() && (localizeMode & LT_LocalizeZ);
+		{
+			std::lock_guard<std::mutex> g(res_mtx);
+			for (int i = 0; i < s->n; i++) {
+				LocalizationResult lr;
+				memcpy(&lr, &r[i], sizeof(lr));
+				if (bias) lr.pos.z = ZLUTBiasCorrection(lr.pos.z, zplanes, lr.job.zlutIndex);
+				results.push_back(lr);
+			}
+		}
+		s->n = 0;
+		s->state = Slot::Free;
+	}
+}
+
+void QueuedHIPTracker::WaitAllLocked(std::unique_lock<std::mutex>& lk)
+{
+	while (true) {
+		Harvest();
+		hipEvent_t ev = nullptr;
+		for (Slot* s : slots)
+			if (s->state == Slot::Launched) { ev = s->ev_done; break; }
+		if (!ev) return;
+		lk.unlock();
+		hipError_t e = hipEventSynchronize(ev);
+		lk.lock();
+		HIP_CHECK(e);
+	}
+}
+
+void QueuedHIPTracker::WaitLaunched()
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	WaitAllLocked(lk);
+}
+
+int QueuedHIPTracker::QueueLengthLocked() const
+{
+	int q = 0;
+	for (const Slot* s : slots)
+		if (s->state != Slot::Free) q += s->n;
+	return q;
+}
+
+void QueuedHIPTracker::Flush()
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	if (filling && filling->n > 0) {
+		Launch(filling);
+		filling = nullptr;
+	}
+	Harvest();
+}
+
+int QueuedHIPTracker::GetQueueLength(int* maxQueueLen)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	Harvest();
+	if (maxQueueLen) *maxQueueLen = numSlots * batchSize;
+	return QueueLengthLocked();
+}
+
+bool QueuedHIPTracker::IsIdle() { return GetQueueLength(nullptr) == 0; }
+
+int QueuedHIPTracker::GetResultCount()
+{
+	{
+		std::unique_lock<std::mutex> lk(mtx);
+		Harvest();
+	}
+	std::lock_guard<std::mutex> g(res_mtx);
+	return (int)results.size();
+}
+
+// LIFO like the CPU tracker (cputrack/QueuedCPUTracker.cpp:458-469).
+int QueuedHIPTracker::FetchResults(LocalizationResult* out, int maxResults)
+{
+	{
+		std::unique_lock<std::mutex> lk(mtx);
+		Harvest();
+	}
+	std::lock_guard<std::mutex> g(res_mtx);
+	int n = 0;
+	while (n < maxResults && !results.empty()) {
+		out[n++] = results.back();
+		results.pop_back();
+	}
+	return n;
+}
+
+void QueuedHIPTracker::ClearResults()
+{
+	std::lock_guard<std::mutex> g(res_mtx);
+	results.clear();
+}
+
+// Flush, wait for idle, then switch (cudatrack/QueuedCUDATracker.cu:420-428).
+void QueuedHIPTracker::SetLocalizationMode(LocMode_t mode)
+{
+	const int supported = LT_QI | LT_LocalizeZ | LT_NormalizeProfile | LT_ClearFirstFourPixels;
+	if (mode & ~supported) {
+		std::ostringstream os;
+		os << "localization mode bits 0x" << std::hex << (mode & ~supported) << " are not implemented";
+		fail(os.str());
+	}
+	Flush();
+	std::unique_lock<std::mutex> lk(mtx);
+	WaitAllLocked(lk);
+	localizeMode = mode;
+}
+
+// ------------------------------------------------------------- Z-LUT
+
+void QueuedHIPTracker::SetRadialZLUT(float* data, int count, int planes)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	WaitAllLocked(lk);
+	const int res = cfg.zlut_radialsteps;
+	SetZLUTBiasCorrection(nullptr, 0, 0);  // the reference drops its bias table here (:368-369)
+	if (count < 0 || planes < 0) fail("negative LUT size");
+	const size_t total = (size_t)count * planes * res;
+	if (total > 0) {
+		h_zlut.assign(total, 0.0f);
+		if (data) memcpy(h_zlut.data(), data, total * 4);
+		zcount = count;
+		zplanes = planes;
+		upload(d_zlut, h_zlut.data(), total * 4);
+		d_cmp.ensure((size_t)count * planes * 4);
+		HIP_CHECK(hipMemset(d_cmp.p, 0, (size_t)count * planes * 4));
+	} else {
+		h_zlut.clear();
+		zcount = zplanes = 0;
+	}
+	h_zlut_stale = false;
+}
+
+void QueuedHIPTracker::GetRadialZLUT(float* dst)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	if (h_zlut_stale && !h_zlut.empty()) {
+		HIP_CHECK(hipMemcpy(h_zlut.data(), d_zlut.p, h_zlut.size() * 4, hipMemcpyDeviceToHost));
+		h_zlut_stale = false;
+	}
+	if (!h_zlut.empty()) memcpy(dst, h_zlut.data(), h_zlut.size() * 4);
+}
+
+void QueuedHIPTracker::GetRadialZLUTSize(int& count, int& planes, int& radialsteps)
+{
+	count = zcount;
+	planes = zplanes;
+	radialsteps = cfg.zlut_radialsteps;
+}
+
+void QueuedHIPTracker::SetRadialWeights(float* w)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	WaitAllLocked(lk);
+	if (w) {
+		upload(d_zcmp, w, (size_t)cfg.zlut_radialsteps * 4);
+		has_zcmp = true;
+	} else
+		has_zcmp = false;
+}
+
+void QueuedHIPTracker::EnableRadialZLUTCompareProfile(bool enabled)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	cmp_enabled = enabled;
+}
+
+void QueuedHIPTracker::GetRadialZLUTCompareProfile(float* dst)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	WaitAllLocked(lk);
+	if (zcount > 0 && d_cmp.p)
+		HIP_CHECK(hipMemcpy(dst, d_cmp.p, (size_t)zcount * zplanes * 4, hipMemcpyDeviceToHost));
+}
+
+// Per-bead calibration, cputrack/QueuedCPUTracker.cpp:165-196: only with a
+// LUT loaded; an image is taken only while none is set, and NULL removes it.
+void QueuedHIPTracker::SetPixelCalibrationImages(float* offset, float* gain)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	WaitAllLocked(lk);
+	if (zcount <= 0) return;
+	const size_t bytes = (size_t)cfg.width * cfg.height * zcount * 4;
+	if (!has_gain && gain) { upload(d_calib_gain, gain, bytes); has_gain = true; }
+	else if (has_gain && !gain) { d_calib_gain.release(); has_gain = false; }
+	if (!has_off && offset) { upload(d_calib_off, offset, bytes); has_off = true; }
+	else if (has_off && !offset) { d_calib_off.release(); has_off = false; }
+}
+
+// Stored but, as in the CPU tracker, not applied (cputrack/QueuedCPUTracker.cpp:266).
+void QueuedHIPTracker::SetPixelCalibrationFactors(float offsetFactor, float gainFactor)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	gc_offsetFactor = offsetFactor;
+	gc_gainFactor = gainFactor;
+}
+
+void QueuedHIPTracker::BeginLUT(uint flags)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	buildFlags = flags;
+}
+
+// QueuedCPUTracker::BuildLUT (cputrack/QueuedCPUTracker.cpp:580-646) on the
+// GPU: image i of the stack belongs to bead i; COM -> QI (no window) or the
+// known position, radial profile, added into plane `plane` of bead i's LUT.
+void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, int plane, vector2f* known_pos)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	WaitAllLocked(lk);
+	if (buildFlags & (BUILDLUT_IMAGELUT | BUILDLUT_FOURIER)) fail("BUILDLUT_IMAGELUT / BUILDLUT_FOURIER are not implemented");
+	if (zcount <= 0) fail("BuildLUT needs a LUT allocated with SetRadialZLUT");
+	if (plane < 0 || plane >= zplanes) fail("BuildLUT plane out of range");
+	const int bpp = PDT_BytesPerPixel(pdt);
+	const size_t row = (size_t)cfg.width * bpp, rb = row * cfg.height;
+	h_build_rois.ensure(rb * zcount);
+	for (int i = 0; i < zcount; i++)
+		for (int y = 0; y < cfg.height; y++)
+			memcpy(h_build_rois.as<uint8_t>() + rb * i + row * y,
+			       (const uint8_t*)data + (size_t)pitch * cfg.height * i + (size_t)pitch * y, row);
+	d_build_rois.ensure(rb * zcount);
+	HIP_CHECK(hipMemcpyAsync(d_build_rois.p, h_build_rois.p, rb * zcount, hipMemcpyHostToDevice, main_stream));
+	LocalizeParams P;
+	FillParams(P, (int)pdt, zcount, d_build_rois.p, nullptr, (int)row, nullptr, nullptr);
+	P.mode = (buildFlags & BUILDLUT_NORMALIZE) ? LT_NormalizeProfile : 0;
+	P.build = 1;
+	P.build_plane = plane;
+	P.build_lut = d_zlut.as<float>();
+	P.zcmp = nullptr;
+	P.cmp_out = nullptr;
+	if (known_pos) {
+		upload(d_build_pos, known_pos, sizeof(vector2f) * zcount);
+		P.known_pos = d_build_pos.as<float>();
+	}
+	HIP_CHECK(launch_localize(P, main_stream));
+	HIP_CHECK(hipStreamSynchronize(main_stream));
+	h_zlut_stale = true;
+}
+
+// Normalizes every plane profile (cputrack/QueuedCPUTracker.cpp:648-659).
+void QueuedHIPTracker::FinalizeLUT()
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	WaitAllLocked(lk);
+	if (zcount <= 0) return;
+	HIP_CHECK(launch_normalize_rows(d_zlut.as<float>(), zcount * zplanes, cfg.zlut_radialsteps, main_stream));
+	HIP_CHECK(hipStreamSynchronize(main_stream));
+	HIP_CHECK(hipMemcpy(h_zlut.data(), d_zlut.p, h_zlut.size() * 4, hipMemcpyDeviceToHost));
+	h_zlut_stale = false;
+}
+
+// ------------------------------------------------------------- config
+
+void QueuedHIPTracker::SetConfigValue(std::string name, std::string value)
+{
+	if (name == "batch_size" || name == "num_streams") {
+		int v = atoi(value.c_str());
+		if (v < 1) fail(name + " must be >= 1");
+		Flush();
+		std::unique_lock<std::mutex> lk(mtx);
+		WaitAllLocked(lk);
+		FreeSlots();
+		if (name == "batch_size") batchSize = v;
+		else numSlots = v;
+		AllocSlots();
+	}
+}
+
+QueuedTracker::ConfigValueMap QueuedHIPTracker::GetConfigValues()
+{
+	ConfigValueMap m;
+	m["batch_size"] = std::to_string(batchSize);
+	m["num_streams"] = std::to_string(numSlots);
+	return m;
+}
+
+std::string QueuedHIPTracker::GetProfileReport()
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	std::ostringstream os;
+	double f = batchesDone ? 1.0 / batchesDone : 0.0;
+	os << "MI355X tracker: " << batchesDone << " batches (" << roisDone << " ROIs), batch size " << batchSize
+	   << ", " << numSlots << " streams\n";
+	os << "kernel ms/batch: " << t_kernel_ms * f << "\ncopies+queue ms/batch: " << t_copy_ms * f << "\n";
+	return os.str();
+}
+
+std::string QueuedHIPTracker::GetWarnings()
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	return warnings;
+}
+
+}  // namespace qtrk

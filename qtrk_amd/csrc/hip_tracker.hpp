@@ -1,0 +1,140 @@
+// hip_tracker.hpp -- QueuedHIPTracker: the MI355X implementation of the
+// QueuedTracker interface (cputrack/QueuedTracker.h:22-96).
+//
+// Host model: scheduled ROIs are copied into pinned batch slots; a slot is
+// launched (H2D, fused localization kernel, D2H, event) when it holds
+// batch_size ROIs or on Flush; completed slots are harvested into the result
+// deque whenever a caller touches the queue.  There is no scheduler thread:
+// every call that needs progress polls the slot events.
+#pragma once
+
+#include "QueuedTracker.h"
+#include "localize.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace qtrk {
+
+struct DeviceBuffer {
+	void* p = nullptr;
+	size_t bytes = 0;
+	void ensure(size_t n);
+	void release();
+	template <typename T> T* as() const { return (T*)p; }
+	~DeviceBuffer() { release(); }
+};
+
+struct PinnedBuffer {
+	void* p = nullptr;
+	size_t bytes = 0;
+	void ensure(size_t n);
+	void release();
+	template <typename T> T* as() const { return (T*)p; }
+	~PinnedBuffer() { release(); }
+};
+
+class QueuedHIPTracker : public QueuedTracker {
+public:
+	explicit QueuedHIPTracker(const QTrkComputedConfig& cc);
+	~QueuedHIPTracker() override;
+
+	void SetLocalizationMode(LocMode_t locType) override;
+	void SetPixelCalibrationImages(float* offset, float* gain) override;
+	void SetPixelCalibrationFactors(float offsetFactor, float gainFactor) override;
+	void ScheduleLocalization(void* data, int pitch, QTRK_PixelDataType pdt, const LocalizationJob* jobInfo) override;
+	void ClearResults() override;
+	void Flush() override;
+	void SetRadialZLUT(float* data, int count, int planes) override;
+	void GetRadialZLUT(float* dst) override;
+	void GetRadialZLUTSize(int& count, int& planes, int& radialsteps) override;
+	void SetRadialWeights(float* zcmp) override;
+	void EnableRadialZLUTCompareProfile(bool enabled) override;
+	void GetRadialZLUTCompareProfile(float* dst) override;
+	void BeginLUT(uint flags) override;
+	void BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, int plane, vector2f* known_pos = 0) override;
+	void FinalizeLUT() override;
+	int GetResultCount() override;
+	int FetchResults(LocalizationResult* results, int maxResults) override;
+	int GetQueueLength(int* maxQueueLen = 0) override;
+	bool IsIdle() override;
+	void SetConfigValue(std::string name, std::string value) override;
+	ConfigValueMap GetConfigValues() override;
+	std::string GetProfileReport() override;
+	std::string GetWarnings() override;
+
+	// Fills the kernel parameters from the tracker's current state for n ROIs
+	// at `src` (contiguous when roi_offset is null).
+	void FillParams(LocalizeParams& P, int pdt, int n, const void* src, const int64_t* roi_offset, int pitch,
+	                const DevJob* jobs, DevResult* results);
+	int Device() const { return dev; }
+	int LocalizeMode() const { return localizeMode; }
+	void WaitLaunched();
+
+private:
+	struct Slot {
+		enum State { Free, Filling, Launched } state = Free;
+		int n = 0, pdt = 0;
+		uint64_t seq = 0;
+		PinnedBuffer h_rois, h_jobs, h_results;
+		DeviceBuffer d_rois, d_jobs, d_results;
+		hipStream_t stream = nullptr;
+		hipEvent_t ev_start = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
+	};
+
+	void BuildTables();
+	void AllocSlots();
+	void FreeSlots();
+	Slot* AcquireSlot(std::unique_lock<std::mutex>& lk);
+	void Launch(Slot* s);
+	void Harvest();
+	void WaitAllLocked(std::unique_lock<std::mutex>& lk);
+	int QueueLengthLocked() const;
+	void Warn(const std::string& w);
+
+	int dev = 0;
+	hipStream_t main_stream = nullptr;
+
+	// per-config device tables
+	DeviceBuffer d_qi_steps, d_qi_stride, d_qi_dirs, d_qi_window, d_perm, d_tw_fwd, d_tw_inv, d_z_dirs;
+	int fft_n = 0, fft_nst = 0;
+	int fft_radix[kMaxFFTStages] = {}, fft_m[kMaxFFTStages] = {}, fft_fs[kMaxFFTStages] = {};
+
+	// radial Z-LUT [count][planes][zlut_radialsteps], weights, compare profiles
+	std::vector<float> h_zlut;
+	bool h_zlut_stale = false;
+	DeviceBuffer d_zlut, d_zcmp, d_cmp;
+	int zcount = 0, zplanes = 0;
+	bool has_zcmp = false, cmp_enabled = false;
+
+	// per-bead calibration images
+	DeviceBuffer d_calib_off, d_calib_gain;
+	bool has_off = false, has_gain = false;
+	float gc_offsetFactor = 1.0f, gc_gainFactor = 1.0f;
+
+	LocMode_t localizeMode = 0;
+	uint buildFlags = 0;
+	DeviceBuffer d_build_rois, d_build_pos;
+	PinnedBuffer h_build_rois;
+
+	int batchSize = 1024, numSlots = 4;
+	std::vector<Slot*> slots;
+	Slot* filling = nullptr;
+	uint64_t seqCounter = 0;
+
+	mutable std::mutex mtx;
+	std::mutex res_mtx;
+	std::deque<LocalizationResult> results;
+
+	std::string warnings;
+	double t_kernel_ms = 0, t_copy_ms = 0;
+	uint64_t batchesDone = 0, roisDone = 0;
+	size_t lds_limit = 0;
+};
+
+}  // namespace qtrk

@@ -1,0 +1,557 @@
+// localize.hip -- fused per-ROI localization chain for gfx950.
+//
+// One workgroup (4 wavefronts, 256 lanes) localizes one ROI entirely out of
+// LDS: pixel load/convert (+ calibration) -> mean/COM -> QI iterations
+// (quadrant profiles, kissfft-ordered radix-4/2 FFT autoconvolution, 7-point
+// peak fit) -> radial profile (+ normalisation) -> Z-LUT scores, best plane
+// and 7-point FP64 fit.  Nothing but the ROI, the bead's LUT rows and the
+// 52-byte result touches HBM.
+//
+// The arithmetic follows the reference CPU tracker operation for operation
+// (this file is built with -ffp-contract=off and IEEE division/sqrt):
+//   * quadrant profiles: one lane per (quadrant, radius) summing its angular
+//     samples in the reference order in float (cputrack/cpu_tracker.cpp:631-677);
+//   * the QI FFTs reproduce kissfft's decimation-in-time butterflies
+//     (cputrack/kissfft.h:78-152) with one lane per butterfly;
+//   * the peak fits are the reference's weighted least-squares solve
+//     (cputrack/LsqQuadraticFit.h:18-24, 64-102, 132-176);
+//   * LUT scores: one lane per plane, the reference's float difference and
+//     double accumulation in radius order (cputrack/cpu_tracker.cpp:829-839).
+// Double sums over many pixels/angles (COM moments, radial-profile bins) are
+// reduced in a fixed tree instead of left to right; for integer pixel data
+// the mean/variance sums are exact either way.
+#include "localize.hpp"
+
+#include <cmath>
+
+namespace qtrk {
+
+__constant__ float kQIW[7] = { 0.14f, 0.5f, 0.85f, 1.0f, 0.85f, 0.5f, 0.14f };
+__constant__ double kZW[7] = { (double)0.15f, (double)0.5f, (double)0.85f, (double)1.0f,
+                               (double)0.85f, (double)0.5f, (double)0.15f };
+
+struct LdsLayout {
+	int img, qprof, fft, tw, perm, zprof, zpart, zcnt, scores, red, st, total;
+};
+
+__host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
+
+__host__ __device__ inline int z_chunks(const LocalizeParams& P)
+{
+	if (P.z_nb <= 0) return 1;
+	int c = kBlock / P.z_nb;
+	return c < 1 ? 1 : c;
+}
+
+__host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P)
+{
+	LdsLayout L;
+	int o = 0;
+	const int N = P.fft_n > 0 ? P.fft_n : 0;
+	L.img = o;    o += align16(P.width * P.height * 4);
+	L.qprof = o;  o += align16(4 * P.qi_nr * 4);
+	L.fft = o;    o += align16(6 * N * 8);
+	L.tw = o;     o += align16(2 * N * 8);
+	L.perm = o;   o += align16(N * 4);
+	L.zprof = o;  o += align16(P.z_nb * 4);
+	const int nch = z_chunks(P);
+	L.zpart = o;  o += align16(nch * P.z_nb * 8);
+	L.zcnt = o;   o += align16(nch * P.z_nb * 4);
+	L.scores = o; o += align16(P.zplanes * 8);
+	L.red = o;    o += align16(64 * 8);
+	L.st = o;     o += align16(16 * 4);
+	L.total = o;
+	return L;
+}
+
+size_t localize_lds_bytes(const LocalizeParams& p) { return (size_t)lds_layout(p).total; }
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b)
+{
+	return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// Bilinear sample with the reference's "outside" rule (cputrack/utils.h:39-58):
+// trunc(x) in [0, W-2] <=> -1 < x < W-1 (for W >= 2); the range form also
+// rejects NaN, as the reference's x86 conversion does.
+__device__ __forceinline__ bool bilinear(const float* img, int W, int H, float x, float y, float& v)
+{
+	if (!(x > -1.0f && y > -1.0f && x < (float)(W - 1) && y < (float)(H - 1))) return false;
+	const int rx = (int)x, ry = (int)y;
+	const float* p = img + ry * W + rx;
+	const float v00 = p[0], v10 = p[1], v01 = p[W], v11 = p[W + 1];
+	const float fx = x - (float)rx;
+	const float v0 = v00 + (v10 - v00) * fx;
+	const float v1 = v01 + (v11 - v01) * fx;
+	v = v0 + (v1 - v0) * (y - (float)ry);
+	return true;
+}
+
+// Weighted quadratic least squares through n <= 7 points (x = i - iMax),
+// returning a and b of a*x^2 + b*x + c.  Fixed 7-slot loops keep the arrays
+// in registers; slots >= n are skipped so the sums run in the reference order.
+template <typename T>
+__device__ __forceinline__ void lsq_quad(int n, const T (&X)[7], const T (&Y)[7], const T (&Wt)[7], T& a, T& b)
+{
+	T Sx = 0, Sy = 0, Sx2 = 0, Sx3 = 0, Sxy = 0, Sx4 = 0, Sx2y = 0, Sw = 0;
+#pragma unroll
+	for (int i = 0; i < 7; i++) {
+		if (i < n) {
+			const T x = X[i], y = Y[i];
+			Sx += x * Wt[i];
+			Sy += y * Wt[i];
+			const T sq = x * x;
+			Sx2 += x * x * Wt[i];
+			Sx3 += sq * x * Wt[i];
+			Sx4 += sq * sq * Wt[i];
+			Sxy += x * y * Wt[i];
+			Sx2y += sq * y * Wt[i];
+			Sw += Wt[i];
+		}
+	}
+	const T s40 = Sx4, s30 = Sx3, s20 = Sx2, s10 = Sx, s21 = Sx2y, s11 = Sxy, s01 = Sy, s00 = Sw;
+	const T d = s40 * (s20 * s00 - s10 * s10) - s30 * (s30 * s00 - s10 * s20) + s20 * (s30 * s10 - s20 * s20);
+	a = (s21 * (s20 * s00 - s10 * s10) - s11 * (s30 * s00 - s10 * s20) + s01 * (s30 * s10 - s20 * s20)) / d;
+	b = (s40 * (s11 * s00 - s01 * s10) - s30 * (s21 * s00 - s01 * s20) + s20 * (s21 * s10 - s11 * s20)) / d;
+}
+
+// Sub-sample peak position: the 7-point fit around the argmax iMax.
+template <typename T, typename Get>
+__device__ __forceinline__ T peak_fit(int iMax, int len, const T* wts, Get get)
+{
+	const int s = iMax - 3 > 0 ? iMax - 3 : 0;
+	const int e = iMax + 4 < len ? iMax + 4 : len;
+	const int np = e - s;
+	if (np < 3) return (T)iMax;
+	T xs[7], ys[7], w[7];
+#pragma unroll
+	for (int i = 0; i < 7; i++) {
+		xs[i] = (T)(s + i - iMax);
+		ys[i] = i < np ? get(s + i) : (T)0;
+		w[i] = wts[i];
+	}
+	T a, b;
+	lsq_quad<T>(np, xs, ys, w, a, b);
+	const T lim = (T)1e-9f;
+	if ((a < 0 ? -a : a) < lim) return (T)iMax;
+	return (T)iMax + (-b / (2 * a));
+}
+
+__device__ __forceinline__ void wave_argmax(float& v, int& i)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const float ov = __shfl_xor(v, off);
+		const int oi = __shfl_xor(i, off);
+		if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+	}
+}
+__device__ __forceinline__ void wave_argmax(double& v, int& i)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const double ov = __shfl_xor(v, off);
+		const int oi = __shfl_xor(i, off);
+		if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+	}
+}
+
+// Sum of K doubles over the workgroup; result in red[32..32+K) for all threads.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* red, int lane, int wid)
+{
+	for (int k = 0; k < K; k++)
+		for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+	if (lane == 0)
+		for (int k = 0; k < K; k++) red[wid * K + k] = v[k];
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int k = 0; k < K; k++) {
+			double s = red[k];
+			for (int w = 1; w < kBlock / 64; w++) s += red[w * K + k];
+			red[32 + k] = s;
+		}
+	}
+	__syncthreads();
+}
+
+// kissfft's stages from the deepest (smallest) to the outermost, one lane per
+// butterfly, `nbufs` transforms of length N side by side (one per wavefront).
+__device__ __forceinline__ void fft_stages(float2* base, int nbufs, int N, const LocalizeParams& P,
+                                           const float2* tw, float nii, int wid, int lane)
+{
+#pragma unroll
+	for (int si = 0; si < kMaxFFTStages; si++) {
+		const int s = kMaxFFTStages - 1 - si;
+		if (s >= P.fft_nst) continue;  // uniform: every thread skips together
+		const int r = P.fft_radix[s], m = P.fft_m[s], fs = P.fft_fs[s];
+		if (wid < nbufs) {
+			float2* F = base + wid * N;
+			const int nbf = N / r;
+			for (int b = lane; b < nbf; b += 64) {
+				const int g = b / m, k = b - g * m;
+				float2* G = F + g * r * m;
+				if (r == 4) {
+					const float2 s0 = cmul(G[k + m], tw[k * fs]);
+					const float2 s1 = cmul(G[k + 2 * m], tw[k * fs * 2]);
+					const float2 s2 = cmul(G[k + 3 * m], tw[k * fs * 3]);
+					float2 f0 = G[k];
+					const float2 s5 = csub(f0, s1);
+					f0 = cadd(f0, s1);
+					const float2 s3 = cadd(s0, s2);
+					float2 s4 = csub(s0, s2);
+					s4 = make_float2(s4.y * nii, -s4.x * nii);
+					G[k + 2 * m] = csub(f0, s3);
+					G[k] = cadd(f0, s3);
+					G[k + m] = cadd(s5, s4);
+					G[k + 3 * m] = csub(s5, s4);
+				} else {
+					const float2 t = cmul(G[m + k], tw[k * fs]);
+					const float2 f0 = G[k];
+					G[m + k] = csub(f0, t);
+					G[k] = cadd(f0, t);
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// QI profile concatenations (cputrack/cpu_tracker.cpp:327-343):
+// X = [rev(q1+q2), q0+q3], Y = [rev(q2+q3), q0+q1].
+__device__ __forceinline__ float qi_input(const float* qp, int nr, int axis, int n)
+{
+	const float* q0 = qp;
+	const float* q1 = qp + nr;
+	const float* q2 = qp + 2 * nr;
+	const float* q3 = qp + 3 * nr;
+	if (n < nr) {
+		const int r = nr - 1 - n;
+		return axis == 0 ? q1[r] + q2[r] : q2[r] + q3[r];
+	}
+	const int r = n - nr;
+	return axis == 0 ? q0[r] + q3[r] : q0[r] + q1[r];
+}
+
+__global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
+{
+	extern __shared__ __align__(16) unsigned char lds[];
+	const LdsLayout L = lds_layout(P);
+	float* img = (float*)(lds + L.img);
+	float* qprof = (float*)(lds + L.qprof);
+	float2* fft = (float2*)(lds + L.fft);
+	float2* tw = (float2*)(lds + L.tw);
+	int* perm = (int*)(lds + L.perm);
+	float* zprof = (float*)(lds + L.zprof);
+	double* zpart = (double*)(lds + L.zpart);
+	int* zcnt = (int*)(lds + L.zcnt);
+	double* scores = (double*)(lds + L.scores);
+	double* red = (double*)(lds + L.red);
+	float* st = (float*)(lds + L.st);
+
+	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	const int roi = blockIdx.x;
+	const int W = P.width, H = P.height, npx = W * H;
+	const bool build = P.build != 0;
+	const DevJob job = P.jobs ? P.jobs[roi] : DevJob{};
+	const int bead = build ? roi : job.zlutIndex;
+
+	// ---- ROI -> LDS as float (cputrack/cpu_tracker.h:123-136), ClearFirstFourPixels
+	// (cputrack/QueuedCPUTracker.cpp:280-282), per-bead offset/gain with unit
+	// factors (cputrack/QueuedCPUTracker.cpp:258-271, cputrack/cpu_tracker.cpp:98-112).
+	{
+		const uint8_t* base = P.src + (P.roi_offset ? P.roi_offset[roi] : (int64_t)roi * H * P.pitch);
+		const bool calib = (P.calib_offset || P.calib_gain) && bead >= 0 && bead < P.zcount;
+		const float* off = calib && P.calib_offset ? P.calib_offset + (size_t)bead * npx : nullptr;
+		const float* gain = calib && P.calib_gain ? P.calib_gain + (size_t)bead * npx : nullptr;
+		const bool clear4 = (P.mode & kModeClearFirstFour) && !build;
+		for (int y = wid; y < H; y += kBlock / 64) {
+			const uint8_t* row = base + (int64_t)y * P.pitch;
+			for (int x = lane; x < W; x += 64) {
+				float v;
+				if (P.pdt == 1) v = (float)((const uint16_t*)row)[x];
+				else if (P.pdt == 0) v = (float)row[x];
+				else v = ((const float*)row)[x];
+				const int i = y * W + x;
+				if (clear4 && i < 4) v = 0.0f;
+				if (off && !gain) v = v + off[i] * 1.0f;
+				else if (gain && !off) v = v * gain[i] * 1.0f;
+				else if (gain && off) v = (v + off[i] * 1.0f) * gain[i] * 1.0f;
+				img[i] = v;
+			}
+		}
+	}
+	__syncthreads();
+
+	// ---- mean, stdev and background-corrected COM (cputrack/cpu_tracker.cpp:681-729)
+	float mean = 0.0f, comx, comy;
+	const bool known = build && P.known_pos;
+	if (known) {
+		comx = P.known_pos[2 * roi];
+		comy = P.known_pos[2 * roi + 1];
+	} else {
+		double m1[2] = { 0.0, 0.0 };
+		for (int i = tid; i < npx; i += kBlock) {
+			const float v = img[i];
+			m1[0] += (double)v;
+			m1[1] += (double)(float)(v * v);
+		}
+		block_sum<2>(m1, red, lane, wid);
+		const float invN = 1.0f / (float)npx;
+		mean = (float)(red[32] * (double)invN);
+		const double var = red[33] * (double)invN - (double)(mean * mean);
+		const float stdev = sqrtf((float)var);
+		const float bgs = (build ? 0.0f : P.com_bgcorrection) * stdev;
+		__syncthreads();
+		double m2[3] = { 0.0, 0.0, 0.0 };
+		for (int y = wid; y < H; y += kBlock / 64) {
+			for (int x = lane; x < W; x += 64) {
+				const float d = fabsf(img[y * W + x] - mean) - bgs;
+				const float v = (0.0f < d) ? d : 0.0f;
+				m2[0] += (double)v;
+				m2[1] += (double)x * (double)v;
+				m2[2] += (double)y * (double)v;
+			}
+		}
+		block_sum<3>(m2, red, lane, wid);
+		comx = (float)(red[33] / red[32]);
+		comy = (float)(red[34] / red[32]);
+		// NaN fallback to the ROI centre (cputrack/QueuedCPUTracker.cpp:294-295); BuildLUT has none.
+		if (!build && (isnan(comx) || isnan(comy))) {
+			comx = (float)(W / 2);
+			comy = (float)(H / 2);
+		}
+	}
+
+	// ---- QI (cputrack/cpu_tracker.cpp:285-406)
+	float cx = comx, cy = comy;
+	int boundary = 0;
+	const bool do_qi = (P.mode & kModeQI) || (build && !known);
+	if (do_qi && P.qi_iters > 0) {
+		const int nr = P.qi_nr, N = P.fft_n;
+		const float maxr = P.qi_maxr, minr = P.qi_minr;
+		const float rstep = (maxr - minr) / (float)nr;
+		const float* window = build ? nullptr : P.qi_window;
+		for (int i = tid; i < N; i += kBlock) {
+			tw[i] = P.tw_fwd[i];
+			tw[N + i] = P.tw_inv[i];
+			perm[i] = P.fft_perm[i];
+		}
+		for (int k = 0; k < P.qi_iters; k++) {
+			boundary = (cx + maxr >= (float)W) || (cx - maxr < 0.0f) || (cy + maxr >= (float)H) || (cy - maxr < 0.0f);
+			const int steps = P.qi_iter_steps[k], stride = P.qi_iter_stride[k];
+			for (int p = tid; p < 4 * nr; p += kBlock) {
+				const int q = p / nr, ri = p - q * nr;
+				const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
+				const float sy = (q < 2) ? 1.0f : -1.0f;
+				const float r = minr + rstep * (float)ri;
+				float sum = 0.0f;
+				int cnt = 0;
+				for (int a = 0; a < steps; a++) {
+					const float2 d = P.qi_dirs[stride * a];
+					const float x = cx + (sx * d.x) * r;
+					const float y = cy + (sy * d.y) * r;
+					float v;
+					if (bilinear(img, W, H, x, y, v)) { sum += v; cnt++; }
+				}
+				float val = cnt >= 4 ? sum / (float)cnt : mean;
+				if (window) val *= window[ri];
+				qprof[p] = val;
+			}
+			__syncthreads();
+			// leaves of the 4 forward transforms: X, reversed X, Y, reversed Y
+			{
+				float2* F = fft + wid * N;
+				const int axis = wid >> 1;
+				for (int j = lane; j < N; j += 64) {
+					int s = perm[j];
+					if (wid & 1) s = N - 1 - s;
+					F[j] = make_float2(qi_input(qprof, nr, axis, s), 0.0f);
+				}
+			}
+			__syncthreads();
+			fft_stages(fft, 4, N, P, tw, 1.0f, wid, lane);
+			// F(p) * conj(F(rev p)), written in leaf order for the inverse transforms
+			for (int t = tid; t < 2 * N; t += kBlock) {
+				const int ax = t >= N ? 1 : 0;
+				const int j = t - ax * N;
+				const int s = perm[j];
+				const float2 a = fft[(2 * ax) * N + s];
+				const float2 b = fft[(2 * ax + 1) * N + s];
+				fft[(4 + ax) * N + j] = cmul(a, make_float2(b.x, -b.y));
+			}
+			__syncthreads();
+			fft_stages(fft + 4 * N, 2, N, P, tw + N, -1.0f, wid, lane);
+			if (wid < 2) {
+				const float2* G = fft + (4 + wid) * N;
+				float bv = 0.0f;
+				int bi = 0x7fffffff;
+				for (int x = lane; x < N; x += 64) {
+					const float v = G[(x + nr) & (N - 1)].x;
+					if (bi == 0x7fffffff || v > bv) { bv = v; bi = x; }
+				}
+				if (bi == 0x7fffffff) bv = -INFINITY;
+				wave_argmax(bv, bi);
+				if (lane == 0) {
+					const float maxPos = peak_fit<float>(bi, N, kQIW, [&](int x) { return G[(x + nr) & (N - 1)].x; });
+					const float offs = (maxPos - (float)nr) * (3.14159265359f / 4);
+					if (wid == 0) st[0] = cx + offs * rstep;
+					else st[1] = cy + offs * rstep;
+				}
+			}
+			__syncthreads();
+			cx = st[0];
+			cy = st[1];
+			__syncthreads();
+		}
+	}
+
+	// ---- radial profile (cputrack/utils.cpp:275-320) and normalisation (:230-261)
+	float z = 0.0f;
+	const bool lz = (P.mode & kModeLocalizeZ) || build;
+	if (lz && P.z_nb > 0) {
+		const int nb = P.z_nb, nch = z_chunks(P);
+		const float rstep = (P.z_maxr - P.z_minr) / (float)nb;
+		for (int p = tid; p < nch * nb; p += kBlock) {
+			const int ch = p / nb, bin = p - ch * nb;
+			const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
+			const float r = P.z_minr + rstep * (float)bin;
+			double s = 0.0;
+			int c = 0;
+			for (int a = a0; a < a1; a++) {
+				const float2 d = P.z_dirs[a];
+				const float x = cx + d.x * r;
+				const float y = cy + d.y * r;
+				float v;
+				if (bilinear(img, W, H, x, y, v)) { s += (double)v; c++; }
+			}
+			zpart[p] = s;
+			zcnt[p] = c;
+		}
+		__syncthreads();
+		for (int bin = tid; bin < nb; bin += kBlock) {
+			double s = zpart[bin];
+			int c = zcnt[bin];
+			for (int ch = 1; ch < nch; ch++) {
+				s += zpart[ch * nb + bin];
+				c += zcnt[ch * nb + bin];
+			}
+			zprof[bin] = c > 4 ? (float)(s / (double)c) : mean;
+		}
+		__syncthreads();
+		if ((P.mode & kModeNormalizeProfile) && tid == 0) {
+			double s = 0.0;
+			for (int i = 0; i < nb; i++) s += (double)zprof[i];
+			const float pm = (float)(s / (double)nb);
+			double s2 = 0.0;
+			for (int i = 0; i < nb; i++) {
+				const float v = zprof[i] - pm;
+				zprof[i] = v;
+				s2 += (double)(float)(v * v);
+			}
+			const double inv = (double)1.0f / sqrt(s2 / (double)nb);
+			for (int i = 0; i < nb; i++) zprof[i] = (float)((double)zprof[i] * inv);
+		}
+		__syncthreads();
+		boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
+		           (cy - P.z_maxr < 0.0f);
+
+		if (build) {
+			float* dst = P.build_lut + ((size_t)roi * P.zplanes + P.build_plane) * nb;
+			for (int bin = tid; bin < nb; bin += kBlock) dst[bin] += zprof[bin];
+		} else if (P.zluts && bead >= 0 && bead < P.zcount && P.zplanes > 0) {
+			// ---- LUT scores, best plane, FP64 peak fit (cputrack/cpu_tracker.cpp:791-868)
+			const float* L0 = P.zluts + (size_t)bead * P.zplanes * nb;
+			for (int k = tid; k < P.zplanes; k += kBlock) {
+				const float* Lk = L0 + (size_t)k * nb;
+				double s = 0.0;
+				for (int r = 0; r < nb; r++) {
+					double d = (double)(zprof[r] - Lk[r]);
+					if (P.zcmp) d *= (double)P.zcmp[r];
+					d = -d * d;
+					s += d;
+				}
+				scores[k] = s;
+				if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k] = (float)s;
+			}
+			__syncthreads();
+			double bv = 0.0;
+			int bi = 0x7fffffff;
+			for (int k = tid; k < P.zplanes; k += kBlock) {
+				const double v = scores[k];
+				if (bi == 0x7fffffff || v > bv) { bv = v; bi = k; }
+			}
+			if (bi == 0x7fffffff) bv = -INFINITY;
+			wave_argmax(bv, bi);
+			if (lane == 0) {
+				red[2 * wid] = bv;
+				((int*)st)[4 + wid] = bi;
+			}
+			__syncthreads();
+			if (tid == 0) {
+				double v = red[0];
+				int i = ((int*)st)[4];
+				for (int w = 1; w < kBlock / 64; w++) {
+					const double ov = red[2 * w];
+					const int oi = ((int*)st)[4 + w];
+					if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+				}
+				z = (float)peak_fit<double>(i, P.zplanes, kZW, [&](int k) { return scores[k]; });
+			}
+		}
+	}
+
+	if (tid == 0 && !build) {
+		DevResult R;
+		R.job = job;
+		R.x = cx;
+		R.y = cy;
+		R.z = z;
+		R.comx = comx;
+		R.comy = comy;
+		R.error = boundary ? 1u : 0u;
+		R.imageMean = mean;
+		P.results[roi] = R;
+	}
+}
+
+hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
+{
+	if (p.n <= 0) return hipSuccess;
+	const size_t lds = localize_lds_bytes(p);
+	static bool attr_done = false;
+	if (!attr_done) {
+		hipFuncSetAttribute((const void*)localize_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		attr_done = true;
+	}
+	hipLaunchKernelGGL(localize_kernel, dim3(p.n), dim3(kBlock), lds, stream, p);
+	return hipGetLastError();
+}
+
+// NormalizeRadialProfile on each row (FinalizeLUT, cputrack/QueuedCPUTracker.cpp:648-659).
+__global__ void normalize_rows_kernel(float* data, int rows, int len)
+{
+	const int row = blockIdx.x * blockDim.x + threadIdx.x;
+	if (row >= rows) return;
+	float* p = data + (size_t)row * len;
+	double s = 0.0;
+	for (int i = 0; i < len; i++) s += (double)p[i];
+	const float m = (float)(s / (double)len);
+	double s2 = 0.0;
+	for (int i = 0; i < len; i++) {
+		const float v = p[i] - m;
+		p[i] = v;
+		s2 += (double)(float)(v * v);
+	}
+	const double inv = (double)1.0f / sqrt(s2 / (double)len);
+	for (int i = 0; i < len; i++) p[i] = (float)((double)p[i] * inv);
+}
+
+hipError_t launch_normalize_rows(float* data, int rows, int len, hipStream_t stream)
+{
+	if (rows <= 0 || len <= 0) return hipSuccess;
+	hipLaunchKernelGGL(normalize_rows_kernel, dim3((rows + 63) / 64), dim3(64), 0, stream, data, rows, len);
+	return hipGetLastError();
+}
+
+}  // namespace qtrk

@@ -1,0 +1,94 @@
+// localize.hpp -- parameter block of the fused localization kernel, shared by
+// the host runtime (hip_tracker.cpp) and the kernels (localize.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qtrk {
+
+// 24-byte job and 52-byte result, field for field the packed LocalizationJob /
+// LocalizationResult of the C ABI (all fields are 4 bytes, so the natural
+// layout has no padding and 4-byte alignment is safe on the GPU).
+struct DevJob {
+	uint32_t frame, timestamp;
+	int32_t zlutIndex;
+	float ix, iy, iz;
+};
+struct DevResult {
+	DevJob job;
+	float x, y, z;
+	float comx, comy;
+	uint32_t error;
+	float imageMean;
+};
+static_assert(sizeof(DevJob) == 24, "job layout");
+static_assert(sizeof(DevResult) == 52, "result layout");
+
+constexpr int kMaxFFTStages = 8;
+constexpr int kBlock = 256;  // threads per ROI workgroup (4 wavefronts)
+
+enum : int {
+	kModeXCor1D = 1, kModeQI = 2, kModeGauss2D = 4, kModeZLUTAlign = 8, kModeLocalizeZ = 16,
+	kModeNormalizeProfile = 64, kModeClearFirstFour = 128, kModeFourierLUT = 256, kModeLocalizeZWeighted = 512
+};
+
+struct LocalizeParams {
+	// ROI source: ROI i starts at src + roi_offset[i] (or src + i*height*pitch
+	// when roi_offset is null); rows are `pitch` bytes apart.
+	const uint8_t* src;
+	const int64_t* roi_offset;
+	int pitch;
+	int pdt;  // 0 u8, 1 u16, 2 f32
+	int n;
+	const DevJob* jobs;
+	DevResult* results;
+	int width, height;
+	int mode;
+	float com_bgcorrection;
+
+	// LUT-build variant (QueuedCPUTracker::BuildLUT): COM without background
+	// correction, QI without the radial window, profile accumulated into
+	// build_lut[(roi*planes + build_plane)*z_nb + r], no LUT compare.
+	int build;
+	int build_plane;
+	const float* known_pos;  // [n][2] or null
+	float* build_lut;
+
+	// QI
+	int qi_iters, qi_nr, qi_A;
+	float qi_minr, qi_maxr;
+	const int* qi_iter_steps;   // [qi_iters] angular samples per quadrant
+	const int* qi_iter_stride;  // [qi_iters] step through the direction table
+	const float2* qi_dirs;      // [qi_A]
+	const float* qi_window;     // [qi_nr] or null
+
+	// kissfft-ordered FFT of length fft_n = 2*qi_nr
+	int fft_n, fft_nst;
+	int fft_radix[kMaxFFTStages], fft_m[kMaxFFTStages], fft_fs[kMaxFFTStages];
+	const int* fft_perm;        // [fft_n] leaf order
+	const float2* tw_fwd;       // [fft_n]
+	const float2* tw_inv;       // [fft_n]
+
+	// radial profile + Z-LUT
+	int z_nb, z_A;
+	float z_minr, z_maxr;
+	const float2* z_dirs;       // [z_A]
+	const float* zluts;         // [zcount][zplanes][z_nb] or null
+	int zcount, zplanes;
+	const float* zcmp;          // [z_nb] or null
+	float* cmp_out;             // [zcount][zplanes] or null
+
+	// per-bead calibration images [zcount][height][width]
+	const float* calib_offset;
+	const float* calib_gain;
+};
+
+// Dynamic LDS bytes needed by localize_kernel for these parameters.
+size_t localize_lds_bytes(const LocalizeParams& p);
+// Launches one workgroup per ROI on `stream`.
+hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream);
+// Normalizes `rows` profiles of `len` values in place (FinalizeLUT).
+hipError_t launch_normalize_rows(float* data, int rows, int len, hipStream_t stream);
+
+}  // namespace qtrk

@@ -1,0 +1,109 @@
+"""Seeded synthetic bead workloads (the benchmark and parity-test inputs).
+
+Beads are rendered from analytic radial LUTs with the reference's image model
+(GenerateImageFromLUT: cubic B-spline in z, linear in r, 4x4 sub-pixel
+samples, pixel centres at integer coordinates -- cputrack/utils.cpp:325-390)
+and given Poisson noise.  The reference's own simulations use the same model
+(cputrack-test/SharedTests.h:342-416, 495-569) with rand()-based noise, which
+cannot be reproduced across C runtimes; here every random draw comes from a
+numpy PCG64 generator seeded per workload (seed = 0x51AB0000 + index).
+
+Expected photon counts per pixel: lambda = bg + peak * (1 + p) / 2 where p in
+[-1, 1] is the rendered LUT value, so lambda >= bg.  Integer formats round and
+clip to their range.
+"""
+import numpy as np
+
+from . import api
+
+SEED_BASE = 0x51AB0000
+
+# QTrkSettings that reproduce each workload's sampling sizes through
+# QTrkComputedConfig::Update (cputrack/QueuedTracker.cpp:6-21).
+SETTINGS = {
+    # 64x64: R=28, qi 32 radial x 64 angular/quadrant, zlut 32 x 123
+    "plain64": dict(width=64, height=64, qi_minradius=2.0, qi_roi_coverage=0.875, qi_radial_coverage=1.25,
+                    qi_angular_coverage=1.46, qi_angstep_factor=1.0, qi_iterations=5, zlut_minradius=2.0,
+                    zlut_roi_coverage=0.875, zlut_radial_coverage=1.25, zlut_angular_coverage=0.7),
+    # 100x100: R=50, qi 64 radial x 128 angular/quadrant, zlut 64 x 219
+    "realtime100": dict(width=100, height=100, qi_minradius=1.0, qi_roi_coverage=1.0, qi_radial_coverage=1.5,
+                        qi_angular_coverage=1.635, qi_angstep_factor=1.0, qi_iterations=5, zlut_minradius=1.0,
+                        zlut_roi_coverage=1.0, zlut_radial_coverage=1.31, zlut_angular_coverage=0.7),
+}
+
+MODE_QI_Z = api.LT_QI | api.LT_LocalizeZ | api.LT_NormalizeProfile
+
+
+def analytic_lut(planes, steps, minr, maxr, period, dperiod=0.04, width0=6.0, dwidth=0.05):
+    """profile(r, z) = exp(-r^2 / (2 (width0 + dwidth z)^2)) cos(2 pi r / (period + dperiod z))
+    at bin radii r_i = minr + i (maxr - minr) / steps and planes z = 0..planes-1; [planes][steps] float32."""
+    r = minr + np.arange(steps, dtype=np.float64) * (maxr - minr) / steps
+    z = np.arange(planes, dtype=np.float64)[:, None]
+    w = width0 + dwidth * z
+    p = np.exp(-r[None, :] ** 2 / (2 * w * w)) * np.cos(2 * np.pi * r[None, :] / (period + dperiod * z))
+    return p.astype(np.float32)
+
+
+def normalize_profiles(lut):
+    """Mean 0 / RMS 1 per profile (the normalisation the tracker applies to measured profiles)."""
+    x = lut.astype(np.float64)
+    x = x - x.mean(axis=-1, keepdims=True)
+    x = x / np.sqrt((x * x).mean(axis=-1, keepdims=True))
+    return x.astype(np.float32)
+
+
+def _render(luts, lut_index, pos, cfg, peak, bg, rng, dtype):
+    img = api.generate_images_from_lut(luts, pos, lut_index, cfg.width, cfg.height, cfg.zlut_minradius,
+                                       cfg.zlut_maxradius, spline=True, ovs=4)
+    lam = bg[:, None, None] + peak[:, None, None] * (1.0 + img.astype(np.float64)) * 0.5
+    cnt = rng.poisson(np.maximum(lam, 0.0))
+    if dtype == np.float32:
+        return cnt.astype(np.float32)
+    info = np.iinfo(dtype)
+    return np.clip(cnt, 0, info.max).astype(dtype)
+
+
+def make_workload(name, n=None, seed=None):
+    """Returns dict(settings, mode, luts, rois, jobs, truth) for a named workload.
+
+    plain64:      1 bead, 100 planes x 32 steps, n (default 1000) float32 64x64 ROIs,
+                  centre +-4 px, z in [5, 95], peak 10,000, background 20.
+    realtime100:  256 beads x 16 frames (n default 4096) uint16 100x100 ROIs, per-bead LUT of
+                  200 planes x 64 steps with ring period U[7, 10] px, offset +-5 px,
+                  z in [10, 190], peak U[5000, 20000], background 50.
+    """
+    idx = {"plain64": 0, "realtime100": 1}[name]
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + idx if seed is None else seed))
+    st = api.default_settings(**SETTINGS[name])
+    cfg = api.computed_config(st)
+    if name == "plain64":
+        n = 1000 if n is None else n
+        nbeads, planes = 1, 100
+        periods = np.array([8.0])
+        render = analytic_lut(planes, cfg.zlut_radialsteps, cfg.zlut_minradius, cfg.zlut_maxradius, 8.0)[None]
+        bead = np.zeros(n, np.int32)
+        frame = np.arange(n, dtype=np.uint32)
+        off = rng.uniform(-4, 4, size=(n, 2))
+        z = rng.uniform(5, 95, size=n)
+        peak = np.full(n, 10000.0)
+        bg = np.full(n, 20.0)
+        dtype = np.float32
+    else:
+        nbeads, nframes, planes = 256, 16, 200
+        n = nbeads * nframes if n is None else n
+        periods = rng.uniform(7, 10, size=nbeads)
+        render = np.stack([analytic_lut(planes, cfg.zlut_radialsteps, cfg.zlut_minradius, cfg.zlut_maxradius, p)
+                           for p in periods])
+        k = np.arange(n)
+        bead = (k % nbeads).astype(np.int32)
+        frame = (k // nbeads).astype(np.uint32)
+        off = rng.uniform(-5, 5, size=(n, 2))
+        z = rng.uniform(10, 190, size=n)
+        peak = rng.uniform(5000, 20000, size=n)
+        bg = np.full(n, 50.0)
+        dtype = np.uint16
+    pos = np.stack([cfg.width / 2 + off[:, 0], cfg.height / 2 + off[:, 1], z], axis=1).astype(np.float32)
+    rois = _render(render, bead, pos, cfg, peak, bg, rng, dtype)
+    jobs = api.make_jobs(n, frame=frame, zlut_index=bead)
+    return dict(name=name, settings=st, cfg=cfg, mode=MODE_QI_Z, luts=normalize_profiles(render),
+                render_luts=render, periods=periods, rois=rois, jobs=jobs, truth=pos)
