@@ -14,6 +14,13 @@
 QTRK_API const char* QTrkGetLastError(void);
 QTRK_API void QTrkClearLastError(void);
 
+/* QueuedTracker::SetConfigValue / GetConfigValues (cputrack/QueuedTracker.h:76-78),
+ * which the reference's C ABI leaves commented out (cputrack/qtrk_c_api.h:193-196).
+ * Keys: "batch_size" (ROIs per GPU launch), "num_streams" (batches in flight).
+ * GetConfigValues writes "key=value" lines. */
+QTRK_API void QTrkSetConfigValue(QueuedTracker* qtrk, const char* name, const char* value);
+QTRK_API void QTrkGetConfigValues(QueuedTracker* qtrk, char* dst, int maxStrLen);
+
 /* Name, compute units and clock of the GPU the library runs on. */
 QTRK_API int QTrkGetDeviceInfo(char* dst, int maxStrLen);
 
@@ -45,6 +52,11 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats);
 QTRK_API float QTrkBatchSync(QTrkBatch* b);
 /* Copies n results back to the host (synchronous). 0 on success. */
 QTRK_API int QTrkBatchDownload(QTrkBatch* b, LocalizationResult* out, int n);
+/* Diagnostics: when enabled, each run also records every ROI's best LUT plane
+ * (-1 when no LUT compare ran) and its radial profile [n][zlut_radialsteps];
+ * either output pointer may be NULL.  0 on success. */
+QTRK_API int QTrkBatchEnableDiagnostics(QTrkBatch* b, bool enable);
+QTRK_API int QTrkBatchDownloadDiagnostics(QTrkBatch* b, int* bestPlane, float* profiles, int n);
 
 /*
  * Renders n bead images of width x height from radial LUTs, on the host

@@ -71,6 +71,9 @@ class Oracle:
             "oq_lut_compare": (fl, [P, P, C.c_int, C.c_int, P, P, P]),
             "oq_localize_batch": (None, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P]),
             "oq_queued_run": (C.c_double, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P]),
+            "oq_localize_trace": (None, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P, P, P]),
+            "oq_build_lut": (None, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int]),
+            "oq_finalize_lut": (None, [P, C.c_int, C.c_int]),
             "oq_bspline_weights": (None, [P, fl]),
             "oq_generate_image_from_lut": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl,
                                                   C.c_int, C.c_int]),
@@ -199,6 +202,37 @@ class Oracle:
         self.L.oq_localize_batch(C.byref(cfg), int(mode), None if luts_a is None else _p(luts_a), nl, npl,
                                  None if zc is None else _p(zc), _p(rois), pdt, _p(jobs), len(jobs), _p(out))
         return out
+
+    def localize_trace(self, cfg, mode, luts, rois, jobs, zcmp=None, calib=None):
+        """As localize, plus each ROI's best LUT plane (-1 without Z) and radial profile."""
+        rois = np.ascontiguousarray(rois)
+        pdt = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[rois.dtype]
+        jobs = np.ascontiguousarray(jobs, JOB)
+        out = np.zeros(len(jobs), RESULT)
+        best = np.zeros(len(jobs), np.int32)
+        prof = np.zeros((len(jobs), max(cfg.zlut_radialsteps, 1)), self.fl)
+        luts_a = None if luts is None else np.ascontiguousarray(luts, np.float32)
+        zc = None if zcmp is None else np.ascontiguousarray(zcmp, np.float32)
+        nl, npl = (0, 0) if luts_a is None else luts_a.shape[:2]
+        self.L.oq_localize_trace(C.byref(cfg), int(mode), None if luts_a is None else _p(luts_a), nl, npl,
+                                 None if zc is None else _p(zc), _p(rois), pdt, _p(jobs), len(jobs), _p(out),
+                                 _p(best), _p(prof))
+        return out, best, prof
+
+    def build_lut(self, cfg, images, plane, zlut, known_pos=None, normalize=False, calib=(None, None)):
+        """Adds plane `plane` of every bead's LUT from its image, in place (float32 [count][planes][steps])."""
+        images = np.ascontiguousarray(images)
+        pdt = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[images.dtype]
+        kp = None if known_pos is None else np.ascontiguousarray(known_pos, np.float32)
+        off = None if calib[0] is None else np.ascontiguousarray(calib[0], np.float32)
+        gain = None if calib[1] is None else np.ascontiguousarray(calib[1], np.float32)
+        assert zlut.dtype == np.float32 and zlut.flags.c_contiguous
+        self.L.oq_build_lut(C.byref(cfg), _p(images), pdt, len(images), int(plane), None if kp is None else _p(kp),
+                            1 if normalize else 0, None if off is None else _p(off),
+                            None if gain is None else _p(gain), _p(zlut), zlut.shape[1])
+
+    def finalize_lut(self, zlut):
+        self.L.oq_finalize_lut(_p(zlut), zlut.shape[0] * zlut.shape[1], zlut.shape[2])
 
     def queued_run(self, cfg, mode, luts, rois, jobs, nthreads, zcmp=None):
         """The QueuedCPUTracker restatement with `nthreads` workers; returns (seconds, results)."""

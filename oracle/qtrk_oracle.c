@@ -685,6 +685,93 @@ void oq_localize_batch(const OConfig* cfg, int mode, const float* zluts, int zco
 	free(t.qi_window);
 }
 
+/* ProcessJob for n ROIs with per-ROI best plane and radial profile. */
+void oq_localize_trace(const OConfig* cfg, int mode, const float* zluts, int zcount, int zplanes, const float* zcmp,
+                       const void* rois, int pdt, const OJob* jobs, int n, OResult* out, int* best, fl* profiles)
+{
+	OTracker t;
+	tracker_setup(&t, cfg, mode, zluts, zcount, zplanes, zcmp);
+	OWorker wk;
+	oq_worker_init(&wk, &t);
+	size_t roibytes = (size_t)cfg->width * cfg->height * pdt_bytes(pdt);
+	double* sc = (double*)malloc(sizeof(double) * (size_t)(zplanes > 0 ? zplanes : 1));
+	fl* prof = (fl*)malloc(sizeof(fl) * (size_t)(cfg->zlut_radialsteps > 0 ? cfg->zlut_radialsteps : 1));
+	for (int i = 0; i < n; i++) {
+		OTrace tr;
+		memset(&tr, 0, sizeof(tr));
+		tr.profile = prof;
+		tr.scores = sc;
+		tr.best_plane = -1;
+		oq_process_job(&t, &wk, (const char*)rois + roibytes * i, cfg->width * pdt_bytes(pdt), pdt, &jobs[i], &out[i], &tr);
+		if (best) best[i] = tr.best_plane;
+		if (profiles) memcpy(profiles + (size_t)i * cfg->zlut_radialsteps, prof, sizeof(fl) * cfg->zlut_radialsteps);
+	}
+	free(sc);
+	free(prof);
+	oq_worker_free(&wk);
+	free(t.qi_window);
+}
+
+/* QueuedCPUTracker::BuildLUT, cputrack/QueuedCPUTracker.cpp:580-646: image i of
+ * `images` (contiguous, pitch = width*bpp) belongs to bead i; COM (no background
+ * correction) then QI without the radial window, or the known position (then the
+ * tracker's mean stays 0); its radial profile is added into plane `plane` of bead
+ * i's LUT ([count][planes][zlut_radialsteps]). */
+void oq_build_lut(const OConfig* cfg, const void* images, int pdt, int count, int plane, const float* known_pos,
+                  int normalize, const float* calib_offset, const float* calib_gain, float* zlut, int planes)
+{
+	OTracker t;
+	tracker_setup(&t, cfg, 0, 0, count, planes, 0);
+	t.calib_offset = calib_offset;
+	t.calib_gain = calib_gain;
+	OWorker wk;
+	oq_worker_init(&wk, &t);
+	const int res = cfg->zlut_radialsteps;
+	size_t roibytes = (size_t)cfg->width * cfg->height * pdt_bytes(pdt);
+	fl* tmp = (fl*)malloc(sizeof(fl) * (size_t)(res > 0 ? res : 1));
+	for (int i = 0; i < count; i++) {
+		oq_set_image(&wk, (const char*)images + roibytes * i, cfg->width * pdt_bytes(pdt), pdt);
+		if (calib_offset || calib_gain) {
+			size_t idx = (size_t)cfg->width * cfg->height * i;
+			for (int k = 0; k < wk.w * wk.h; k++) {
+				if (calib_offset && !calib_gain) wk.img[k] = wk.img[k] + calib_offset[idx + k] * 1.0f;
+				if (calib_gain && !calib_offset) wk.img[k] = wk.img[k] * calib_gain[idx + k] * 1.0f;
+				if (calib_gain && calib_offset) wk.img[k] = (wk.img[k] + calib_offset[idx + k] * 1.0f) * calib_gain[idx + k] * 1.0f;
+			}
+		}
+		fl px, py;
+		if (known_pos) {
+			px = known_pos[2 * i];
+			py = known_pos[2 * i + 1];
+		} else {
+			fl cx, cy;
+			int bh;
+			oq_com(&wk, 0, &cx, &cy);
+			oq_qi(&wk, cx, cy, cfg->qi_iterations, cfg->qi_radialsteps, cfg->qi_angstepspq, cfg->qi_angstep_factor,
+			      (fl)cfg->qi_minradius, (fl)cfg->qi_maxradius, 0, &px, &py, &bh, 0);
+		}
+		oq_radial_profile(wk.img, wk.w, wk.h, tmp, res, cfg->zlut_angularsteps, (fl)cfg->zlut_minradius,
+		                  (fl)cfg->zlut_maxradius, px, py, wk.mean, normalize);
+		float* dst = zlut + ((size_t)i * planes + plane) * res;
+		for (int r = 0; r < res; r++) dst[r] = (float)(dst[r] + tmp[r]);
+	}
+	free(tmp);
+	oq_worker_free(&wk);
+	free(t.qi_window);
+}
+
+/* QueuedCPUTracker::FinalizeLUT, cputrack/QueuedCPUTracker.cpp:648-659 */
+void oq_finalize_lut(float* zlut, int rows, int res)
+{
+	fl* tmp = (fl*)malloc(sizeof(fl) * (size_t)(res > 0 ? res : 1));
+	for (int i = 0; i < rows; i++) {
+		for (int r = 0; r < res; r++) tmp[r] = zlut[(size_t)i * res + r];
+		oq_normalize_profile(tmp, res);
+		for (int r = 0; r < res; r++) zlut[(size_t)i * res + r] = (float)tmp[r];
+	}
+	free(tmp);
+}
+
 /* ----------------------------------------------- queued CPU tracker */
 
 /* Restatement of QueuedCPUTracker's job queue (cputrack/QueuedCPUTracker.cpp:37-94,

@@ -144,6 +144,12 @@ double oq_queued_run(const OConfig* cfg, int mode, const float* zluts, int zcoun
                      const float* zcmp, const void* rois, int pdt, const OJob* jobs, int n,
                      int nthreads, OResult* out);
 
+void oq_localize_trace(const OConfig* cfg, int mode, const float* zluts, int zcount, int zplanes, const float* zcmp,
+                       const void* rois, int pdt, const OJob* jobs, int n, OResult* out, int* best, fl* profiles);
+void oq_build_lut(const OConfig* cfg, const void* images, int pdt, int count, int plane, const float* known_pos,
+                  int normalize, const float* calib_offset, const float* calib_gain, float* zlut, int planes);
+void oq_finalize_lut(float* zlut, int rows, int res);
+
 /* Image model and noise-free image generators (cputrack/utils.cpp:325-402, CubicBSpline.h:15-24). */
 void oq_bspline_weights(fl* w, fl t);
 void oq_generate_image_from_lut(fl* img, int w, int h, const float* zlut, int planes, int steps,

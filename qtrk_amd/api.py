@@ -153,6 +153,8 @@ SIGNATURES = {
     "QTrkGetLastError": (C.c_char_p, []),
     "QTrkClearLastError": (None, []),
     "QTrkGetDeviceInfo": (C.c_int, [C.c_char_p, C.c_int]),
+    "QTrkSetConfigValue": (None, [_P, C.c_char_p, C.c_char_p]),
+    "QTrkGetConfigValues": (None, [_P, C.c_char_p, C.c_int]),
     "QTrkBatchCreate": (_P, [_P, C.c_int, C.c_int]),
     "QTrkBatchFree": (None, [_P]),
     "QTrkBatchUpload": (C.c_int, [_P, _P, _P, C.c_int]),
@@ -162,6 +164,8 @@ SIGNATURES = {
     "QTrkBatchRun": (C.c_int, [_P, C.c_int, C.c_int]),
     "QTrkBatchSync": (C.c_float, [_P]),
     "QTrkBatchDownload": (C.c_int, [_P, _P, C.c_int]),
+    "QTrkBatchEnableDiagnostics": (C.c_int, [_P, C.c_bool]),
+    "QTrkBatchDownloadDiagnostics": (C.c_int, [_P, _P, _P, C.c_int]),
     "QTrkGenerateImagesFromLUT": (None, [_FP, C.c_int, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.c_int,
                                          C.c_float, C.c_float, _FP, _IP, C.c_bool, C.c_int]),
 }
@@ -355,6 +359,14 @@ class QTrkInstance:
     def finalize_lut(self):
         self._call("QTrkFinalizeLUT")
 
+    def set_config_value(self, name, value):
+        self._call("QTrkSetConfigValue", str(name).encode(), str(value).encode())
+
+    def get_config_values(self):
+        buf = C.create_string_buffer(4096)
+        self._call("QTrkGetConfigValues", buf, 4096)
+        return dict(l.split("=", 1) for l in buf.value.decode().splitlines() if "=" in l)
+
     def profile_report(self):
         buf = C.create_string_buffer(4096)
         self._call("QTrkGetProfileReport", buf, 4096)
@@ -418,6 +430,18 @@ class DeviceBatch:
         if lib().QTrkBatchDownload(self.h, _ptr(out), int(n)) != 0:
             check()
         return out
+
+    def enable_diagnostics(self, on=True):
+        if lib().QTrkBatchEnableDiagnostics(self.h, bool(on)) != 0:
+            check()
+
+    def diagnostics(self, n):
+        """(best plane per ROI, radial profile per ROI) of the last run."""
+        best = np.zeros(n, np.int32)
+        prof = np.zeros((n, max(self.trk.cfg.zlut_radialsteps, 1)), np.float32)
+        if lib().QTrkBatchDownloadDiagnostics(self.h, _ptr(best), _ptr(prof), int(n)) != 0:
+            check()
+        return best, prof
 
     def free(self):
         if self.h:

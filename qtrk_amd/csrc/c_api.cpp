@@ -6,6 +6,7 @@
 #include "hip_tracker.hpp"
 #include "qtrk_amd.h"
 
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <set>
@@ -222,6 +223,19 @@ QTRK_API void QTrkGetProfileReport(QueuedTracker* q, char* dst, int maxStrLen)
 }
 QTRK_API void QTrkGetWarnings(QueuedTracker* q, char* dst, int maxStrLen) { GUARD(copy_str(q->GetWarnings(), dst, maxStrLen)); }
 
+QTRK_API void QTrkSetConfigValue(QueuedTracker* q, const char* name, const char* value)
+{
+	GUARD(q->SetConfigValue(name ? name : "", value ? value : ""));
+}
+QTRK_API void QTrkGetConfigValues(QueuedTracker* q, char* dst, int maxStrLen)
+{
+	GUARD({
+		std::string s;
+		for (auto& kv : q->GetConfigValues()) s += kv.first + "=" + kv.second + "\n";
+		copy_str(s, dst, maxStrLen);
+	});
+}
+
 QTRK_API void QTrkFreeROIPositions(ROIPosition* data) { delete[] data; }
 QTRK_API ROIPosition* QTrkFindBeads(ImageData*, int, int, int, float, float, int* beadcount, ImageData*)
 {
@@ -235,7 +249,8 @@ QTRK_API ROIPosition* QTrkFindBeads(ImageData*, int, int, int, float, float, int
 struct QTrkBatch {
 	QueuedHIPTracker* trk = nullptr;
 	int maxROIs = 0, pdt = 0;
-	DeviceBuffer rois, jobs, results;
+	DeviceBuffer rois, jobs, results, best, prof;
+	bool diag = false;
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 };
@@ -297,6 +312,10 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 		LocalizeParams P;
 		const int pitch = b->trk->cfg.width * PDT_BytesPerPixel((QTRK_PixelDataType)b->pdt);
 		b->trk->FillParams(P, b->pdt, n, b->rois.p, nullptr, pitch, b->jobs.as<DevJob>(), b->results.as<DevResult>());
+		if (b->diag) {
+			P.best_out = b->best.as<int>();
+			P.prof_out = b->prof.as<float>();
+		}
 		HIP_CHECK(hipEventRecord(b->e0, b->stream));
 		for (int r = 0; r < repeats; r++) HIP_CHECK(launch_localize(P, b->stream));
 		HIP_CHECK(hipEventRecord(b->e1, b->stream));
@@ -320,6 +339,32 @@ QTRK_API int QTrkBatchDownload(QTrkBatch* b, LocalizationResult* out, int n)
 		if (n < 0 || n > b->maxROIs) fail("n out of range");
 		HIP_CHECK(hipStreamSynchronize(b->stream));
 		HIP_CHECK(hipMemcpy(out, b->results.p, sizeof(DevResult) * n, hipMemcpyDeviceToHost));
+	});
+	return ok ? 0 : -1;
+}
+
+QTRK_API int QTrkBatchEnableDiagnostics(QTrkBatch* b, bool enable)
+{
+	bool ok = GUARD({
+		if (enable) {
+			b->best.ensure(sizeof(int) * b->maxROIs);
+			b->prof.ensure(sizeof(float) * b->maxROIs * (size_t)std::max(1, b->trk->cfg.zlut_radialsteps));
+		}
+		b->diag = enable;
+	});
+	return ok ? 0 : -1;
+}
+
+QTRK_API int QTrkBatchDownloadDiagnostics(QTrkBatch* b, int* bestPlane, float* profiles, int n)
+{
+	bool ok = GUARD({
+		if (!b->diag) fail("diagnostics are not enabled");
+		if (n < 0 || n > b->maxROIs) fail("n out of range");
+		HIP_CHECK(hipStreamSynchronize(b->stream));
+		if (bestPlane) HIP_CHECK(hipMemcpy(bestPlane, b->best.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+		if (profiles)
+			HIP_CHECK(hipMemcpy(profiles, b->prof.p, sizeof(float) * n * (size_t)b->trk->cfg.zlut_radialsteps,
+			                    hipMemcpyDeviceToHost));
 	});
 	return ok ? 0 : -1;
 }

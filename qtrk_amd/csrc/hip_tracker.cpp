@@ -244,6 +244,8 @@ void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void*
 	P.height = cfg.height;
 	P.mode = localizeMode;
 	P.com_bgcorrection = cfg.com_bgcorrection;
+	P.gauss_iters = cfg.gauss2D_iterations;
+	P.gauss_sigma = cfg.gauss2D_sigma;
 	P.qi_iters = cfg.qi_iterations;
 	P.qi_nr = cfg.qi_radialsteps;
 	P.qi_A = cfg.qi_angstepspq;
@@ -465,7 +467,7 @@ void QueuedHIPTracker::ClearResults()
 // Flush, wait for idle, then switch (cudatrack/QueuedCUDATracker.cu:420-428).
 void QueuedHIPTracker::SetLocalizationMode(LocMode_t mode)
 {
-	const int supported = LT_QI | LT_LocalizeZ | LT_NormalizeProfile | LT_ClearFirstFourPixels;
+	const int supported = LT_QI | LT_Gaussian2D | LT_LocalizeZ | LT_NormalizeProfile | LT_ClearFirstFourPixels;
 	if (mode & ~supported) {
 		std::ostringstream os;
 		os << "localization mode bits 0x" << std::hex << (mode & ~supported) << " are not implemented";
@@ -487,6 +489,12 @@ void QueuedHIPTracker::SetRadialZLUT(float* data, int count, int planes)
 	const int res = cfg.zlut_radialsteps;
 	SetZLUTBiasCorrection(nullptr, 0, 0);  // the reference drops its bias table here (:368-369)
 	if (count < 0 || planes < 0) fail("negative LUT size");
+	{
+		LocalizeParams P;
+		FillParams(P, 2, 1, nullptr, nullptr, 0, nullptr, nullptr);
+		P.zplanes = planes;
+		if (localize_lds_bytes(P) > lds_limit) fail("too many LUT planes for the per-workgroup LDS");
+	}
 	const size_t total = (size_t)count * planes * res;
 	if (total > 0) {
 		h_zlut.assign(total, 0.0f);

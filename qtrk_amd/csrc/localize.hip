@@ -406,6 +406,58 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		}
 	}
 
+	// ---- 2D Gaussian MLE when QI is off (cputrack/cpu_tracker.cpp:514-589, selected at
+	// cputrack/QueuedCPUTracker.cpp:309-313): pixel-integrated Gaussian, independent Newton
+	// steps for x, y, I0 and bg, float model terms summed in double over the ROI.
+	if (!build && !(P.mode & kModeQI) && (P.mode & kModeGauss2D)) {
+		const float sigma = P.gauss_sigma;
+		float px = comx, py = comy;
+		float I0 = mean * 0.5f * (float)W * (float)H;
+		float bg = mean * 0.5f;
+		const float s2 = 1.0f / (sqrtf(2.0f) * sigma);
+		const float s2pi = 1.0f / (sqrtf(2 * 3.14159265359f) * sigma);
+		const float s2pi3 = 1.0f / (sqrtf(2 * 3.14159265359f) * sigma * sigma * sigma);
+		for (int it = 0; it < P.gauss_iters; it++) {
+			double acc[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+			for (int y = wid; y < H; y += kBlock / 64) {
+				const float Ye0 = ((float)y - py + 0.5f) * s2;
+				const float Ye1 = ((float)y - py - 0.5f) * s2;
+				const float DY = 0.5f * erff(Ye0) - 0.5f * erff(Ye1);
+				const float eY1 = expf(-Ye1 * Ye1), eY0 = expf(-Ye0 * Ye0);
+				for (int x = lane; x < W; x += 64) {
+					const float Xe0 = ((float)x - px + 0.5f) * s2;
+					const float Xe1 = ((float)x - px - 0.5f) * s2;
+					const float DX = 0.5f * erff(Xe0) - 0.5f * erff(Xe1);
+					const float mu = bg + I0 * DX * DY;
+					const float eX1 = expf(-Xe1 * Xe1), eX0 = expf(-Xe0 * Xe0);
+					const float dmx = I0 * s2pi * (eX1 - eX0) * DY;
+					const float dmy = I0 * s2pi * (eY1 - eY0) * DX;
+					const float dmI = DX * DY;
+					const float smp = img[y * W + x];
+					const float f = smp / mu - 1;
+					acc[0] += (double)(dmx * f);
+					acc[1] += (double)(dmy * f);
+					acc[2] += (double)(dmI * f);
+					acc[3] += (double)f;
+					const float d2x = I0 * s2pi3 * (((float)x - px - 0.5f) * eX1 - ((float)x - px + 0.5f) * eX0) * DY;
+					const float d2y = I0 * s2pi3 * (((float)y - py - 0.5f) * eY1 - ((float)y - py + 0.5f) * eY0) * DX;
+					acc[4] += (double)(d2x * f - dmx * dmx * smp / (mu * mu));
+					acc[5] += (double)(d2y * f - dmy * dmy * smp / (mu * mu));
+					acc[6] += (double)(-dmI * dmI * smp / (mu * mu));
+					acc[7] += (double)(-smp / (mu * mu));
+				}
+			}
+			block_sum<8>(acc, red, lane, wid);
+			px = (float)((double)px - red[32] / red[36]);
+			py = (float)((double)py - red[33] / red[37]);
+			I0 = (float)((double)I0 - red[34] / red[38]);
+			bg = (float)((double)bg - red[35] / red[39]);
+			__syncthreads();
+		}
+		cx = px;
+		cy = py;
+	}
+
 	// ---- radial profile (cputrack/utils.cpp:275-320) and normalisation (:230-261)
 	float z = 0.0f;
 	const bool lz = (P.mode & kModeLocalizeZ) || build;
@@ -455,6 +507,8 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		__syncthreads();
 		boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
 		           (cy - P.z_maxr < 0.0f);
+		if (P.prof_out && !build)
+			for (int bin = tid; bin < nb; bin += kBlock) P.prof_out[(size_t)roi * nb + bin] = zprof[bin];
 
 		if (build) {
 			float* dst = P.build_lut + ((size_t)roi * P.zplanes + P.build_plane) * nb;
@@ -497,11 +551,13 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
 				}
 				z = (float)peak_fit<double>(i, P.zplanes, kZW, [&](int k) { return scores[k]; });
+				if (P.best_out) P.best_out[roi] = i;
 			}
 		}
 	}
 
 	if (tid == 0 && !build) {
+		if (P.best_out && !(lz && P.zluts && bead >= 0 && bead < P.zcount && P.zplanes > 0)) P.best_out[roi] = -1;
 		DevResult R;
 		R.job = job;
 		R.x = cx;

@@ -46,6 +46,8 @@ struct LocalizeParams {
 	int width, height;
 	int mode;
 	float com_bgcorrection;
+	int gauss_iters;      // LT_Gaussian2D: Newton iterations and fixed sigma
+	float gauss_sigma;
 
 	// LUT-build variant (QueuedCPUTracker::BuildLUT): COM without background
 	// correction, QI without the radial window, profile accumulated into
@@ -82,6 +84,10 @@ struct LocalizeParams {
 	// per-bead calibration images [zcount][height][width]
 	const float* calib_offset;
 	const float* calib_gain;
+
+	// optional diagnostics per ROI: best LUT plane (-1 without a compare) and the radial profile
+	int* best_out;              // [n] or null
+	float* prof_out;            // [n][z_nb] or null
 };
 
 // Dynamic LDS bytes needed by localize_kernel for these parameters.

@@ -72,6 +72,8 @@ def make_workload(name, n=None, seed=None):
                   200 planes x 64 steps with ring period U[7, 10] px, offset +-5 px,
                   z in [10, 190], peak U[5000, 20000], background 50.
     """
+    if name == "gauss16":
+        return _gauss_workload(n, seed)
     idx = {"plain64": 0, "realtime100": 1}[name]
     rng = np.random.Generator(np.random.PCG64(SEED_BASE + idx if seed is None else seed))
     st = api.default_settings(**SETTINGS[name])
@@ -107,3 +109,32 @@ def make_workload(name, n=None, seed=None):
     jobs = api.make_jobs(n, frame=frame, zlut_index=bead)
     return dict(name=name, settings=st, cfg=cfg, mode=MODE_QI_Z, luts=normalize_profiles(render),
                 render_luts=render, periods=periods, rois=rois, jobs=jobs, truth=pos)
+
+
+def gaussian_spots(pos, sigma, I0, bg, w, h):
+    """Pixel-integrated Gaussian spots, GenerateGaussianSpotImage (cputrack/utils.cpp:393-402), float64."""
+    from scipy.special import erf
+    ed = 1.0 / np.sqrt(2 * sigma * sigma)
+    x = np.arange(w)[None, :] - pos[:, 0:1]
+    y = np.arange(h)[None, :] - pos[:, 1:2]
+    dx = 0.5 * erf((x + 0.5) * ed) - 0.5 * erf((x - 0.5) * ed)
+    dy = 0.5 * erf((y + 0.5) * ed) - 0.5 * erf((y - 0.5) * ed)
+    return bg[:, None, None] + I0[:, None, None] * dy[:, :, None] * dx[:, None, :]
+
+
+def _gauss_workload(n, seed):
+    """gauss16: n (default 65,536) float32 16x16 ROIs of a sigma=1.5 px Gaussian spot, centre +-2 px,
+    I0 U[2000, 20000], background U[5, 50], Poisson noise; LT_Gaussian2D with 10 Newton iterations."""
+    n = 65536 if n is None else n
+    rng = np.random.Generator(np.random.PCG64(SEED_BASE + 3 if seed is None else seed))
+    st = api.default_settings(width=16, height=16, gauss2D_sigma=1.5, gauss2D_iterations=10)
+    cfg = api.computed_config(st)
+    pos = np.stack([8 + rng.uniform(-2, 2, n), 8 + rng.uniform(-2, 2, n)], 1)
+    I0 = rng.uniform(2000, 20000, n)
+    bg = rng.uniform(5, 50, n)
+    lam = gaussian_spots(pos, 1.5, I0, bg, 16, 16)
+    rois = rng.poisson(lam).astype(np.float32)
+    jobs = api.make_jobs(n)
+    truth = np.concatenate([pos, I0[:, None], bg[:, None]], 1).astype(np.float32)
+    return dict(name="gauss16", settings=st, cfg=cfg, mode=api.LT_Gaussian2D, luts=None, rois=rois, jobs=jobs,
+                truth=truth)

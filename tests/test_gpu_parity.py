@@ -1,0 +1,384 @@
+"""Parity of the MI355X path with the CPU oracle, through the C ABI.
+
+Bars (the reference's own tolerances for this path):
+  * ROI extraction, indexing and the best LUT plane: bit-exact;
+  * x/y within 1e-3 px and z within 1e-3 planes of the oracle (in practice the
+    kernel reproduces the oracle's float rounding, so most values are identical);
+  * every result's error against the FP64 oracle is printed.
+"""
+import numpy as np
+import pytest
+
+from qtrk_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+XY_TOL = 1e-3   # px
+Z_TOL = 1e-3    # LUT planes
+
+
+def _tracker(w, mode=None, luts=True):
+    trk = api.QTrkInstance(w["settings"])
+    if luts:
+        trk.set_radial_zlut(w["luts"])
+    trk.set_localization_mode(w["mode"] if mode is None else mode)
+    return trk
+
+
+def _batch(trk, rois, jobs, diag=True):
+    b = api.DeviceBatch(trk, len(jobs), api.PDT_OF_DTYPE[rois.dtype])
+    if diag:
+        b.enable_diagnostics(True)
+    b.upload(rois, jobs)
+    b.run(len(jobs), 1)
+    b.sync()
+    res = b.download(len(jobs))
+    best, prof = b.diagnostics(len(jobs)) if diag else (None, None)
+    b.free()
+    return res, best, prof
+
+
+def _oracle(oracle, w, mode=None, rois=None, jobs=None, luts="default", zcmp=None):
+    cfg = oracle.config_from_bytes(bytes(w["cfg"]))
+    return oracle.localize_trace(cfg, w["mode"] if mode is None else mode, w["luts"] if luts == "default" else luts,
+                                 w["rois"] if rois is None else rois, w["jobs"] if jobs is None else jobs, zcmp=zcmp)
+
+
+def _report(tag, got, ref):
+    n = len(got)
+    parts = []
+    for k in ("comx", "comy", "x", "y", "z"):
+        same = int((got[k] == ref[k]).sum())
+        d = float(np.abs(got[k].astype(np.float64) - ref[k]).max()) if n else 0.0
+        parts.append(f"{k}: {same}/{n} identical, max|d|={d:.2e}")
+    print(f"[{tag}] " + "; ".join(parts), flush=True)
+
+
+def _close(a, b, tol):
+    """|a - b| <= tol elementwise, with NaN matching NaN (a flat ROI normalises 0/0 in both)."""
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all(both_nan | (np.abs(a - b) <= tol)))
+
+
+def _assert_parity(got, ref, best=None, rbest=None, check_z=True):
+    assert np.array_equal(got["frame"], ref["frame"]) and np.array_equal(got["zlutIndex"], ref["zlutIndex"])
+    assert np.array_equal(got["timestamp"], ref["timestamp"])
+    for k in ("x", "y", "comx", "comy"):
+        assert _close(got[k], ref[k], XY_TOL), k
+    assert np.array_equal(got["error"], ref["error"])
+    np.testing.assert_allclose(got["imageMean"], ref["imageMean"], rtol=1e-6)
+    if check_z:
+        assert _close(got["z"], ref["z"], Z_TOL)
+    if best is not None:
+        assert np.array_equal(best, rbest)
+
+
+@pytest.mark.parametrize("name,n", [("plain64", 1000), ("realtime100", 1024)])
+def test_workload_parity(gpu, oracle, oracle64, name, n):
+    w = synth.make_workload(name, n=n)
+    ref, rbest, rprof = _oracle(oracle, w)
+    trk = _tracker(w)
+    got, best, prof = _batch(trk, w["rois"], w["jobs"])
+    trk.free()
+    _report(name, got, ref)
+    _assert_parity(got, ref, best, rbest)
+    np.testing.assert_allclose(prof, rprof, atol=1e-5)
+    r64, _, _ = oracle64.localize_trace(oracle64.config_from_bytes(bytes(w["cfg"])), w["mode"], w["luts"],
+                                        w["rois"], w["jobs"])
+    e = {k: float(np.abs(got[k] - r64[k]).max()) for k in "xyz"}
+    t = {k: float(np.sqrt(np.mean((got[k] - w["truth"][:, i]) ** 2))) for i, k in enumerate("xyz")}
+    print(f"[{name}] max |GPU - FP64 oracle|: x {e['x']:.2e} px, y {e['y']:.2e} px, z {e['z']:.2e} planes; "
+          f"rms vs truth x {t['x']:.4f} y {t['y']:.4f} z {t['z']:.4f}", flush=True)
+    assert e["x"] < XY_TOL and e["y"] < XY_TOL and e["z"] < Z_TOL
+
+
+def test_queue_api_matches_batch_and_is_lifo(gpu, oracle):
+    w = synth.make_workload("plain64", n=300)
+    trk = _tracker(w)
+    batch, _, _ = _batch(trk, w["rois"], w["jobs"], diag=False)
+    got = trk.localize_all(w["rois"], w["jobs"])
+    assert np.array_equal(got, batch)
+    # LIFO fetch like the CPU tracker (cputrack/QueuedCPUTracker.cpp:458-469)
+    for i in range(10):
+        trk.schedule_localization(w["rois"][i], w["jobs"][i])
+    trk.wait_for(10)
+    r = trk.fetch_results(10)
+    assert list(r["frame"]) == list(range(9, -1, -1))
+    assert trk.is_idle() and trk.result_count() == 0
+    trk.free()
+
+
+def test_small_batches_many_streams_block_and_complete(gpu, oracle):
+    """Batches of 8 on 2 streams: ScheduleLocalization blocks while 16 ROIs are queued."""
+    w = synth.make_workload("plain64", n=200)
+    trk = _tracker(w)
+    trk.set_config_value("batch_size", "8")
+    trk.set_config_value("num_streams", "2")
+    assert trk.get_config_values()["batch_size"] == "8"
+    for i in range(200):
+        trk.schedule_localization(w["rois"][i], w["jobs"][i])
+        q, qmax = trk.queue_length()
+        assert qmax == 16 and q <= qmax
+    trk.wait_for(200)
+    r = trk.fetch_results(200)
+    r = r[np.argsort(r["frame"])]
+    assert list(r["frame"]) == list(range(200))
+    ref, _, _ = _oracle(oracle, w)
+    _assert_parity(r, ref)
+    trk.free()
+
+
+def test_pixel_formats_and_pitch(gpu, oracle):
+    w = synth.make_workload("plain64", n=64)
+    scale = w["rois"] / w["rois"].max()
+    for dt, vmax in ((np.uint8, 255), (np.uint16, 60000), (np.float32, 1.0)):
+        rois = (scale * vmax).astype(dt)
+        ref, rbest, _ = _oracle(oracle, w, rois=rois)
+        trk = _tracker(w)
+        got, best, _ = _batch(trk, rois, w["jobs"])
+        _assert_parity(got, ref, best, rbest)
+        # a padded source pitch through ScheduleLocalization gives the same answer
+        padded = np.zeros((rois.shape[0], rois.shape[1], rois.shape[2] + 7), dt)
+        padded[:, :, :rois.shape[2]] = rois
+        for i in range(8):
+            trk.schedule_localization(padded[i][:, :rois.shape[2]], w["jobs"][i])
+        trk.wait_for(8)
+        r = trk.fetch_results(8)
+        r = r[np.argsort(r["frame"])]
+        assert np.array_equal(r, got[:8])
+        trk.free()
+
+
+def test_mode_variants(gpu, oracle):
+    w = synth.make_workload("plain64", n=128)
+    for mode in (api.LT_OnlyCOM, api.LT_QI, api.LT_QI | api.LT_LocalizeZ, api.LT_LocalizeZ | api.LT_NormalizeProfile,
+                 w["mode"] | api.LT_ClearFirstFourPixels):
+        ref, rbest, _ = _oracle(oracle, w, mode=mode)
+        trk = _tracker(w, mode=mode)
+        got, best, _ = _batch(trk, w["rois"], w["jobs"])
+        trk.free()
+        _report(f"mode {mode}", got, ref)
+        _assert_parity(got, ref, best, rbest)
+
+
+def test_radial_weights_and_missing_lut(gpu, oracle):
+    w = synth.make_workload("realtime100", n=256)
+    zcmp = np.linspace(0.2, 1.5, w["cfg"].zlut_radialsteps).astype(np.float32)
+    ref, rbest, _ = _oracle(oracle, w, zcmp=zcmp)
+    trk = _tracker(w)
+    trk.set_radial_weights(zcmp)
+    got, best, _ = _batch(trk, w["rois"], w["jobs"])
+    _assert_parity(got, ref, best, rbest)
+    trk.set_radial_weights(None)
+    # bead indices beyond the LUT: z = 0, no compare (the reference would read out of bounds)
+    jobs = w["jobs"].copy()
+    jobs["zlutIndex"][:32] = 5000
+    ref2, rb2, _ = _oracle(oracle, w, jobs=jobs)
+    got2, b2, _ = _batch(trk, w["rois"], jobs)
+    _assert_parity(got2, ref2, b2, rb2)
+    assert np.all(got2["z"][:32] == 0) and np.all(b2[:32] == -1)
+    trk.free()
+    # no LUT at all: LocalizeZ still computes the profile, z = 0
+    trk = _tracker(w, luts=False)
+    got3, b3, _ = _batch(trk, w["rois"], w["jobs"])
+    assert np.all(got3["z"] == 0) and np.all(b3 == -1)
+    trk.free()
+
+
+def test_edge_cases_constant_and_boundary(gpu, oracle):
+    w = synth.make_workload("plain64", n=16)
+    rois = w["rois"].copy()
+    rois[0] = 123.0                     # constant ROI: COM 0/0 -> centre fallback
+    rois[1] = 0.0                       # all zero
+    rois[2][:, :] = np.arange(64, dtype=np.float32)[None, :]   # ramp: COM far off-centre -> boundary hits
+    rois[3][30:34, 1:3] = 50000         # bright spot at the edge
+    ref, rbest, _ = _oracle(oracle, w, rois=rois)
+    trk = _tracker(w)
+    got, best, _ = _batch(trk, rois, w["jobs"])
+    trk.free()
+    _report("edge", got, ref)
+    _assert_parity(got, ref, best, rbest)
+    assert (got["comx"][0], got["comy"][0]) == (32.0, 32.0)
+    assert got["error"].sum() >= 1
+
+
+def test_calibration_images(gpu, oracle):
+    w = synth.make_workload("realtime100", n=64)
+    cfg = w["cfg"]
+    nb = w["luts"].shape[0]
+    rng = np.random.default_rng(7)
+    off = rng.normal(0, 5, (nb, cfg.height, cfg.width)).astype(np.float32)
+    gain = rng.normal(1, 0.05, (nb, cfg.height, cfg.width)).astype(np.float32)
+    o_cfg = oracle.config_from_bytes(bytes(cfg))
+    # (v + offset) * gain with unit factors (cputrack/cpu_tracker.cpp:108-111), applied to the
+    # float pixels before the pipeline -- the same operations in the same order as the oracle's
+    rois_f = w["rois"].astype(np.float32)
+    corr = (rois_f + off[w["jobs"]["zlutIndex"]]) * gain[w["jobs"]["zlutIndex"]]
+    ref, rbest, _ = oracle.localize_trace(o_cfg, w["mode"], w["luts"], corr.astype(np.float32), w["jobs"])
+    trk = _tracker(w)
+    trk.set_pixel_calibration_images(off, gain)
+    got, best, _ = _batch(trk, w["rois"], w["jobs"])
+    trk.free()
+    _assert_parity(got, ref, best, rbest)
+
+
+def test_schedule_frame_extraction(gpu):
+    w = synth.make_workload("plain64", n=12)
+    H, W = 300, 410
+    frame = np.zeros((H, W), np.float32)
+    pos = []
+    for i in range(12):
+        x, y = 10 + (i % 4) * 95, 20 + (i // 4) * 90
+        frame[y:y + 64, x:x + 64] = w["rois"][i]
+        pos.append((x, y))
+    pos += [(-1, 5), (W - 63, 0), (0, H - 63)]     # outside the frame: skipped
+    trk = _tracker(w)
+    job = np.zeros(1, api.JOB_DTYPE)[0]
+    job["frame"] = 77
+    job["zlutIndex"] = 0
+    n = trk.schedule_frame(frame, pos, job)
+    assert n == 12
+    trk.wait_for(12)
+    r = trk.fetch_results(12)
+    r = r[np.argsort(r["zlutIndex"])]
+    assert list(r["zlutIndex"]) == list(range(12)) and np.all(r["frame"] == 77)
+    jobs = w["jobs"].copy()
+    jobs["frame"] = 77
+    jobs["zlutIndex"] = np.arange(12)
+    luts = np.repeat(w["luts"], 12, axis=0)
+    trk2 = api.QTrkInstance(w["settings"])
+    trk2.set_radial_zlut(luts)
+    trk2.set_localization_mode(w["mode"])
+    ref, _, _ = _batch(trk2, w["rois"], jobs, diag=False)
+    trk2.free()
+    trk.free()
+    # the single LUT applies to bead 0 only; compare the xy part for all, z for bead 0
+    assert np.array_equal(r["x"], ref["x"]) and np.array_equal(r["y"], ref["y"])
+    assert r["z"][0] == ref["z"][0]
+
+
+def test_build_lut_matches_oracle(gpu, oracle):
+    w = synth.make_workload("plain64", n=8)
+    cfg = w["cfg"]
+    o_cfg = oracle.config_from_bytes(bytes(cfg))
+    planes, count = 20, 3
+    rng = np.random.default_rng(11)
+    lut_src = w["render_luts"][0]
+    for known, norm in ((False, True), (True, False)):
+        trk = api.QTrkInstance(w["settings"])
+        trk.set_radial_zlut((count, planes))
+        trk.begin_lut(api.BUILDLUT_NORMALIZE if norm else 0)
+        ref = np.zeros((count, planes, cfg.zlut_radialsteps), np.float32)
+        for p in range(planes):
+            posn = np.stack([32 + rng.uniform(-2, 2, count), 32 + rng.uniform(-2, 2, count),
+                             np.full(count, 5 + 4.5 * p)], 1).astype(np.float32)
+            imgs = api.generate_images_from_lut(lut_src[None], posn, np.zeros(count, np.int32), 64, 64,
+                                                cfg.zlut_minradius, cfg.zlut_maxradius)
+            imgs = (1000 + 800 * imgs).astype(np.float32)
+            kp = posn[:, :2] if known else None
+            trk.build_lut(imgs, p, kp)
+            oracle.build_lut(o_cfg, imgs, p, ref, known_pos=kp, normalize=norm)
+        pre = trk.get_radial_zlut()
+        assert np.abs(pre - ref).max() <= 1e-4 * max(1.0, np.abs(ref).max())
+        trk.finalize_lut()
+        oracle.finalize_lut(ref)
+        got = trk.get_radial_zlut()
+        np.testing.assert_allclose(got, ref, atol=1e-4)
+        trk.free()
+
+
+def test_zeroed_lut_and_sizes(gpu):
+    w = synth.make_workload("plain64", n=4)
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_radial_zlut((3, 17))
+    assert trk.get_radial_zlut_size() == (3, 17, w["cfg"].zlut_radialsteps)
+    assert np.all(trk.get_radial_zlut() == 0)
+    trk.set_radial_zlut(w["luts"])
+    assert np.array_equal(trk.get_radial_zlut(), w["luts"])
+    trk.free()
+
+
+def test_full_batch_properties(gpu, oracle):
+    """The full 4,096-ROI batch: a seeded sample against the oracle, and size-independent checks for all."""
+    w = synth.make_workload("realtime100")
+    trk = _tracker(w)
+    got, best, _ = _batch(trk, w["rois"], w["jobs"])
+    # the same batch twice gives identical results (no cross-ROI interference, no races)
+    again, best2, _ = _batch(trk, w["rois"], w["jobs"])
+    trk.free()
+    assert np.array_equal(got, again) and np.array_equal(best, best2)
+    assert np.array_equal(got["frame"], w["jobs"]["frame"]) and np.array_equal(got["zlutIndex"], w["jobs"]["zlutIndex"])
+    assert np.all(np.isfinite(got["x"])) and np.all(np.isfinite(got["z"]))
+    assert np.all((best >= 0) & (best < 200))
+    ex, ez = np.abs(got["x"] - w["truth"][:, 0]), np.abs(got["z"] - w["truth"][:, 2])
+    # the oracle on the same batch: rms 0.056 px, 99.5th percentile 0.31 px (QI has rare outliers)
+    assert np.sqrt(np.mean(ex ** 2)) < 0.08 and np.percentile(ex, 99.5) < 0.5
+    assert np.sqrt(np.mean(ez ** 2)) < 1.0
+    sel = np.random.default_rng(3).choice(len(got), 300, replace=False)
+    ref, rbest, _ = _oracle(oracle, w, rois=w["rois"][sel], jobs=w["jobs"][sel])
+    _assert_parity(got[sel], ref, best[sel], rbest)
+
+
+def test_large_roi_default_settings(gpu, oracle):
+    """150x150 ROIs with the reference defaults: QI 256 radial steps (512-point FFT)."""
+    st = api.default_settings(width=150, height=150, qi_iterations=3)
+    cfg = api.computed_config(st)
+    assert cfg.qi_radialsteps == 256
+    lut = synth.analytic_lut(40, cfg.zlut_radialsteps, cfg.zlut_minradius, cfg.zlut_maxradius, 9.0)
+    n = 16
+    rng = np.random.default_rng(4)
+    pos = np.stack([75 + rng.uniform(-3, 3, n), 75 + rng.uniform(-3, 3, n), rng.uniform(5, 35, n)], 1)
+    rois = (100 + 2000 * (1 + api.generate_images_from_lut(lut[None], pos, np.zeros(n, np.int32), 150, 150,
+                                                           cfg.zlut_minradius, cfg.zlut_maxradius))).astype(np.float32)
+    jobs = api.make_jobs(n)
+    luts = synth.normalize_profiles(lut)[None]
+    mode = synth.MODE_QI_Z
+    ref, rbest, _ = oracle.localize_trace(oracle.config_from_bytes(bytes(cfg)), mode, luts, rois, jobs)
+    trk = api.QTrkInstance(st)
+    trk.set_radial_zlut(luts)
+    trk.set_localization_mode(mode)
+    got, best, _ = _batch(trk, rois, jobs)
+    trk.free()
+    _report("150x150", got, ref)
+    _assert_parity(got, ref, best, rbest)
+
+
+def test_unsupported_mode_raises(gpu):
+    w = synth.make_workload("plain64", n=2)
+    trk = api.QTrkInstance(w["settings"])
+    with pytest.raises(api.QTrkError):
+        trk.set_localization_mode(api.LT_FourierLUT | api.LT_QI)
+    trk.free()
+
+
+def test_gaussian2d_mle(gpu, oracle, oracle64):
+    """LT_Gaussian2D (cputrack/cpu_tracker.cpp:514-589) on sigma=1.5 spots in 16x16 ROIs.
+
+    The reference's independent (diagonal) Newton steps do not converge on every ROI, so the
+    result is sensitive to rounding: the FP32 and FP64 oracles themselves differ by up to ~0.06 px
+    on this workload.  The GPU takes erff/expf from the device math library instead of the host
+    CRT, so its agreement with the FP32 oracle is bounded by that same spread; the bulk agrees
+    to 1e-3 px and the precision against the truth is the same."""
+    w = synth.make_workload("gauss16", n=2048)
+    ref, _, _ = _oracle(oracle, w, luts=None)
+    r64, _, _ = oracle64.localize_trace(oracle64.config_from_bytes(bytes(w["cfg"])), w["mode"], None,
+                                        w["rois"], w["jobs"])
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_localization_mode(api.LT_Gaussian2D)
+    got, _, _ = _batch(trk, w["rois"], w["jobs"], diag=False)
+    trk.free()
+    _report("gauss16", got, ref)
+    assert np.array_equal(got["frame"], ref["frame"])
+    assert np.array_equal(got["comx"], ref["comx"]) and np.array_equal(got["comy"], ref["comy"])
+    for k in "xy":
+        d = np.abs(got[k] - ref[k])
+        spread = np.abs(ref[k] - r64[k]).max()
+        print(f"[gauss16] {k}: p99 |GPU-oracle| {np.percentile(d, 99):.2e}, max {d.max():.2e}; "
+              f"oracle FP32-FP64 spread {spread:.2e}")
+        assert np.percentile(d, 99) <= XY_TOL
+        assert d.max() <= spread + XY_TOL
+    e_gpu = np.hypot(got["x"] - w["truth"][:, 0], got["y"] - w["truth"][:, 1])
+    e_ref = np.hypot(ref["x"] - w["truth"][:, 0], ref["y"] - w["truth"][:, 1])
+    assert abs(np.sqrt(np.mean(e_gpu ** 2)) - np.sqrt(np.mean(e_ref ** 2))) < 1e-3
