@@ -183,6 +183,8 @@ def lib(path=None):
         raise QTrkError(f"{p} not found: run __graft_entry__.build() (hipcc) first")
     L = C.CDLL(p)
     for name, (res, args) in SIGNATURES.items():
+        if not hasattr(L, name):  # an older build: calling it raises AttributeError
+            continue
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args

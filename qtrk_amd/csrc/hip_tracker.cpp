@@ -149,6 +149,7 @@ void QueuedHIPTracker::BuildTables()
 
 	// FFT of length 2*nr: stages, leaf order and twiddles (cputrack/kissfft.h:11-16)
 	fft_n = 2 * nr;
+	if (fft_n > 4 * kBlock / 2) fail("qi_radialsteps > 256 is not supported");
 	int rem[kMaxFFTStages];
 	fft_nst = fft_factor(fft_n, fft_radix, rem);
 	int fs = 1;
@@ -277,6 +278,9 @@ void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void*
 	P.cmp_out = cmp_enabled && zcount > 0 ? d_cmp.as<float>() : nullptr;
 	P.calib_offset = has_off ? d_calib_off.as<float>() : nullptr;
 	P.calib_gain = has_gain ? d_calib_gain.as<float>() : nullptr;
+	// The ROI is staged in LDS as float: measured faster than u16 staging, whose
+	// unaligned pixel-pair reads cost more than the occupancy they buy.
+	P.lds_pix = 2;
 }
 
 // ------------------------------------------------------------- queueing

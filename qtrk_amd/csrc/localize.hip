@@ -30,9 +30,20 @@ __constant__ float kQIW[7] = { 0.14f, 0.5f, 0.85f, 1.0f, 0.85f, 0.5f, 0.14f };
 __constant__ double kZW[7] = { (double)0.15f, (double)0.5f, (double)0.85f, (double)1.0f,
                                (double)0.85f, (double)0.5f, (double)0.15f };
 
+// LDS map.  The ROI, the radial profile and the reduction scratch live for the
+// whole kernel; the QI tables/buffers, the radial-profile partial sums and the
+// LUT scores are used in disjoint phases and share one region.
 struct LdsLayout {
-	int img, qprof, fft, tw, perm, zprof, zpart, zcnt, scores, red, st, total;
+	int img, zprof, red, st;
+	int qdir, qprof, fft, tw, perm;  // QI phase
+	int zdir, zpart, zcnt;           // radial-profile phase
+	int scores;                      // LUT phase
+	int total;
 };
+
+// Bytes per ROI pixel as staged in LDS: the source format when it converts to
+// float exactly and no calibration rewrites it (u8, u16), float otherwise.
+__host__ __device__ inline int lds_pix_bytes(const LocalizeParams& P) { return P.lds_pix == 0 ? 1 : P.lds_pix == 1 ? 2 : 4; }
 
 __host__ __device__ inline int align16(int x) { return (x + 15) & ~15; }
 
@@ -48,18 +59,29 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P)
 	LdsLayout L;
 	int o = 0;
 	const int N = P.fft_n > 0 ? P.fft_n : 0;
-	L.img = o;    o += align16(P.width * P.height * 4);
-	L.qprof = o;  o += align16(4 * P.qi_nr * 4);
-	L.fft = o;    o += align16(6 * N * 8);
-	L.tw = o;     o += align16(2 * N * 8);
-	L.perm = o;   o += align16(N * 4);
+	L.img = o;    o += align16(P.width * P.height * lds_pix_bytes(P));
 	L.zprof = o;  o += align16(P.z_nb * 4);
-	const int nch = z_chunks(P);
-	L.zpart = o;  o += align16(nch * P.z_nb * 8);
-	L.zcnt = o;   o += align16(nch * P.z_nb * 4);
-	L.scores = o; o += align16(P.zplanes * 8);
 	L.red = o;    o += align16(64 * 8);
 	L.st = o;     o += align16(16 * 4);
+	const int u = o;
+	// QI phase: directions, quadrant profiles, 4 FFT buffers, twiddles, leaf order
+	int q = u;
+	L.qdir = q;   q += align16(P.qi_A * 8);
+	L.qprof = q;  q += align16(4 * P.qi_nr * 4);
+	L.fft = q;    q += align16(4 * N * 8);
+	L.tw = q;     q += align16(2 * N * 8);
+	L.perm = q;   q += align16(N * 4);
+	// radial-profile phase
+	int z = u;
+	const int nch = z_chunks(P);
+	L.zdir = z;   z += align16(P.z_A * 8);
+	L.zpart = z;  z += align16(nch * P.z_nb * 8);
+	L.zcnt = z;   z += align16(nch * P.z_nb * 4);
+	// LUT phase
+	L.scores = u;
+	int s = u + align16(P.zplanes * 8);
+	o = q > z ? q : z;
+	o = o > s ? o : s;
 	L.total = o;
 	return L;
 }
@@ -73,20 +95,43 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
-// Bilinear sample with the reference's "outside" rule (cputrack/utils.h:39-58):
-// trunc(x) in [0, W-2] <=> -1 < x < W-1 (for W >= 2); the range form also
-// rejects NaN, as the reference's x86 conversion does.
-__device__ __forceinline__ bool bilinear(const float* img, int W, int H, float x, float y, float& v)
+// Bilinear samples with the reference's "outside" rule (cputrack/utils.h:39-58):
+// a sample is inside iff trunc(x) is in [0, W-2] and trunc(y) in [0, H-2].  The
+// coordinate is clamped to [-2, W] first (a no-op for every inside sample) so the
+// truncating conversion never overflows; the caller passes ok = false for a NaN
+// centre, whose samples the reference's x86 conversion also puts outside.
+// Outside samples read pixel (0,0) and are masked to +0.0, which leaves the sum
+// unchanged (it starts at +0.0, so it never holds -0.0) and keeps every load
+// unconditional; the additions run in the reference's order.
+template <typename T, typename Acc, int U>
+__device__ __forceinline__ void sample_n_acc(const T* img, int W, float Wf, float Hf, unsigned Wm2, unsigned Hm2,
+                                             bool ok, const float (&x)[U], const float (&y)[U], Acc& sum, int& cnt)
 {
-	if (!(x > -1.0f && y > -1.0f && x < (float)(W - 1) && y < (float)(H - 1))) return false;
-	const int rx = (int)x, ry = (int)y;
-	const float* p = img + ry * W + rx;
-	const float v00 = p[0], v10 = p[1], v01 = p[W], v11 = p[W + 1];
-	const float fx = x - (float)rx;
-	const float v0 = v00 + (v10 - v00) * fx;
-	const float v1 = v01 + (v11 - v01) * fx;
-	v = v0 + (v1 - v0) * (y - (float)ry);
-	return true;
+	int in[U], rx[U], ry[U];
+	float v00[U], v10[U], v01[U], v11[U];
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		rx[u] = (int)__builtin_amdgcn_fmed3f(x[u], -2.0f, Wf);
+		ry[u] = (int)__builtin_amdgcn_fmed3f(y[u], -2.0f, Hf);
+		in[u] = ok && (unsigned)rx[u] <= Wm2 && (unsigned)ry[u] <= Hm2;
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const T* p = img + (in[u] ? ry[u] * W + rx[u] : 0);
+		v00[u] = (float)p[0];
+		v10[u] = (float)p[1];
+		v01[u] = (float)p[W];
+		v11[u] = (float)p[W + 1];
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const float fx = x[u] - (float)rx[u];
+		const float a0 = v00[u] + (v10[u] - v00[u]) * fx;
+		const float a1 = v01[u] + (v11[u] - v01[u]) * fx;
+		const float v = a0 + (a1 - a0) * (y[u] - (float)ry[u]);
+		sum += (Acc)__int_as_float(__float_as_int(v) & -in[u]);
+		cnt += in[u];
+	}
 }
 
 // Weighted quadratic least squares through n <= 7 points (x = i - iMax),
@@ -233,11 +278,14 @@ __device__ __forceinline__ float qi_input(const float* qp, int nr, int axis, int
 	return axis == 0 ? q0[r] + q3[r] : q0[r] + q1[r];
 }
 
+template <typename SPix>
 __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 {
 	extern __shared__ __align__(16) unsigned char lds[];
 	const LdsLayout L = lds_layout(P);
-	float* img = (float*)(lds + L.img);
+	SPix* img = (SPix*)(lds + L.img);
+	float2* qdir = (float2*)(lds + L.qdir);
+	float2* zdir = (float2*)(lds + L.zdir);
 	float* qprof = (float*)(lds + L.qprof);
 	float2* fft = (float2*)(lds + L.fft);
 	float2* tw = (float2*)(lds + L.tw);
@@ -277,11 +325,13 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				if (off && !gain) v = v + off[i] * 1.0f;
 				else if (gain && !off) v = v * gain[i] * 1.0f;
 				else if (gain && off) v = (v + off[i] * 1.0f) * gain[i] * 1.0f;
-				img[i] = v;
+				img[i] = (SPix)v;  // exact: SPix is float whenever v is not an integer of the source range
 			}
 		}
 	}
 	__syncthreads();
+	const float Wf = (float)W, Hf = (float)H;
+	const unsigned Wm2 = (unsigned)(W - 2), Hm2 = (unsigned)(H - 2);
 
 	// ---- mean, stdev and background-corrected COM (cputrack/cpu_tracker.cpp:681-729)
 	float mean = 0.0f, comx, comy;
@@ -292,7 +342,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	} else {
 		double m1[2] = { 0.0, 0.0 };
 		for (int i = tid; i < npx; i += kBlock) {
-			const float v = img[i];
+			const float v = (float)img[i];
 			m1[0] += (double)v;
 			m1[1] += (double)(float)(v * v);
 		}
@@ -306,7 +356,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		double m2[3] = { 0.0, 0.0, 0.0 };
 		for (int y = wid; y < H; y += kBlock / 64) {
 			for (int x = lane; x < W; x += 64) {
-				const float d = fabsf(img[y * W + x] - mean) - bgs;
+				const float d = fabsf((float)img[y * W + x] - mean) - bgs;
 				const float v = (0.0f < d) ? d : 0.0f;
 				m2[0] += (double)v;
 				m2[1] += (double)x * (double)v;
@@ -337,26 +387,43 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			tw[N + i] = P.tw_inv[i];
 			perm[i] = P.fft_perm[i];
 		}
+		for (int j = tid; j < P.qi_A; j += kBlock) qdir[j] = P.qi_dirs[j];
+		__syncthreads();
 		for (int k = 0; k < P.qi_iters; k++) {
 			boundary = (cx + maxr >= (float)W) || (cx - maxr < 0.0f) || (cy + maxr >= (float)H) || (cy - maxr < 0.0f);
 			const int steps = P.qi_iter_steps[k], stride = P.qi_iter_stride[k];
-			for (int p = tid; p < 4 * nr; p += kBlock) {
-				const int q = p / nr, ri = p - q * nr;
-				const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
-				const float sy = (q < 2) ? 1.0f : -1.0f;
-				const float r = minr + rstep * (float)ri;
-				float sum = 0.0f;
-				int cnt = 0;
-				for (int a = 0; a < steps; a++) {
-					const float2 d = P.qi_dirs[stride * a];
-					const float x = cx + (sx * d.x) * r;
-					const float y = cy + (sy * d.y) * r;
-					float v;
-					if (bilinear(img, W, H, x, y, v)) { sum += v; cnt++; }
+			const bool ok = !isnan(cx) && !isnan(cy);
+			// uniform loop over (quadrant, radius) slots: the angle index is the same in
+			// every lane, so the direction reads from LDS are broadcasts
+			for (int base = 0; base < 4 * nr; base += kBlock) {
+				const int p = base + tid;
+				if (p < 4 * nr) {
+					const int q = p / nr, ri = p - q * nr;
+					const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
+					const float sy = (q < 2) ? 1.0f : -1.0f;
+					const float r = minr + rstep * (float)ri;
+					float sum = 0.0f;
+					int cnt = 0;
+					int a = 0;
+					for (; a + 4 <= steps; a += 4) {
+						float x[4], y[4];
+#pragma unroll
+						for (int u = 0; u < 4; u++) {
+							const float2 d = qdir[stride * (a + u)];
+							x[u] = cx + (sx * d.x) * r;
+							y[u] = cy + (sy * d.y) * r;
+						}
+						sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+					}
+					for (; a < steps; a++) {
+						const float2 d = qdir[stride * a];
+						const float x[1] = { cx + (sx * d.x) * r }, y[1] = { cy + (sy * d.y) * r };
+						sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+					}
+					float val = cnt >= 4 ? sum / (float)cnt : mean;
+					if (window) val *= window[ri];
+					qprof[p] = val;
 				}
-				float val = cnt >= 4 ? sum / (float)cnt : mean;
-				if (window) val *= window[ri];
-				qprof[p] = val;
 			}
 			__syncthreads();
 			// leaves of the 4 forward transforms: X, reversed X, Y, reversed Y
@@ -371,19 +438,33 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			}
 			__syncthreads();
 			fft_stages(fft, 4, N, P, tw, 1.0f, wid, lane);
-			// F(p) * conj(F(rev p)), written in leaf order for the inverse transforms
-			for (int t = tid; t < 2 * N; t += kBlock) {
-				const int ax = t >= N ? 1 : 0;
-				const int j = t - ax * N;
-				const int s = perm[j];
-				const float2 a = fft[(2 * ax) * N + s];
-				const float2 b = fft[(2 * ax + 1) * N + s];
-				fft[(4 + ax) * N + j] = cmul(a, make_float2(b.x, -b.y));
+			// F(p) * conj(F(rev p)) for X and Y, held in registers until every thread has
+			// read the forward spectra, then written in leaf order into buffers 0 and 1
+			// for the inverse transforms (2N / kBlock <= 4 values per thread)
+			{
+				float2 prod[4];
+#pragma unroll
+				for (int i = 0; i < 4; i++) {
+					const int t = tid + i * kBlock;
+					if (t < 2 * N) {
+						const int ax = t >= N ? 1 : 0;
+						const int s = perm[t - ax * N];
+						const float2 a = fft[(2 * ax) * N + s];
+						const float2 b = fft[(2 * ax + 1) * N + s];
+						prod[i] = cmul(a, make_float2(b.x, -b.y));
+					}
+				}
+				__syncthreads();
+#pragma unroll
+				for (int i = 0; i < 4; i++) {
+					const int t = tid + i * kBlock;
+					if (t < 2 * N) fft[t] = prod[i];  // t < N: X into buffer 0, else Y into buffer 1
+				}
 			}
 			__syncthreads();
-			fft_stages(fft + 4 * N, 2, N, P, tw + N, -1.0f, wid, lane);
+			fft_stages(fft, 2, N, P, tw + N, -1.0f, wid, lane);
 			if (wid < 2) {
-				const float2* G = fft + (4 + wid) * N;
+				const float2* G = fft + wid * N;
 				float bv = 0.0f;
 				int bi = 0x7fffffff;
 				for (int x = lane; x < N; x += 64) {
@@ -433,7 +514,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					const float dmx = I0 * s2pi * (eX1 - eX0) * DY;
 					const float dmy = I0 * s2pi * (eY1 - eY0) * DX;
 					const float dmI = DX * DY;
-					const float smp = img[y * W + x];
+					const float smp = (float)img[y * W + x];
 					const float f = smp / mu - 1;
 					acc[0] += (double)(dmx * f);
 					acc[1] += (double)(dmy * f);
@@ -464,21 +545,36 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	if (lz && P.z_nb > 0) {
 		const int nb = P.z_nb, nch = z_chunks(P);
 		const float rstep = (P.z_maxr - P.z_minr) / (float)nb;
-		for (int p = tid; p < nch * nb; p += kBlock) {
-			const int ch = p / nb, bin = p - ch * nb;
-			const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
-			const float r = P.z_minr + rstep * (float)bin;
-			double s = 0.0;
-			int c = 0;
-			for (int a = a0; a < a1; a++) {
-				const float2 d = P.z_dirs[a];
-				const float x = cx + d.x * r;
-				const float y = cy + d.y * r;
-				float v;
-				if (bilinear(img, W, H, x, y, v)) { s += (double)v; c++; }
+		const bool ok = !isnan(cx) && !isnan(cy);
+		for (int j = tid; j < P.z_A; j += kBlock) zdir[j] = P.z_dirs[j];
+		__syncthreads();
+		for (int base = 0; base < nch * nb; base += kBlock) {
+			const int p = base + tid;
+			if (p < nch * nb) {
+				const int ch = p / nb, bin = p - ch * nb;
+				const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
+				const float r = P.z_minr + rstep * (float)bin;
+				double s = 0.0;
+				int c = 0;
+				int a = a0;
+				for (; a + 4 <= a1; a += 4) {
+					float x[4], y[4];
+#pragma unroll
+					for (int u = 0; u < 4; u++) {
+						const float2 d = zdir[a + u];
+						x[u] = cx + d.x * r;
+						y[u] = cy + d.y * r;
+					}
+					sample_n_acc<SPix, double, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+				}
+				for (; a < a1; a++) {
+					const float2 d = zdir[a];
+					const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+					sample_n_acc<SPix, double, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+				}
+				zpart[p] = s;
+				zcnt[p] = c;
 			}
-			zpart[p] = s;
-			zcnt[p] = c;
 		}
 		__syncthreads();
 		for (int bin = tid; bin < nb; bin += kBlock) {
@@ -577,10 +673,14 @@ hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
 	const size_t lds = localize_lds_bytes(p);
 	static bool attr_done = false;
 	if (!attr_done) {
-		hipFuncSetAttribute((const void*)localize_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		hipFuncSetAttribute((const void*)localize_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		hipFuncSetAttribute((const void*)localize_kernel<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+		hipFuncSetAttribute((const void*)localize_kernel<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
 		attr_done = true;
 	}
-	hipLaunchKernelGGL(localize_kernel, dim3(p.n), dim3(kBlock), lds, stream, p);
+	if (p.lds_pix == 0) hipLaunchKernelGGL(localize_kernel<uint8_t>, dim3(p.n), dim3(kBlock), lds, stream, p);
+	else if (p.lds_pix == 1) hipLaunchKernelGGL(localize_kernel<uint16_t>, dim3(p.n), dim3(kBlock), lds, stream, p);
+	else hipLaunchKernelGGL(localize_kernel<float>, dim3(p.n), dim3(kBlock), lds, stream, p);
 	return hipGetLastError();
 }
 

@@ -40,6 +40,7 @@ struct LocalizeParams {
 	const int64_t* roi_offset;
 	int pitch;
 	int pdt;  // 0 u8, 1 u16, 2 f32
+	int lds_pix;  // pixel format staged in LDS (same codes): the source format unless calibration is on
 	int n;
 	const DevJob* jobs;
 	DevResult* results;
