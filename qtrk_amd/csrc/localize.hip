@@ -66,7 +66,7 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P)
 	const int u = o;
 	// QI phase: directions, quadrant profiles, 4 FFT buffers, twiddles, leaf order
 	int q = u;
-	L.qdir = q;   q += align16(P.qi_A * 8);
+	L.qdir = q;   q += align16(4 * P.qi_A * 8);  // per quadrant, signs applied
 	L.qprof = q;  q += align16(4 * P.qi_nr * 4);
 	L.fft = q;    q += align16(4 * N * 8);
 	L.tw = q;     q += align16(2 * N * 8);
@@ -95,14 +95,23 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b)
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
+// Truncation toward zero with the hardware's defined saturation (out-of-range
+// values clamp to INT_MIN/INT_MAX, NaN gives 0); unlike a C++ cast it has no
+// undefined behaviour the optimiser could exploit.
+__device__ __forceinline__ int trunc_sat(float x)
+{
+	int r;
+	asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+	return r;
+}
+
 // Bilinear samples with the reference's "outside" rule (cputrack/utils.h:39-58):
-// a sample is inside iff trunc(x) is in [0, W-2] and trunc(y) in [0, H-2].  The
-// coordinate is clamped to [-2, W] first (a no-op for every inside sample) so the
-// truncating conversion never overflows; the caller passes ok = false for a NaN
-// centre, whose samples the reference's x86 conversion also puts outside.
-// Outside samples read pixel (0,0) and are masked to +0.0, which leaves the sum
-// unchanged (it starts at +0.0, so it never holds -0.0) and keeps every load
-// unconditional; the additions run in the reference's order.
+// a sample is inside iff trunc(x) is in [0, W-2] and trunc(y) in [0, H-2] (the
+// saturating conversion keeps huge coordinates out of range); the caller passes
+// ok = false for a NaN centre, whose samples the reference's x86 conversion also
+// puts outside.  Outside samples read pixel (0,0) and are masked to +0.0, which
+// leaves the sum unchanged (it starts at +0.0, so it never holds -0.0) and keeps
+// every load unconditional; the additions run in the reference's order.
 template <typename T, typename Acc, int U>
 __device__ __forceinline__ void sample_n_acc(const T* img, int W, float Wf, float Hf, unsigned Wm2, unsigned Hm2,
                                              bool ok, const float (&x)[U], const float (&y)[U], Acc& sum, int& cnt)
@@ -111,13 +120,14 @@ __device__ __forceinline__ void sample_n_acc(const T* img, int W, float Wf, floa
 	float v00[U], v10[U], v01[U], v11[U];
 #pragma unroll
 	for (int u = 0; u < U; u++) {
-		rx[u] = (int)__builtin_amdgcn_fmed3f(x[u], -2.0f, Wf);
-		ry[u] = (int)__builtin_amdgcn_fmed3f(y[u], -2.0f, Hf);
+		rx[u] = trunc_sat(x[u]);
+		ry[u] = trunc_sat(y[u]);
 		in[u] = ok && (unsigned)rx[u] <= Wm2 && (unsigned)ry[u] <= Hm2;
 	}
 #pragma unroll
 	for (int u = 0; u < U; u++) {
-		const T* p = img + (in[u] ? ry[u] * W + rx[u] : 0);
+		// ROIs are far below 2^24 pixels: a 24-bit multiply-add forms the index
+		const T* p = img + (in[u] ? __mul24(ry[u], W) + rx[u] : 0);
 		v00[u] = (float)p[0];
 		v10[u] = (float)p[1];
 		v01[u] = (float)p[W];
@@ -387,7 +397,15 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			tw[N + i] = P.tw_inv[i];
 			perm[i] = P.fft_perm[i];
 		}
-		for (int j = tid; j < P.qi_A; j += kBlock) qdir[j] = P.qi_dirs[j];
+		// direction table per quadrant with the quadrant's signs applied: (mx*dx, my*dy),
+		// exact sign flips of the reference's mx*quadrantDirs[i].x (cputrack/cpu_tracker.cpp:658-659)
+		for (int j = tid; j < 4 * P.qi_A; j += kBlock) {
+			const int q = j / P.qi_A;
+			const float2 d = P.qi_dirs[j - q * P.qi_A];
+			const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
+			const float sy = (q < 2) ? 1.0f : -1.0f;
+			qdir[j] = make_float2(sx * d.x, sy * d.y);
+		}
 		__syncthreads();
 		for (int k = 0; k < P.qi_iters; k++) {
 			boundary = (cx + maxr >= (float)W) || (cx - maxr < 0.0f) || (cy + maxr >= (float)H) || (cy - maxr < 0.0f);
@@ -399,8 +417,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				const int p = base + tid;
 				if (p < 4 * nr) {
 					const int q = p / nr, ri = p - q * nr;
-					const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
-					const float sy = (q < 2) ? 1.0f : -1.0f;
+					const float2* qd = qdir + q * P.qi_A;
 					const float r = minr + rstep * (float)ri;
 					float sum = 0.0f;
 					int cnt = 0;
@@ -409,15 +426,15 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 						float x[4], y[4];
 #pragma unroll
 						for (int u = 0; u < 4; u++) {
-							const float2 d = qdir[stride * (a + u)];
-							x[u] = cx + (sx * d.x) * r;
-							y[u] = cy + (sy * d.y) * r;
+							const float2 d = qd[stride * (a + u)];
+							x[u] = cx + d.x * r;
+							y[u] = cy + d.y * r;
 						}
 						sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
 					}
 					for (; a < steps; a++) {
-						const float2 d = qdir[stride * a];
-						const float x[1] = { cx + (sx * d.x) * r }, y[1] = { cy + (sy * d.y) * r };
+						const float2 d = qd[stride * a];
+						const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
 						sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
 					}
 					float val = cnt >= 4 ? sum / (float)cnt : mean;
