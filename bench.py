@@ -8,6 +8,7 @@ the oracle's QueuedCPUTracker restatement on the host cores on a bounded
 sample of the same workload.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py --copy-probe     # plain 256 MiB device copy, to calibrate HBM counters
 """
 import argparse
 import json
@@ -26,6 +27,42 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic HBM bytes per localization: 20,000 B ROI (100x100 u16) + 3,200 B of LUT
 # (51,200 B per bead shared by its 16 frames) + 24 B job + 52 B result.
 BYTES_PER_LOC = 20000 + 3200 + 24 + 52
+# HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (calibrated against a plain copy)
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "localize_realtime100_hbm_traffic.txt")
+
+
+SQ_PROFILE = os.path.join(ROOT, "profiles", "localize_realtime100_sq_counters.txt")
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
+
+
+def _profile_value(path, key):
+    try:
+        for line in open(path):
+            parts = line.lstrip("#").split()
+            if parts and parts[0].rstrip(":") == key:
+                return int(parts[1].replace(",", ""))
+    except (OSError, IndexError, ValueError):
+        pass
+    return None
+
+
+def measured_traffic():
+    """Bytes per launch recorded by the committed counter summary, or None."""
+    return _profile_value(TRAFFIC_PROFILE, "traffic_bytes_per_launch")
+
+
+def copy_probe(mib=256, reps=3):
+    """A plain device-to-device copy of known size: the counter calibration kernel."""
+    import torch
+
+    a = torch.ones(mib * 2**18, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": a.numel() * 4, "copy_probe_bytes_written": b.numel() * 4,
+                      "launches": reps}), flush=True)
 
 
 def cpu_baseline(w, seconds_target=12.0):
@@ -56,7 +93,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rois", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe()
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards by bead group across "
               "GPUs with no exchange step, so the whole-job rate is N x the one-GPU rate", file=sys.stderr)
@@ -106,9 +147,16 @@ def main():
                    "rms_error_vs_truth": {"x_px": round(err["x"], 4), "y_px": round(err["y"], 4),
                                           "z_planes": round(err["z"], 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
                      "kernel_ms": round(kern_s * 1e3, 4)},
     }
+    valu = _profile_value(SQ_PROFILE, "SQ_INSTS_VALU")
+    if valu and a.rois == 4096:
+        # the sampling stages are VALU-issue bound: instructions per launch from the counter summary
+        out["roofline"]["valu_issue"] = {"wave_instr_per_launch": valu,
+                                         "achieved": round(valu / kern_s / 1e9, 1),
+                                         "peak": VALU_PEAK_WAVE_INSTR / 1e9, "unit": "G wave-instr/s",
+                                         "frac": round(valu / kern_s / VALU_PEAK_WAVE_INSTR, 4)}
     b.free()
     trk.free()
     if not a.no_cpu:
