@@ -69,4 +69,49 @@ QTRK_API void QTrkGenerateImagesFromLUT(float* out, int n, int width, int height
                                         int planes, int steps, float minradius, float maxradius,
                                         const float* pos, const int* lutIndex, bool useSplineInterp, int ovs);
 
+/*
+ * Grouped radial Z-LUT match.  Holds `beads` LUTs of planes x radialsteps
+ * (the reference's [bead][plane][r] layout, cputrack/cpu_tracker.h:37) in GPU
+ * memory and matches, in one launch, every bead's profiles against its own
+ * LUT.  Each profile gets the best plane and z of the reference's
+ * LUTProfileCompare + 7-point FP64 fit (cputrack/cpu_tracker.cpp:791-868),
+ * which replaces the per-profile compare inside the tracker for callers that
+ * hold many profiles per bead.
+ *
+ * precision: QTRK_MATCH_FP32 scores every plane in the reference arithmetic
+ * (float difference, double sum).  QTRK_MATCH_BF16 / QTRK_MATCH_FP16 compute
+ * approximate scores for all planes as a BF16/FP16 matrix-core GEMM with FP32
+ * accumulation, rescore the 8 best planes (and the fit window around the best
+ * of them) in the reference arithmetic and fit in FP64; the result equals the
+ * FP32 path's whenever the exact best plane is among those 8.  The first run in
+ * a low precision after QTrkLUTMatchSetLUT converts the LUT (once).
+ */
+#define QTRK_MATCH_FP32 0
+#define QTRK_MATCH_BF16 1
+#define QTRK_MATCH_FP16 2
+
+typedef struct QTrkLUTMatch QTrkLUTMatch;
+
+/* On the current HIP device.  NULL on error (QTrkGetLastError). */
+QTRK_API QTrkLUTMatch* QTrkLUTMatchCreate(int beads, int planes, int radialsteps);
+QTRK_API void QTrkLUTMatchFree(QTrkLUTMatch* m);
+/* Copies the LUTs ([beads][planes][radialsteps], host memory) and optional
+ * radial weights ([radialsteps] or NULL), as SetRadialZLUT/SetRadialWeights do.  0 on success. */
+QTRK_API int QTrkLUTMatchSetLUT(QTrkLUTMatch* m, const float* luts, const float* radialweights);
+/* Copies framesPerBead profiles per bead ([beads][framesPerBead][radialsteps],
+ * host memory) to the GPU.  0 on success. */
+QTRK_API int QTrkLUTMatchUploadProfiles(QTrkLUTMatch* m, const float* profiles, int framesPerBead);
+/* Device buffer for [beads][framesPerBead][radialsteps] profiles, for callers
+ * that fill it on the GPU; NULL on error. */
+QTRK_API float* QTrkLUTMatchDeviceProfiles(QTrkLUTMatch* m, int framesPerBead);
+/* Launches `repeats` matches of the current profiles (asynchronous, on the
+ * object's own stream; HIP events bracket them).  0 on success. */
+QTRK_API int QTrkLUTMatchRun(QTrkLUTMatch* m, int precision, int repeats);
+/* Waits; returns the event-timed milliseconds of the last run, negative on error. */
+QTRK_API float QTrkLUTMatchSync(QTrkLUTMatch* m);
+/* Copies n = beads * framesPerBead results back (profile order): z in planes,
+ * best plane, and for the BF16/FP16 paths the 8 candidate planes per profile
+ * ([n][8], -1 where none; left untouched by the FP32 path).  Any output may be NULL.  0 on success. */
+QTRK_API int QTrkLUTMatchDownload(QTrkLUTMatch* m, float* z, int* bestPlane, int* candidates, int n);
+
 #endif

@@ -168,7 +168,17 @@ SIGNATURES = {
     "QTrkBatchDownloadDiagnostics": (C.c_int, [_P, _P, _P, C.c_int]),
     "QTrkGenerateImagesFromLUT": (None, [_FP, C.c_int, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.c_int,
                                          C.c_float, C.c_float, _FP, _IP, C.c_bool, C.c_int]),
+    "QTrkLUTMatchCreate": (_P, [C.c_int, C.c_int, C.c_int]),
+    "QTrkLUTMatchFree": (None, [_P]),
+    "QTrkLUTMatchSetLUT": (C.c_int, [_P, _FP, _FP]),
+    "QTrkLUTMatchUploadProfiles": (C.c_int, [_P, _FP, C.c_int]),
+    "QTrkLUTMatchDeviceProfiles": (_FP, [_P, C.c_int]),
+    "QTrkLUTMatchRun": (C.c_int, [_P, C.c_int, C.c_int]),
+    "QTrkLUTMatchSync": (C.c_float, [_P]),
+    "QTrkLUTMatchDownload": (C.c_int, [_P, _FP, _IP, _IP, C.c_int]),
 }
+
+MATCH_FP32, MATCH_BF16, MATCH_FP16 = 0, 1, 2
 
 _lib = None
 
@@ -462,3 +472,61 @@ def generate_images_from_lut(luts, pos, lut_index, width, height, minr, maxr, sp
                                     luts.shape[2], minr, maxr, _fptr(pos), li.ctypes.data_as(_IP), spline, ovs)
     check()
     return out
+
+
+class LUTMatch:
+    """Grouped radial Z-LUT match of many profiles per bead (include/qtrk_amd.h)."""
+
+    def __init__(self, luts, radial_weights=None):
+        luts = np.ascontiguousarray(luts, np.float32)
+        self.beads, self.planes, self.res = luts.shape
+        self.h = lib().QTrkLUTMatchCreate(self.beads, self.planes, self.res)
+        check()
+        self.frames = 0
+        self.set_lut(luts, radial_weights)
+
+    def set_lut(self, luts, radial_weights=None):
+        luts = np.ascontiguousarray(luts, np.float32)
+        w = None if radial_weights is None else np.ascontiguousarray(radial_weights, np.float32)
+        if lib().QTrkLUTMatchSetLUT(self.h, _fptr(luts), None if w is None else _fptr(w)) != 0:
+            check()
+
+    def upload(self, profiles):
+        """profiles: [beads][frames][radialsteps] float32."""
+        p = np.ascontiguousarray(profiles, np.float32)
+        assert p.shape[0] == self.beads and p.shape[2] == self.res, p.shape
+        if lib().QTrkLUTMatchUploadProfiles(self.h, _fptr(p), p.shape[1]) != 0:
+            check()
+        self.frames = p.shape[1]
+
+    def run(self, precision=MATCH_FP32, repeats=1):
+        if lib().QTrkLUTMatchRun(self.h, int(precision), int(repeats)) != 0:
+            check()
+
+    def sync(self):
+        ms = lib().QTrkLUTMatchSync(self.h)
+        if ms < 0:
+            check()
+        return ms
+
+    def download(self):
+        """(z [beads][frames], best plane, candidates [beads][frames][8])."""
+        n = self.beads * self.frames
+        z = np.zeros(n, np.float32)
+        best = np.zeros(n, np.int32)
+        cand = np.full((n, 8), -1, np.int32)
+        if lib().QTrkLUTMatchDownload(self.h, _fptr(z), best.ctypes.data_as(_IP), cand.ctypes.data_as(_IP), n) != 0:
+            check()
+        shp = (self.beads, self.frames)
+        return z.reshape(shp), best.reshape(shp), cand.reshape(shp + (8,))
+
+    def match(self, profiles, precision=MATCH_FP32):
+        self.upload(profiles)
+        self.run(precision)
+        self.sync()
+        return self.download()
+
+    def free(self):
+        if self.h:
+            lib().QTrkLUTMatchFree(self.h)
+            self.h = None

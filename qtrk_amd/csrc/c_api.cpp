@@ -41,8 +41,6 @@ static void unregister(QueuedTracker* t)
 	if (g_last_tracker == t) g_last_tracker = g_trackers.empty() ? nullptr : *g_trackers.rbegin();
 }
 
-#define GUARD(...) guard(__func__, [&] { __VA_ARGS__; })
-
 QTRK_API const char* QTrkGetLastError(void) { return g_last_error.c_str(); }
 QTRK_API void QTrkClearLastError(void) { g_last_error.clear(); }
 

@@ -42,3 +42,5 @@ bool guard(const char* fn, F&& f)
 }  // namespace qtrk
 
 #define HIP_CHECK(x) ::qtrk::hip_check((x), #x)
+// Body of a C-ABI function: exceptions become the last error; true on success.
+#define GUARD(...) ::qtrk::guard(__func__, [&] { __VA_ARGS__; })

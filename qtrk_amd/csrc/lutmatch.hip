@@ -1,0 +1,563 @@
+// lutmatch.hip -- grouped radial Z-LUT matching for gfx950 (exact path and
+// BF16/FP16 matrix-core path; the scoring rule is stated in lutmatch.hpp).
+//
+// Matrix-core path, one workgroup (4 wavefronts) per bead and 64 profiles:
+//   * the bead's profiles are staged in LDS (float) and turned into MFMA
+//     B operands once: B[k=r][n=profile] = w_r^2 * prof[r], rounded to BF16/FP16;
+//   * the LUT was split once (launch_lutmatch_prepare) into FP32 tile means and
+//     residual A-operand fragments, [bead][32-plane tile][16-radius step][lane]
+//     x 8 values, so each wavefront's operand load is one contiguous, fully
+//     coalesced 1 KB read;
+//   * v_mfma_f32_32x32x16_{bf16,f16} accumulates p.D_w for a 32-plane x
+//     32-profile tile in FP32; the epilogue adds the FP32 tile-mean term
+//     2 p.M_t, subtracts |L|^2_w and folds the 16 values each lane holds into a
+//     per-profile top-8 kept in registers (sorting networks on keys that carry
+//     the plane index in their low bits);
+//   * the top-8 lists of the two half-waves and the four wavefronts are merged,
+//     the 8 candidates and the fit window are rescored in the reference
+//     arithmetic, and the FP64 7-point fit gives z.
+#include "lutmatch.hpp"
+#include "peakfit.hpp"
+
+#include <cfloat>
+
+namespace qtrk {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int PREC> struct MfmaOp;
+template <> struct MfmaOp<kMatchBF16> {
+	typedef bf16x8 V;
+	typedef __bf16 E;
+	static __device__ __forceinline__ f32x16 mma(V a, V b, f32x16 c)
+	{
+		return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+	}
+};
+template <> struct MfmaOp<kMatchFP16> {
+	typedef f16x8 V;
+	typedef _Float16 E;
+	static __device__ __forceinline__ f32x16 mma(V a, V b, f32x16 c)
+	{
+		return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+	}
+};
+
+// Workgroup -> (bead, profile tile) such that all tiles of one bead run on the
+// same XCD (workgroups are dispatched round-robin over the 8 XCDs), so the
+// bead's LUT is fetched into one L2 only.  The grid has ceil(beads/8)*8*chunks
+// workgroups; those mapped past the last bead exit.
+__device__ __forceinline__ void xcd_map(int id, int chunks, int& bead, int& chunk)
+{
+	const int x = id & 7, q = id >> 3;
+	chunk = q % chunks;
+	bead = (q / chunks) * 8 + x;
+}
+
+// One reference score: float difference, optional weight, negated square and
+// sum in double, radius by radius (cputrack/cpu_tracker.cpp:829-839).
+__device__ __forceinline__ double exact_score(const float* pr, const float* L, const float* w, int R)
+{
+	double s = 0;
+	for (int r = 0; r < R; r++) {
+		double d = (double)(pr[r] - L[r]);
+		if (w) d *= (double)w[r];
+		d = -d * d;
+		s += d;
+	}
+	return s;
+}
+
+// ------------------------------------------------------------ top-8 selection
+
+__device__ __forceinline__ void cx(float& a, float& b)
+{
+	const float hi = fmaxf(a, b), lo = fminf(a, b);
+	a = hi;
+	b = lo;
+}
+
+// Descending sort of 8 values: Batcher's odd-even merge sort (19 comparators).
+__device__ __forceinline__ void sort8(float (&v)[8])
+{
+	cx(v[0], v[1]); cx(v[2], v[3]); cx(v[4], v[5]); cx(v[6], v[7]);
+	cx(v[0], v[2]); cx(v[1], v[3]); cx(v[4], v[6]); cx(v[5], v[7]);
+	cx(v[1], v[2]); cx(v[5], v[6]);
+	cx(v[0], v[4]); cx(v[1], v[5]); cx(v[2], v[6]); cx(v[3], v[7]);
+	cx(v[2], v[4]); cx(v[3], v[5]);
+	cx(v[1], v[2]); cx(v[3], v[4]); cx(v[5], v[6]);
+}
+
+// a <- the 8 largest of two descending lists, descending: max(a_i, b_{7-i}) is
+// bitonic and holds exactly those 8; a bitonic merger sorts it.
+__device__ __forceinline__ void merge8(float (&a)[8], const float (&b)[8])
+{
+	float c[8];
+#pragma unroll
+	for (int i = 0; i < 8; i++) c[i] = fmaxf(a[i], b[7 - i]);
+	cx(c[0], c[4]); cx(c[1], c[5]); cx(c[2], c[6]); cx(c[3], c[7]);
+	cx(c[0], c[2]); cx(c[1], c[3]); cx(c[4], c[6]); cx(c[5], c[7]);
+	cx(c[0], c[1]); cx(c[2], c[3]); cx(c[4], c[5]); cx(c[6], c[7]);
+#pragma unroll
+	for (int i = 0; i < 8; i++) a[i] = c[i];
+}
+
+// Selection key: the approximate score with its low idx_bits mantissa bits
+// replaced by the plane index (a relative perturbation below 2^(idx_bits-23),
+// far under the BF16/FP16 rounding of the score itself).  -FLT_MAX marks "no
+// plane"; a NaN score gives a NaN key, which fmaxf never selects.
+__device__ __forceinline__ float make_key(float s, unsigned p, unsigned imask)
+{
+	return __uint_as_float((__float_as_uint(s) & ~imask) | p);
+}
+
+// ------------------------------------------------------------ LDS maps
+
+struct MfmaLds {
+	int prof, w2, keys, cand, ex, row, total;
+};
+__host__ __device__ inline int a16(int x) { return (x + 15) & ~15; }
+__host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS)
+{
+	MfmaLds L;
+	const int F = FT * 32;
+	int o = 0;
+	L.prof = o;  o += a16(F * (R + 1) * 4);
+	L.w2 = o;    o += a16(KS * 16 * 4);
+	L.keys = o;  o += a16(4 * F * 8 * 4);
+	L.cand = o;  o += a16(F * 8 * 4);
+	L.ex = o;    o += a16(F * 8 * 8);
+	L.row = o;   o += a16(F * 4);
+	L.total = o;
+	return L;
+}
+
+struct ExactLds {
+	int prof, sc, row, total;
+};
+__host__ __device__ inline ExactLds exact_lds(int R, int planes)
+{
+	ExactLds L;
+	int o = 0;
+	L.prof = o;  o += a16(kMatchExactTile * R * 4);
+	L.sc = o;    o += a16(kMatchExactTile * planes * 8);
+	L.row = o;   o += a16(kMatchExactTile * 4);
+	L.total = o;
+	return L;
+}
+
+// ------------------------------------------------------------ matrix-core path
+
+template <int PREC, int FT, int KSMAX>
+__global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
+{
+	typedef typename MfmaOp<PREC>::V V;
+	typedef typename MfmaOp<PREC>::E E;
+	extern __shared__ __align__(16) unsigned char smem[];
+
+	int bead, chunk;
+	xcd_map(blockIdx.x, P.chunks, bead, chunk);
+	if (bead >= P.beads) return;
+	const int s0 = P.start[bead] + chunk * kMatchFrameTile;
+	const int rem = P.start[bead + 1] - s0;
+	const int nf = rem < FT * 32 ? rem : FT * 32;
+	if (nf <= 0) return;
+
+	const int R = P.res, RS = R + 1, KS = P.ksteps, NP = P.planes;
+	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	const int n = lane & 31, h = lane >> 5;
+	const MfmaLds L = mfma_lds(FT, R, KS);
+	float* prof_s = (float*)(smem + L.prof);
+	float* w2_s = (float*)(smem + L.w2);
+	float* keys_s = (float*)(smem + L.keys);
+	int* cand_s = (int*)(smem + L.cand);
+	double* ex_s = (double*)(smem + L.ex);
+	int* row_s = (int*)(smem + L.row);
+
+	for (int i = tid; i < FT * 32; i += 256) row_s[i] = i < nf ? (P.rowidx ? P.rowidx[s0 + i] : s0 + i) : -1;
+	for (int k = tid; k < KS * 16; k += 256) w2_s[k] = k < R ? (P.w ? P.w[k] * P.w[k] : 1.0f) : 0.0f;
+	__syncthreads();
+	for (int e = tid; e < FT * 32 * R; e += 256) {
+		const int f = e / R, r = e - f * R;
+		prof_s[f * RS + r] = f < nf ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
+	}
+	__syncthreads();
+
+	// B operands: lane (n, h) holds profile column n, radii 16*ks + 8*h + [0, 8)
+	V bq[FT][KSMAX];
+#pragma unroll
+	for (int ft = 0; ft < FT; ft++) {
+		const float* pr = prof_s + (ft * 32 + n) * RS;
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++) {
+#pragma unroll
+			for (int j = 0; j < 8; j++) {
+				const int k = ks * 16 + h * 8 + j;
+				bq[ft][ks][j] = (E)((ks < KS && k < R) ? pr[k] * w2_s[k] : 0.0f);
+			}
+		}
+	}
+
+	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
+	const float* nb = P.norm + (size_t)bead * P.ptiles * 32 + 4 * h;
+	const unsigned imask = (1u << P.idx_bits) - 1u;
+	float top[FT][8];
+#pragma unroll
+	for (int ft = 0; ft < FT; ft++)
+#pragma unroll
+		for (int i = 0; i < 8; i++) top[ft][i] = -FLT_MAX;
+
+	uint4 a[KSMAX], an[KSMAX];
+	int t = wid;
+	if (t < P.ptiles) {
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++)
+			if (ks < KS) a[ks] = fb[((size_t)t * KS + ks) * 64];
+	}
+	for (; t < P.ptiles; t += 4) {
+		const int tn = t + 4;
+		if (tn < P.ptiles) {
+#pragma unroll
+			for (int ks = 0; ks < KSMAX; ks++)
+				if (ks < KS) an[ks] = fb[((size_t)tn * KS + ks) * 64];
+		}
+		f32x16 acc[FT];
+#pragma unroll
+		for (int ft = 0; ft < FT; ft++)
+#pragma unroll
+			for (int i = 0; i < 16; i++) acc[ft][i] = 0.0f;
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++) {
+			if (ks < KS) {
+				const V av = __builtin_bit_cast(V, a[ks]);
+#pragma unroll
+				for (int ft = 0; ft < FT; ft++) acc[ft] = MfmaOp<PREC>::mma(av, bq[ft][ks], acc[ft]);
+			}
+		}
+		// 2 p.M_t for this lane's profile columns in FP32, the two half-waves
+		// taking alternate radii
+		const float* mt = P.mean + ((size_t)bead * P.ptiles + t) * R;
+		float cm[FT];
+#pragma unroll
+		for (int ft = 0; ft < FT; ft++) {
+			const float* pr = prof_s + (ft * 32 + n) * RS;
+			float s = 0.0f;
+			for (int r = h; r < R; r += 2) s = __builtin_fmaf(pr[r], mt[r], s);
+			s += __shfl_xor(s, 32);
+			cm[ft] = 2.0f * s;
+		}
+		// this lane's rows: 8*(i/4) + 4*h + i%4 of the tile
+		float nv[16];
+#pragma unroll
+		for (int j = 0; j < 4; j++) {
+			const float4 v = *(const float4*)(nb + t * 32 + 8 * j);
+			nv[4 * j] = v.x; nv[4 * j + 1] = v.y; nv[4 * j + 2] = v.z; nv[4 * j + 3] = v.w;
+		}
+#pragma unroll
+		for (int ft = 0; ft < FT; ft++) {
+			float g0[8], g1[8];
+#pragma unroll
+			for (int i = 0; i < 16; i++) {
+				const int p = t * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+				const float s = 2.0f * acc[ft][i] + cm[ft] - nv[i];
+				const float key = p < NP ? make_key(s, (unsigned)p, imask) : -FLT_MAX;
+				if (i < 8) g0[i] = key; else g1[i - 8] = key;
+			}
+			sort8(g0);
+			sort8(g1);
+			merge8(g0, g1);
+			merge8(top[ft], g0);
+		}
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++) a[ks] = an[ks];
+	}
+
+	// lanes n and n+32 hold the same profile column: merge, then across wavefronts
+#pragma unroll
+	for (int ft = 0; ft < FT; ft++) {
+		float o[8];
+#pragma unroll
+		for (int i = 0; i < 8; i++) o[i] = __shfl_xor(top[ft][i], 32);
+		merge8(top[ft], o);
+		if (h == 0)
+#pragma unroll
+			for (int i = 0; i < 8; i++) keys_s[((wid * FT + ft) * 32 + n) * 8 + i] = top[ft][i];
+	}
+	__syncthreads();
+	if (tid < FT * 32) {
+		float m[8];
+#pragma unroll
+		for (int i = 0; i < 8; i++) m[i] = keys_s[tid * 8 + i];
+		for (int w = 1; w < 4; w++) {
+			float o[8];
+#pragma unroll
+			for (int i = 0; i < 8; i++) o[i] = keys_s[(w * FT * 32 + tid) * 8 + i];
+			merge8(m, o);
+		}
+#pragma unroll
+		for (int i = 0; i < 8; i++) cand_s[tid * 8 + i] = m[i] > -FLT_MAX ? (int)(__float_as_uint(m[i]) & imask) : -1;
+	}
+	__syncthreads();
+
+	// rescore the candidates in the reference arithmetic
+	const float* Lb = P.lut + (size_t)bead * NP * R;
+	for (int it = tid; it < FT * 32 * 8; it += 256) {
+		const int f = it >> 3, p = cand_s[it];
+		ex_s[it] = (f < nf && p >= 0) ? exact_score(prof_s + f * RS, Lb + (size_t)p * R, P.w, R) : 0.0;
+	}
+	__syncthreads();
+
+	if (tid < nf) {
+		const int* cp = cand_s + tid * 8;
+		const double* ep = ex_s + tid * 8;
+		// best candidate: highest score, the lower plane on ties, NaN never (the
+		// reference's strict '>' scan keeps plane 0 when every score is NaN)
+		int bi = -1;
+		double bv = 0;
+		for (int c = 0; c < 8; c++) {
+			const int p = cp[c];
+			const double v = ep[c];
+			if (p < 0 || v != v) continue;
+			if (bi < 0 || v > bv || (v == bv && p < bi)) { bi = p; bv = v; }
+		}
+		if (bi < 0) bi = 0;
+		const float* pr = prof_s + tid * RS;
+		const double zf = peak_fit<double>(bi, NP, kZW, [&](int k) -> double {
+			for (int c = 0; c < 8; c++)
+				if (cp[c] == k) return ep[c];
+			return exact_score(pr, Lb + (size_t)k * R, P.w, R);
+		});
+		const int row = row_s[tid];
+		P.z[row] = (float)zf;
+		P.best[row] = bi;
+		if (P.cand)
+			for (int c = 0; c < 8; c++) P.cand[(size_t)row * 8 + c] = cp[c];
+	}
+}
+
+// ------------------------------------------------------------ exact path
+
+// Up to 16 profiles of one bead per workgroup; lane per plane, all 16 scores
+// accumulated together so each LUT row is read once per 16 profiles.
+template <bool WEIGHTED>
+__global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	int bead, chunk;
+	xcd_map(blockIdx.x, P.chunks, bead, chunk);
+	if (bead >= P.beads) return;
+	const int s0 = P.start[bead] + chunk * kMatchExactTile;
+	const int rem = P.start[bead + 1] - s0;
+	const int nf = rem < kMatchExactTile ? rem : kMatchExactTile;
+	if (nf <= 0) return;
+
+	const int R = P.res, NP = P.planes;
+	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	const ExactLds L = exact_lds(R, NP);
+	float* prof_s = (float*)(smem + L.prof);
+	double* sc_s = (double*)(smem + L.sc);
+	int* row_s = (int*)(smem + L.row);
+
+	if (tid < kMatchExactTile) row_s[tid] = tid < nf ? (P.rowidx ? P.rowidx[s0 + tid] : s0 + tid) : -1;
+	__syncthreads();
+	for (int e = tid; e < kMatchExactTile * R; e += 256) {
+		const int f = e / R, r = e - f * R;
+		prof_s[f * R + r] = f < nf ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
+	}
+	__syncthreads();
+
+	const float* Lb = P.lut + (size_t)bead * NP * R;
+	for (int p = tid; p < NP; p += 256) {
+		const float* Lr = Lb + (size_t)p * R;
+		double acc[kMatchExactTile];
+#pragma unroll
+		for (int f = 0; f < kMatchExactTile; f++) acc[f] = 0.0;
+		for (int r = 0; r < R; r++) {
+			const float l = Lr[r];
+			const double wr = WEIGHTED ? (double)P.w[r] : 1.0;
+#pragma unroll
+			for (int f = 0; f < kMatchExactTile; f++) {
+				double d = (double)(prof_s[f * R + r] - l);
+				if (WEIGHTED) d *= wr;
+				d = -d * d;
+				acc[f] += d;
+			}
+		}
+#pragma unroll
+		for (int f = 0; f < kMatchExactTile; f++)
+			if (f < nf) sc_s[f * NP + p] = acc[f];
+	}
+	__syncthreads();
+
+	// argmax with the reference's strict '>' scan from plane 0 (NaN scores never
+	// win; a NaN at plane 0 keeps plane 0), then the FP64 fit
+	for (int f = wid; f < nf; f += 4) {
+		const double* sc = sc_s + f * NP;
+		double bv = 0;
+		int bi = -1;
+		for (int p = lane; p < NP; p += 64) {
+			const double v = sc[p];
+			if (v != v) continue;
+			if (bi < 0 || v > bv) { bv = v; bi = p; }
+		}
+		for (int off = 32; off > 0; off >>= 1) {
+			const double ov = __shfl_xor(bv, off);
+			const int oi = __shfl_xor(bi, off);
+			if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) { bv = ov; bi = oi; }
+		}
+		if (lane == 0) {
+			if (bi < 0 || sc[0] != sc[0]) bi = 0;
+			const double zf = peak_fit<double>(bi, NP, kZW, [&](int k) { return sc[k]; });
+			const int row = row_s[f];
+			P.z[row] = (float)zf;
+			P.best[row] = bi;
+		}
+	}
+}
+
+// ------------------------------------------------------------ preparation
+
+// Tile means: raw[b][t][r] = mean of the tile's valid rows, mean = w_r^2 * raw.
+__global__ void lutmatch_mean_kernel(LutMatchParams P, float* mean, float* raw)
+{
+	const size_t total = (size_t)P.beads * P.ptiles * P.res;
+	for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const int r = (int)(g % P.res);
+		const size_t q = g / P.res;
+		const int t = (int)(q % P.ptiles);
+		const size_t b = q / P.ptiles;
+		const int p0 = t * 32;
+		const int cnt = P.planes - p0 < 32 ? P.planes - p0 : 32;
+		float s = 0.0f;
+		for (int i = 0; i < cnt; i++) s += P.lut[(b * P.planes + p0 + i) * P.res + r];
+		s /= (float)cnt;
+		raw[g] = s;
+		mean[g] = (P.w ? P.w[r] * P.w[r] : 1.0f) * s;
+	}
+}
+
+// Residual fragments: lane (row = lane % 32, radii 8*(lane/32) + [0, 8)) of each
+// (tile, 16-radius step) holds L - M_t rounded to E; padded rows and radii are 0.
+template <typename E>
+__global__ void lutmatch_frag_kernel(LutMatchParams P, uint4* frag, const float* raw)
+{
+	typedef E V8 __attribute__((ext_vector_type(8)));
+	const size_t total = (size_t)P.beads * P.ptiles * P.ksteps * 64;
+	for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const int lane = (int)(g & 63);
+		size_t q = g >> 6;
+		const int ks = (int)(q % P.ksteps);
+		q /= P.ksteps;
+		const int t = (int)(q % P.ptiles);
+		const size_t b = q / P.ptiles;
+		const int row = t * 32 + (lane & 31);
+		const int k0 = ks * 16 + (lane >> 5) * 8;
+		const float* mt = raw + (b * P.ptiles + t) * P.res;
+		V8 v;
+#pragma unroll
+		for (int j = 0; j < 8; j++) {
+			const int k = k0 + j;
+			v[j] = (E)((row < P.planes && k < P.res) ? P.lut[(b * P.planes + row) * P.res + k] - mt[k] : 0.0f);
+		}
+		frag[g] = __builtin_bit_cast(uint4, v);
+	}
+}
+
+__global__ void lutmatch_norm_kernel(LutMatchParams P, float* norm)
+{
+	const int PP = P.ptiles * 32;
+	const size_t total = (size_t)P.beads * PP;
+	for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const int p = (int)(g % PP);
+		const size_t b = g / PP;
+		float s = 0.0f;
+		if (p < P.planes) {
+			const float* Lr = P.lut + (b * P.planes + p) * P.res;
+			for (int r = 0; r < P.res; r++) {
+				const float w2 = P.w ? P.w[r] * P.w[r] : 1.0f;
+				s += w2 * Lr[r] * Lr[r];
+			}
+		}
+		norm[g] = s;
+	}
+}
+
+static int grid_for(size_t n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
+
+hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, float* norm, float* mean,
+                                   float* raw_mean, hipStream_t stream)
+{
+	if (prec != kMatchBF16 && prec != kMatchFP16) return hipErrorInvalidValue;
+	lutmatch_mean_kernel<<<grid_for((size_t)p.beads * p.ptiles * p.res), 256, 0, stream>>>(p, mean, raw_mean);
+	hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return e;
+	const size_t nfrag = (size_t)p.beads * p.ptiles * p.ksteps * 64;
+	if (prec == kMatchBF16)
+		lutmatch_frag_kernel<__bf16><<<grid_for(nfrag), 256, 0, stream>>>(p, frag, raw_mean);
+	else
+		lutmatch_frag_kernel<_Float16><<<grid_for(nfrag), 256, 0, stream>>>(p, frag, raw_mean);
+	e = hipGetLastError();
+	if (e != hipSuccess) return e;
+	lutmatch_norm_kernel<<<grid_for((size_t)p.beads * p.ptiles * 32), 256, 0, stream>>>(p, norm);
+	return hipGetLastError();
+}
+
+template <int PREC, int FT, int KSMAX>
+static hipError_t launch_mfma(const LutMatchParams& q, int grid, hipStream_t stream)
+{
+	const size_t lds = (size_t)mfma_lds(FT, q.res, q.ksteps).total;
+	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_mfma_kernel<PREC, FT, KSMAX>,
+	                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	if (e != hipSuccess) return e;
+	lutmatch_mfma_kernel<PREC, FT, KSMAX><<<grid, 256, lds, stream>>>(q);
+	return hipGetLastError();
+}
+
+template <int PREC>
+static hipError_t launch_mfma_prec(const LutMatchParams& q, int FT, int grid, hipStream_t stream)
+{
+	if (q.ksteps <= 8)
+		return FT == 1 ? launch_mfma<PREC, 1, 8>(q, grid, stream) : launch_mfma<PREC, 2, 8>(q, grid, stream);
+	return FT == 1 ? launch_mfma<PREC, 1, 16>(q, grid, stream) : launch_mfma<PREC, 2, 16>(q, grid, stream);
+}
+
+hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream)
+{
+	if (p.beads <= 0 || max_per_bead <= 0) return hipSuccess;
+	LutMatchParams q = p;
+	if (prec == kMatchFP32) {
+		q.chunks = (max_per_bead + kMatchExactTile - 1) / kMatchExactTile;
+		const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
+		const size_t lds = (size_t)exact_lds(p.res, p.planes).total;
+		if (lds > 160 * 1024) return hipErrorInvalidValue;
+		if (p.w) {
+			hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<true>,
+			                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+			if (e != hipSuccess) return e;
+			lutmatch_exact_kernel<true><<<grid, 256, lds, stream>>>(q);
+		} else {
+			hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<false>,
+			                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+			if (e != hipSuccess) return e;
+			lutmatch_exact_kernel<false><<<grid, 256, lds, stream>>>(q);
+		}
+		return hipGetLastError();
+	}
+	if (p.res > kMatchMaxRes) return hipErrorInvalidValue;
+	const int FT = max_per_bead > 32 ? 2 : 1;
+	q.chunks = (max_per_bead + kMatchFrameTile - 1) / kMatchFrameTile;
+	const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
+	if (prec == kMatchBF16) return launch_mfma_prec<kMatchBF16>(q, FT, grid, stream);
+	if (prec == kMatchFP16) return launch_mfma_prec<kMatchFP16>(q, FT, grid, stream);
+	return hipErrorInvalidValue;
+}
+
+size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead)
+{
+	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes).total;
+	return (size_t)mfma_lds(max_per_bead > 32 ? 2 : 1, res, ksteps).total;
+}
+
+}  // namespace qtrk

@@ -1,0 +1,68 @@
+// lutmatch.hpp -- grouped radial Z-LUT matching: many profiles of each bead
+// against that bead's LUT, as one launch over all beads.
+//
+// The reference scores one profile at a time (CPUTracker::LUTProfileCompare,
+// cputrack/cpu_tracker.cpp:791-868):
+//     score[k] = sum_r -(w_r * (float)(prof[r] - L[k][r]))^2     (double sum)
+// followed by the 7-point FP64 fit around the best plane.  Here the profiles
+// of one bead are matched together:
+//   * exact path: every plane scored in exactly that arithmetic, with each
+//     LUT row read once per 16 profiles;
+//   * matrix-core path: -score = |p|^2_w - 2 p.L_w + |L|^2_w, and the cross
+//     term for all (profile, plane) pairs is computed on the matrix cores.
+//     Each tile of 32 planes is stored as its FP32 mean row M_t plus BF16/FP16
+//     residuals D = L - M_t: p.L = p.M_t (FP32, per profile and tile) +
+//     p.D (MFMA GEMM, FP32 accumulation).  Neighbouring planes of a LUT are
+//     similar, so |D| is a fraction of |L| and so is the rounding error of the
+//     approximate score -- what decides whether the right planes reach the
+//     candidate list.  The 8 planes with the highest approximate score are
+//     rescored in the reference arithmetic, the best of them is the best plane,
+//     and the fit window around it is scored in the reference arithmetic too.
+//     Whenever the reference's best plane is among the 8 candidates, the
+//     result (best plane and z) is identical to the reference's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace qtrk {
+
+enum : int { kMatchFP32 = 0, kMatchBF16 = 1, kMatchFP16 = 2 };
+constexpr int kMatchTopK = 8;
+constexpr int kMatchMaxRes = 256;     // radial steps supported by the matrix-core path
+constexpr int kMatchFrameTile = 64;   // profiles per workgroup, matrix-core path
+constexpr int kMatchExactTile = 16;   // profiles per workgroup, exact path
+
+struct LutMatchParams {
+	int beads, planes, res;
+	int ptiles, ksteps;       // ceil(planes / 32), ceil(res / 16)
+	int idx_bits;             // plane-index bits embedded in candidate keys
+	const float* lut;         // [beads][planes][res] float32
+	const uint4* frag;        // [beads][ptiles][ksteps][64 lanes] x 8 16-bit residuals (MFMA A-operand order)
+	const float* norm;        // [beads][ptiles * 32]: sum_r w_r^2 L^2 (float)
+	const float* mean;        // [beads][ptiles][res]: w_r^2 * M_t[r], the weighted tile means
+	const float* w;           // [res] radial weights, or null
+	// profiles, grouped by bead: slots start[b] .. start[b+1]-1 belong to bead b;
+	// slot s reads profile row rowidx[s] (s when rowidx is null) of prof[rows][res]
+	const float* prof;
+	const int* start;         // [beads + 1]
+	const int* rowidx;
+	int chunks;               // profile tiles per bead in the grid
+	// outputs, indexed by profile row
+	float* z;
+	int* best;
+	int* cand;                // [rows][8] candidate planes (matrix-core path), or null
+};
+
+// Splits the FP32 LUT into tile means (raw_mean: [beads][ptiles][res]
+// unweighted; mean: weighted) and residual MFMA A-operand fragments of `prec`
+// (kMatchBF16 or kMatchFP16), and computes the weighted plane norms.
+hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, float* norm, float* mean,
+                                   float* raw_mean, hipStream_t stream);
+// One grouped match of all beads' profiles (prec: kMatchFP32 / BF16 / FP16);
+// max_per_bead bounds start[b+1] - start[b].
+hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream);
+// Dynamic LDS bytes of that launch.
+size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead);
+
+}  // namespace qtrk
