@@ -9,6 +9,7 @@ sample of the same workload.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
     python bench.py --copy-probe     # plain 256 MiB device copy, to calibrate HBM counters
+    python bench.py --workload precision   # grouped Z-LUT match, exact vs BF16/FP16 matrix cores
 """
 import argparse
 import json
@@ -86,6 +87,123 @@ def cpu_baseline(w, seconds_target=12.0):
                       f"QueuedCPUTracker restatement; 1 thread: {k1 / t1:.0f} loc/s"}
 
 
+def precision_profiles(beads, frames, planes, seed):
+    """LUTs and measured profiles of the precision workload: per-bead analytic LUTs of `planes` x 128
+    radial steps (ring period U[7, 10] px), and one 100x100 uint16 ROI per profile rendered from the
+    bead's LUT at 10,000 photons over a background of 50 with Poisson noise, centre +-5 px, z in
+    [10, planes - 10]; the profiles are the tracker's normalised radial profiles of those ROIs."""
+    import torch
+    from qtrk_amd import api, synth
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    st = api.default_settings(**dict(synth.SETTINGS["realtime100"], zlut_radial_coverage=2.6224))
+    cfg = api.computed_config(st)
+    assert cfg.zlut_radialsteps == 128, cfg.zlut_radialsteps
+    scale = 200.0 / planes  # the realtime100 family's z range spread over `planes` planes
+    render = np.stack([synth.analytic_lut(planes, 128, cfg.zlut_minradius, cfg.zlut_maxradius, p,
+                                          dperiod=0.04 * scale, dwidth=0.05 * scale)
+                       for p in rng.uniform(7, 10, size=beads)])
+    luts = synth.normalize_profiles(render)
+    n = beads * frames
+    bead = (np.arange(n) // frames).astype(np.int32)
+    off = rng.uniform(-5, 5, size=(n, 2))
+    z = rng.uniform(10, planes - 10, size=n)
+    pos = np.stack([50 + off[:, 0], 50 + off[:, 1], z], axis=1).astype(np.float32)
+    trk = api.QTrkInstance(st)
+    trk.set_radial_zlut(luts)
+    trk.set_localization_mode(synth.MODE_QI_Z)
+    chunk = 8192
+    batch = api.DeviceBatch(trk, chunk, api.QTrkU16)
+    batch.enable_diagnostics()
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(seed)
+    prof = np.zeros((n, 128), np.float32)
+    for c0 in range(0, n, chunk):
+        c1 = min(n, c0 + chunk)
+        img = api.generate_images_from_lut(render, pos[c0:c1], bead[c0:c1], 100, 100, cfg.zlut_minradius,
+                                           cfg.zlut_maxradius)
+        lam = 50.0 + 10000.0 * (1.0 + torch.from_numpy(img).cuda()) * 0.5
+        cnt = torch.poisson(lam.clamp_min(0), generator=gen).clamp_(0, 65535).to(torch.int32).cpu().numpy()
+        batch.upload(cnt.astype(np.uint16), api.make_jobs(c1 - c0, zlut_index=bead[c0:c1]))
+        batch.run(c1 - c0)
+        batch.sync()
+        prof[c0:c1] = batch.diagnostics(c1 - c0)[1]
+    batch.free()
+    trk.free()
+    return luts, prof.reshape(beads, frames, 128), z.reshape(beads, frames)
+
+
+def precision_bench(a):
+    """Grouped Z-LUT match, 1,024 beads x 500 planes x 128 radial steps, 64 profiles per bead:
+    the reference arithmetic over every plane (fp32) against the BF16 and FP16 matrix-core paths."""
+    from oracle.bind import Oracle
+    from qtrk_amd import api
+
+    beads, frames, planes = 1024, 64, 500
+    luts, prof, ztrue = precision_profiles(beads, frames, planes, 0x51AB0002)
+    n = beads * frames
+    m = api.LUTMatch(luts)
+    m.upload(prof)
+    ptiles, ksteps = (planes + 31) // 32, 128 // 16
+    out = {}
+    for name, prec in (("fp32", api.MATCH_FP32), ("bf16", api.MATCH_BF16), ("fp16", api.MATCH_FP16)):
+        m.run(prec, a.warmup)
+        m.sync()
+        t0 = time.perf_counter()
+        m.run(prec, a.steps)
+        ev_ms = m.sync()
+        wall = time.perf_counter() - t0
+        z, best, cand = m.download()
+        kern_s = ev_ms / a.steps / 1e3
+        r = {"profiles_per_s": round(n * a.steps / wall, 1), "kernel_ms": round(kern_s * 1e3, 4),
+             "rms_z_vs_truth": round(float(np.sqrt(np.mean((z - ztrue) ** 2))), 4)}
+        if prec == api.MATCH_FP32:
+            bytes_ = luts.nbytes + prof.nbytes + n * 8
+            z32, best32 = z, best
+        else:
+            # distinct (bead, plane) rows rescored in FP32: the candidates and the fit window
+            win = best[..., None] + np.arange(-3, 4)
+            rows = np.concatenate([cand, np.clip(win, 0, planes - 1)], axis=-1).reshape(beads, -1)
+            distinct = sum(len(np.unique(rr[rr >= 0])) for rr in rows)
+            bytes_ = (beads * ptiles * ksteps * 64 * 16 + beads * ptiles * 128 * 4 + beads * ptiles * 32 * 4
+                      + prof.nbytes + n * (4 + 4 + 32) + distinct * 128 * 4)
+            r["mfma_tflops"] = round(2.0 * n * planes * 128 / kern_s / 1e12, 2)
+            r["mfma_frac_of_dense_peak"] = round(2.0 * n * planes * 128 / kern_s / 2.5e15, 5)
+            r["best_plane_equal_to_fp32_path"] = int(np.sum(best == best32))
+            r["max_abs_dz_vs_fp32_path"] = float(np.max(np.abs(z - z32)))
+        r["hbm_bytes_per_launch"] = int(bytes_)
+        r["hbm_gbs"] = round(bytes_ / kern_s / 1e9, 1)
+        out[name] = r
+        if prec == api.MATCH_BF16:
+            zb = z
+    m.free()
+    # parity sample against the oracle (FP32 restatement) and the FP64 oracle: 16 beads x 64 profiles
+    o, o64 = Oracle(), Oracle(fp64=True)
+    nb = 16
+    t0 = time.perf_counter()
+    zo = np.array([[o.lut_compare(prof[b, f], luts[b])[0] for f in range(frames)] for b in range(nb)])
+    t_cpu = time.perf_counter() - t0
+    z64 = np.array([[o64.lut_compare(prof[b, f], luts[b])[0] for f in range(frames)] for b in range(nb)])
+    res = {
+        "metric": "profiles/s (grouped Z-LUT match, BF16 MFMA + FP32 top-8 rescore + FP64 fit)",
+        "value": out["bf16"]["profiles_per_s"], "unit": "profiles/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"precision: {beads} beads x {planes} planes x 128 radial steps, {frames} profiles "
+                               "per bead from 100x100 u16 ROIs at 10,000 photons with Poisson noise",
+                   "paths": out,
+                   "oracle_sample": {"profiles": nb * frames,
+                                     "bf16_equal_to_oracle": bool(np.array_equal(zb[:nb], zo.astype(np.float32))),
+                                     "max_abs_dz_bf16_vs_fp64_oracle": float(np.max(np.abs(zb[:nb] - z64)))}},
+        "roofline": {"bound": "hbm", "achieved": out["bf16"]["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(out["bf16"]["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms": out["bf16"]["kernel_ms"]},
+        "cpu_baseline": {"value": round(nb * frames / t_cpu, 1), "unit": "profiles/s", "cores": 1, "kind": "port",
+                         "sample": f"{nb * frames} profiles through the oracle's LUTProfileCompare + FP64 fit"},
+    }
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +212,7 @@ def main():
     ap.add_argument("--rois", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--workload", default="realtime100", choices=["realtime100", "precision"])
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -102,6 +221,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the path shards by bead group across "
               "GPUs with no exchange step, so the whole-job rate is N x the one-GPU rate", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "precision":
+        precision_bench(a)
+        return
 
     from qtrk_amd import api, synth
 

@@ -57,16 +57,38 @@ __device__ __forceinline__ void xcd_map(int id, int chunks, int& bead, int& chun
 }
 
 // One reference score: float difference, optional weight, negated square and
-// sum in double, radius by radius (cputrack/cpu_tracker.cpp:829-839).
+// sum in double, radius by radius (cputrack/cpu_tracker.cpp:829-839).  pr and
+// w are in LDS; the LUT row L is read from global memory 32 radii at a time
+// (eight 16-byte loads in flight) when R is a multiple of 4, which keeps the
+// row 16-byte aligned.
+__device__ __forceinline__ void score_step(double& s, float p, float l, const float* w, int r)
+{
+	double d = (double)(p - l);
+	if (w) d *= (double)w[r];
+	d = -d * d;
+	s += d;
+}
 __device__ __forceinline__ double exact_score(const float* pr, const float* L, const float* w, int R)
 {
 	double s = 0;
-	for (int r = 0; r < R; r++) {
-		double d = (double)(pr[r] - L[r]);
-		if (w) d *= (double)w[r];
-		d = -d * d;
-		s += d;
+	int r = 0;
+	if ((R & 3) == 0) {
+		const float4* L4 = (const float4*)L;
+		for (; r + 32 <= R; r += 32) {
+			float4 v[8];
+#pragma unroll
+			for (int j = 0; j < 8; j++) v[j] = L4[(r >> 2) + j];
+#pragma unroll
+			for (int j = 0; j < 8; j++) {
+				const int q = r + 4 * j;
+				score_step(s, pr[q], v[j].x, w, q);
+				score_step(s, pr[q + 1], v[j].y, w, q + 1);
+				score_step(s, pr[q + 2], v[j].z, w, q + 2);
+				score_step(s, pr[q + 3], v[j].w, w, q + 3);
+			}
+		}
 	}
+	for (; r < R; r++) score_step(s, pr[r], L[r], w, r);
 	return s;
 }
 
@@ -115,8 +137,12 @@ __device__ __forceinline__ float make_key(float s, unsigned p, unsigned imask)
 
 // ------------------------------------------------------------ LDS maps
 
+// Profile rows in LDS: 16-byte aligned, zero-padded past R, and at least the
+// padded tile-mean length (4*ceil(R/4) + 4 >= 8*ceil(R/8)).
+__host__ __device__ inline int prof_stride(int R) { return ((R + 3) & ~3) + 4; }
+
 struct MfmaLds {
-	int prof, w2, keys, cand, ex, row, total;
+	int prof, w, w2, row, bq, keys, cand, ex, total;
 };
 __host__ __device__ inline int a16(int x) { return (x + 15) & ~15; }
 __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS)
@@ -124,24 +150,31 @@ __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS)
 	MfmaLds L;
 	const int F = FT * 32;
 	int o = 0;
-	L.prof = o;  o += a16(F * (R + 1) * 4);
+	L.prof = o;  o += a16(F * prof_stride(R) * 4);
+	L.w = o;     o += a16(KS * 16 * 4);
 	L.w2 = o;    o += a16(KS * 16 * 4);
+	L.row = o;   o += a16(F * 4);
+	// the B fragments live until the tile loop ends; the selection and
+	// rescoring scratch reuses their space afterwards
+	L.bq = o;
+	const int ebq = o + a16(FT * KS * 64 * 16);
 	L.keys = o;  o += a16(4 * F * 8 * 4);
 	L.cand = o;  o += a16(F * 8 * 4);
 	L.ex = o;    o += a16(F * 8 * 8);
-	L.row = o;   o += a16(F * 4);
-	L.total = o;
+	L.total = ebq > o ? ebq : o;
 	return L;
 }
 
 struct ExactLds {
-	int prof, sc, row, total;
+	int prof, w, sc, row, total;
 };
 __host__ __device__ inline ExactLds exact_lds(int R, int planes)
 {
 	ExactLds L;
+	const int R4 = (R + 3) & ~3;
 	int o = 0;
-	L.prof = o;  o += a16(kMatchExactTile * R * 4);
+	L.prof = o;  o += a16(kMatchExactTile * R4 * 4);
+	L.w = o;     o += a16(R4 * 4);
 	L.sc = o;    o += a16(kMatchExactTile * planes * 8);
 	L.row = o;   o += a16(kMatchExactTile * 4);
 	L.total = o;
@@ -151,7 +184,7 @@ __host__ __device__ inline ExactLds exact_lds(int R, int planes)
 // ------------------------------------------------------------ matrix-core path
 
 template <int PREC, int FT, int KSMAX>
-__global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void lutmatch_mfma_kernel(LutMatchParams P)
 {
 	typedef typename MfmaOp<PREC>::V V;
 	typedef typename MfmaOp<PREC>::E E;
@@ -165,43 +198,72 @@ __global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
 	const int nf = rem < FT * 32 ? rem : FT * 32;
 	if (nf <= 0) return;
 
-	const int R = P.res, RS = R + 1, KS = P.ksteps, NP = P.planes;
+	const int R = P.res, RS = prof_stride(R), KS = P.ksteps, NP = P.planes;
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	const int n = lane & 31, h = lane >> 5;
 	const MfmaLds L = mfma_lds(FT, R, KS);
 	float* prof_s = (float*)(smem + L.prof);
+	float* w_s = (float*)(smem + L.w);
 	float* w2_s = (float*)(smem + L.w2);
+	int* row_s = (int*)(smem + L.row);
+	uint4* bq_s = (uint4*)(smem + L.bq);
 	float* keys_s = (float*)(smem + L.keys);
 	int* cand_s = (int*)(smem + L.cand);
 	double* ex_s = (double*)(smem + L.ex);
-	int* row_s = (int*)(smem + L.row);
 
 	for (int i = tid; i < FT * 32; i += 256) row_s[i] = i < nf ? (P.rowidx ? P.rowidx[s0 + i] : s0 + i) : -1;
-	for (int k = tid; k < KS * 16; k += 256) w2_s[k] = k < R ? (P.w ? P.w[k] * P.w[k] : 1.0f) : 0.0f;
-	__syncthreads();
-	for (int e = tid; e < FT * 32 * R; e += 256) {
-		const int f = e / R, r = e - f * R;
-		prof_s[f * RS + r] = f < nf ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
+	for (int k = tid; k < KS * 16; k += 256) {
+		const float wk = k < R ? (P.w ? P.w[k] : 1.0f) : 0.0f;
+		w_s[k] = wk;
+		w2_s[k] = wk * wk;
 	}
 	__syncthreads();
-
-	// B operands: lane (n, h) holds profile column n, radii 16*ks + 8*h + [0, 8)
-	V bq[FT][KSMAX];
-#pragma unroll
-	for (int ft = 0; ft < FT; ft++) {
-		const float* pr = prof_s + (ft * 32 + n) * RS;
-#pragma unroll
-		for (int ks = 0; ks < KSMAX; ks++) {
+	// stage the profiles, zero-padded rows; 16-byte loads, 8 in flight, when rows are 16-byte aligned
+	if ((R & 3) == 0) {
+		const int q4 = R >> 2, qs = RS >> 2, total = FT * 32 * qs;
+		for (int e0 = tid; e0 < total; e0 += 8 * 256) {
+			float4 v[8];
 #pragma unroll
 			for (int j = 0; j < 8; j++) {
-				const int k = ks * 16 + h * 8 + j;
-				bq[ft][ks][j] = (E)((ks < KS && k < R) ? pr[k] * w2_s[k] : 0.0f);
+				const int e = e0 + j * 256, f = e / qs, q = e - f * qs;
+				v[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+				if (e < total && f < nf && q < q4) v[j] = ((const float4*)(P.prof + (size_t)row_s[f] * R))[q];
 			}
+#pragma unroll
+			for (int j = 0; j < 8; j++)
+				if (e0 + j * 256 < total) ((float4*)prof_s)[e0 + j * 256] = v[j];
+		}
+	} else {
+		for (int e = tid; e < FT * 32 * RS; e += 256) {
+			const int f = e / RS, r = e - f * RS;
+			prof_s[e] = (f < nf && r < R) ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
 		}
 	}
+	__syncthreads();
+	const float* wl = P.w ? w_s : nullptr;  // weights for the reference-arithmetic scores
 
+	// B operands in MFMA order, shared by the four wavefronts: fragment (ft, ks)
+	// lane l holds profile column l%32 of tile ft, radii 16*ks + 8*(l/32) + [0, 8),
+	// weighted by w^2 and rounded to E
+	for (int i = tid; i < FT * KS * 64; i += 256) {
+		const int l = i & 63, fk = i >> 6, ks = fk % KS, ft = fk / KS;
+		const float* pr = prof_s + (ft * 32 + (l & 31)) * RS;
+		const int k0 = ks * 16 + (l >> 5) * 8;
+		V v;
+#pragma unroll
+		for (int j = 0; j < 8; j++) {
+			const int k = k0 + j;
+			v[j] = (E)(k < R ? pr[k] * w2_s[k] : 0.0f);
+		}
+		bq_s[i] = __builtin_bit_cast(uint4, v);
+	}
+	__syncthreads();
+
+	const int wu = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform: tile-mean rows load through the scalar cache
 	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
 	const float* nb = P.norm + (size_t)bead * P.ptiles * 32 + 4 * h;
+	const int mq = P.mres >> 2;
+	const float4* prow = (const float4*)(prof_s + (lane < FT * 32 ? lane : 0) * RS);
 	const unsigned imask = (1u << P.idx_bits) - 1u;
 	float top[FT][8];
 #pragma unroll
@@ -209,19 +271,29 @@ __global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
 #pragma unroll
 		for (int i = 0; i < 8; i++) top[ft][i] = -FLT_MAX;
 
-	uint4 a[KSMAX], an[KSMAX];
-	int t = wid;
-	if (t < P.ptiles) {
+	uint4 a[KSMAX];
+	if (wu < P.ptiles) {
 #pragma unroll
 		for (int ks = 0; ks < KSMAX; ks++)
-			if (ks < KS) a[ks] = fb[((size_t)t * KS + ks) * 64];
+			if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
 	}
-	for (; t < P.ptiles; t += 4) {
-		const int tn = t + 4;
-		if (tn < P.ptiles) {
+	for (int t = wu; t < P.ptiles; t += 4) {
+		// FP32 tile-mean term p.M_t for profile `lane`
+		const float* mt = P.mean + ((size_t)bead * P.ptiles + t) * P.mres;
+		float c = 0.0f;
+		for (int q = 0; q < mq; q++) {
+			const float4 pv = prow[q];
+			c = __builtin_fmaf(pv.x, mt[4 * q], c);
+			c = __builtin_fmaf(pv.y, mt[4 * q + 1], c);
+			c = __builtin_fmaf(pv.z, mt[4 * q + 2], c);
+			c = __builtin_fmaf(pv.w, mt[4 * q + 3], c);
+		}
+		// |L|^2_w of this lane's rows: 8*(i/4) + 4*h + i%4 of the tile
+		float nv[16];
 #pragma unroll
-			for (int ks = 0; ks < KSMAX; ks++)
-				if (ks < KS) an[ks] = fb[((size_t)tn * KS + ks) * 64];
+		for (int j = 0; j < 4; j++) {
+			const float4 v = *(const float4*)(nb + t * 32 + 8 * j);
+			nv[4 * j] = v.x; nv[4 * j + 1] = v.y; nv[4 * j + 2] = v.z; nv[4 * j + 3] = v.w;
 		}
 		f32x16 acc[FT];
 #pragma unroll
@@ -233,28 +305,21 @@ __global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
 			if (ks < KS) {
 				const V av = __builtin_bit_cast(V, a[ks]);
 #pragma unroll
-				for (int ft = 0; ft < FT; ft++) acc[ft] = MfmaOp<PREC>::mma(av, bq[ft][ks], acc[ft]);
+				for (int ft = 0; ft < FT; ft++)
+					acc[ft] = MfmaOp<PREC>::mma(av, __builtin_bit_cast(V, bq_s[(ft * KS + ks) * 64 + lane]), acc[ft]);
 			}
 		}
-		// 2 p.M_t for this lane's profile columns in FP32, the two half-waves
-		// taking alternate radii
-		const float* mt = P.mean + ((size_t)bead * P.ptiles + t) * R;
+		// the next tile's operands load while this tile's epilogue runs
+		if (t + 4 < P.ptiles) {
+#pragma unroll
+			for (int ks = 0; ks < KSMAX; ks++)
+				if (ks < KS) a[ks] = fb[((size_t)(t + 4) * KS + ks) * 64];
+		}
+		// lane (n, h) needs p.M_t of profiles n and 32 + n: its own value and lane n^32's
+		const float co = __shfl_xor(c, 32);
 		float cm[FT];
-#pragma unroll
-		for (int ft = 0; ft < FT; ft++) {
-			const float* pr = prof_s + (ft * 32 + n) * RS;
-			float s = 0.0f;
-			for (int r = h; r < R; r += 2) s = __builtin_fmaf(pr[r], mt[r], s);
-			s += __shfl_xor(s, 32);
-			cm[ft] = 2.0f * s;
-		}
-		// this lane's rows: 8*(i/4) + 4*h + i%4 of the tile
-		float nv[16];
-#pragma unroll
-		for (int j = 0; j < 4; j++) {
-			const float4 v = *(const float4*)(nb + t * 32 + 8 * j);
-			nv[4 * j] = v.x; nv[4 * j + 1] = v.y; nv[4 * j + 2] = v.z; nv[4 * j + 3] = v.w;
-		}
+		cm[0] = 2.0f * (h ? co : c);
+		if (FT > 1) cm[FT - 1] = 2.0f * (h ? c : co);
 #pragma unroll
 		for (int ft = 0; ft < FT; ft++) {
 			float g0[8], g1[8];
@@ -270,9 +335,8 @@ __global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
 			merge8(g0, g1);
 			merge8(top[ft], g0);
 		}
-#pragma unroll
-		for (int ks = 0; ks < KSMAX; ks++) a[ks] = an[ks];
 	}
+	__syncthreads();  // the B fragments' space is reused from here on
 
 	// lanes n and n+32 hold the same profile column: merge, then across wavefronts
 #pragma unroll
@@ -305,7 +369,7 @@ __global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
 	const float* Lb = P.lut + (size_t)bead * NP * R;
 	for (int it = tid; it < FT * 32 * 8; it += 256) {
 		const int f = it >> 3, p = cand_s[it];
-		ex_s[it] = (f < nf && p >= 0) ? exact_score(prof_s + f * RS, Lb + (size_t)p * R, P.w, R) : 0.0;
+		ex_s[it] = (f < nf && p >= 0) ? exact_score(prof_s + f * RS, Lb + (size_t)p * R, wl, R) : 0.0;
 	}
 	__syncthreads();
 
@@ -327,7 +391,7 @@ __global__ __launch_bounds__(256) void lutmatch_mfma_kernel(LutMatchParams P)
 		const double zf = peak_fit<double>(bi, NP, kZW, [&](int k) -> double {
 			for (int c = 0; c < 8; c++)
 				if (cp[c] == k) return ep[c];
-			return exact_score(pr, Lb + (size_t)k * R, P.w, R);
+			return exact_score(pr, Lb + (size_t)k * R, wl, R);
 		});
 		const int row = row_s[tid];
 		P.z[row] = (float)zf;
@@ -353,18 +417,21 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 	const int nf = rem < kMatchExactTile ? rem : kMatchExactTile;
 	if (nf <= 0) return;
 
-	const int R = P.res, NP = P.planes;
+	const int R = P.res, R4 = (R + 3) & ~3, NP = P.planes;
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	const ExactLds L = exact_lds(R, NP);
 	float* prof_s = (float*)(smem + L.prof);
+	float* w_s = (float*)(smem + L.w);
 	double* sc_s = (double*)(smem + L.sc);
 	int* row_s = (int*)(smem + L.row);
 
 	if (tid < kMatchExactTile) row_s[tid] = tid < nf ? (P.rowidx ? P.rowidx[s0 + tid] : s0 + tid) : -1;
+	if (WEIGHTED)
+		for (int r = tid; r < R4; r += 256) w_s[r] = r < R ? P.w[r] : 0.0f;
 	__syncthreads();
-	for (int e = tid; e < kMatchExactTile * R; e += 256) {
-		const int f = e / R, r = e - f * R;
-		prof_s[f * R + r] = f < nf ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
+	for (int e = tid; e < kMatchExactTile * R4; e += 256) {
+		const int f = e / R4, r = e - f * R4;
+		prof_s[e] = (f < nf && r < R) ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
 	}
 	__syncthreads();
 
@@ -374,17 +441,37 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 		double acc[kMatchExactTile];
 #pragma unroll
 		for (int f = 0; f < kMatchExactTile; f++) acc[f] = 0.0;
-		for (int r = 0; r < R; r++) {
-			const float l = Lr[r];
-			const double wr = WEIGHTED ? (double)P.w[r] : 1.0;
+		// radius r for all 16 profiles: float difference, weight, negated square, double sum
+		auto step = [&](int r, float l) {
+			const double wr = WEIGHTED ? (double)w_s[r] : 1.0;
 #pragma unroll
 			for (int f = 0; f < kMatchExactTile; f++) {
-				double d = (double)(prof_s[f * R + r] - l);
+				double d = (double)(prof_s[f * R4 + r] - l);
 				if (WEIGHTED) d *= wr;
 				d = -d * d;
 				acc[f] += d;
 			}
+		};
+		int r = 0;
+		if ((R & 7) == 0) {  // 16-byte aligned rows: the next 8 radii load while 8 are scored
+			const float4* L4 = (const float4*)Lr;
+			const int nq = R >> 2;
+			float4 c0 = L4[0], c1 = L4[1];
+			for (int q = 0; q < nq; q += 2) {
+				float4 n0 = c0, n1 = c1;
+				if (q + 2 < nq) {
+					n0 = L4[q + 2];
+					n1 = L4[q + 3];
+				}
+				const int rr = q * 4;
+				step(rr, c0.x); step(rr + 1, c0.y); step(rr + 2, c0.z); step(rr + 3, c0.w);
+				step(rr + 4, c1.x); step(rr + 5, c1.y); step(rr + 6, c1.z); step(rr + 7, c1.w);
+				c0 = n0;
+				c1 = n1;
+			}
+			r = R;
 		}
+		for (; r < R; r++) step(r, Lr[r]);
 #pragma unroll
 		for (int f = 0; f < kMatchExactTile; f++)
 			if (f < nf) sc_s[f * NP + p] = acc[f];
@@ -419,22 +506,25 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 
 // ------------------------------------------------------------ preparation
 
-// Tile means: raw[b][t][r] = mean of the tile's valid rows, mean = w_r^2 * raw.
+// Tile means: raw[b][t][r] = mean of the tile's valid rows, mean = w_r^2 * raw;
+// rows are mres long, zero past res.
 __global__ void lutmatch_mean_kernel(LutMatchParams P, float* mean, float* raw)
 {
-	const size_t total = (size_t)P.beads * P.ptiles * P.res;
+	const size_t total = (size_t)P.beads * P.ptiles * P.mres;
 	for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-		const int r = (int)(g % P.res);
-		const size_t q = g / P.res;
+		const int r = (int)(g % P.mres);
+		const size_t q = g / P.mres;
 		const int t = (int)(q % P.ptiles);
 		const size_t b = q / P.ptiles;
 		const int p0 = t * 32;
 		const int cnt = P.planes - p0 < 32 ? P.planes - p0 : 32;
 		float s = 0.0f;
-		for (int i = 0; i < cnt; i++) s += P.lut[(b * P.planes + p0 + i) * P.res + r];
-		s /= (float)cnt;
+		if (r < P.res) {
+			for (int i = 0; i < cnt; i++) s += P.lut[(b * P.planes + p0 + i) * P.res + r];
+			s /= (float)cnt;
+		}
 		raw[g] = s;
-		mean[g] = (P.w ? P.w[r] * P.w[r] : 1.0f) * s;
+		mean[g] = (r < P.res && P.w ? P.w[r] * P.w[r] : 1.0f) * s;
 	}
 }
 
@@ -454,7 +544,7 @@ __global__ void lutmatch_frag_kernel(LutMatchParams P, uint4* frag, const float*
 		const size_t b = q / P.ptiles;
 		const int row = t * 32 + (lane & 31);
 		const int k0 = ks * 16 + (lane >> 5) * 8;
-		const float* mt = raw + (b * P.ptiles + t) * P.res;
+		const float* mt = raw + (b * P.ptiles + t) * P.mres;
 		V8 v;
 #pragma unroll
 		for (int j = 0; j < 8; j++) {
@@ -490,7 +580,7 @@ hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* fra
                                    float* raw_mean, hipStream_t stream)
 {
 	if (prec != kMatchBF16 && prec != kMatchFP16) return hipErrorInvalidValue;
-	lutmatch_mean_kernel<<<grid_for((size_t)p.beads * p.ptiles * p.res), 256, 0, stream>>>(p, mean, raw_mean);
+	lutmatch_mean_kernel<<<grid_for((size_t)p.beads * p.ptiles * p.mres), 256, 0, stream>>>(p, mean, raw_mean);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
 	const size_t nfrag = (size_t)p.beads * p.ptiles * p.ksteps * 64;

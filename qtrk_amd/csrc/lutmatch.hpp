@@ -40,7 +40,8 @@ struct LutMatchParams {
 	const float* lut;         // [beads][planes][res] float32
 	const uint4* frag;        // [beads][ptiles][ksteps][64 lanes] x 8 16-bit residuals (MFMA A-operand order)
 	const float* norm;        // [beads][ptiles * 32]: sum_r w_r^2 L^2 (float)
-	const float* mean;        // [beads][ptiles][res]: w_r^2 * M_t[r], the weighted tile means
+	const float* mean;        // [beads][ptiles][mres]: w_r^2 * M_t[r], the weighted tile means (0 past res)
+	int mres;                 // tile-mean row stride: res rounded up to a multiple of 8
 	const float* w;           // [res] radial weights, or null
 	// profiles, grouped by bead: slots start[b] .. start[b+1]-1 belong to bead b;
 	// slot s reads profile row rowidx[s] (s when rowidx is null) of prof[rows][res]
@@ -54,7 +55,7 @@ struct LutMatchParams {
 	int* cand;                // [rows][8] candidate planes (matrix-core path), or null
 };
 
-// Splits the FP32 LUT into tile means (raw_mean: [beads][ptiles][res]
+// Splits the FP32 LUT into tile means (raw_mean: [beads][ptiles][mres]
 // unweighted; mean: weighted) and residual MFMA A-operand fragments of `prec`
 // (kMatchBF16 or kMatchFP16), and computes the weighted plane norms.
 hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, float* norm, float* mean,

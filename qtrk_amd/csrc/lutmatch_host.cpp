@@ -32,6 +32,7 @@ struct QTrkLUTMatch {
 		p.frag = prec > 0 ? frag[prec].as<uint4>() : nullptr;
 		p.norm = norm.as<float>();
 		p.mean = mean.as<float>();
+		p.mres = (res + 7) & ~7;
 		p.w = have_w ? w.as<float>() : nullptr;
 		p.prof = prof.as<float>();
 		p.start = start.as<int>();
@@ -156,8 +157,9 @@ QTRK_API int QTrkLUTMatchRun(QTrkLUTMatch* m, int precision, int repeats)
 			if (!m->prepared[precision]) {
 				m->frag[precision].ensure((size_t)m->beads * m->ptiles * m->ksteps * 64 * 16);
 				m->norm.ensure((size_t)m->beads * m->ptiles * 32 * sizeof(float));
-				m->mean.ensure((size_t)m->beads * m->ptiles * m->res * sizeof(float));
-				m->raw_mean.ensure((size_t)m->beads * m->ptiles * m->res * sizeof(float));
+				const size_t mres = (size_t)((m->res + 7) & ~7);
+				m->mean.ensure((size_t)m->beads * m->ptiles * mres * sizeof(float));
+				m->raw_mean.ensure((size_t)m->beads * m->ptiles * mres * sizeof(float));
 				HIP_CHECK(launch_lutmatch_prepare(m->params(precision), precision, m->frag[precision].as<uint4>(),
 				                                  m->norm.as<float>(), m->mean.as<float>(), m->raw_mean.as<float>(),
 				                                  m->stream));
