@@ -114,4 +114,43 @@ QTRK_API float QTrkLUTMatchSync(QTrkLUTMatch* m);
  * ([n][8], -1 where none; left untouched by the FP32 path).  Any output may be NULL.  0 on success. */
 QTRK_API int QTrkLUTMatchDownload(QTrkLUTMatch* m, float* z, int* bestPlane, int* candidates, int n);
 
+/*
+ * Differentiable bead-image model rendered from radial Z-LUTs, its adjoint,
+ * and a Poisson maximum-likelihood fit with per-ROI Fisher matrix and CRLB.
+ *
+ * mu(px) = bg + I * g(px; x, y, z), g = GenerateImageFromLUT (cputrack/utils.cpp:325-390:
+ * cubic B-spline or linear interpolation between planes, linear in r, ovs x ovs
+ * sub-samples, pixel centres at integer coordinates).  theta = (x, y, z, I, bg)
+ * per ROI, x/y in pixels of the width x height ROI, z in LUT planes.
+ * The reference evaluates this model on the CPU only, as a test image
+ * generator and inside SampleFisherMatrix (cputrack/FisherMatrix.h:8-137),
+ * whose 3x3 (x, y, z) Fisher matrix by central differences this model's
+ * analytic 5x5 Fisher matrix extends.
+ *
+ * All array arguments are GPU memory; `stream` is a hipStream_t (NULL: the
+ * default stream).  Calls are asynchronous on that stream.  0 on success.
+ */
+typedef struct QTrkLUTModel QTrkLUTModel;
+
+/* luts: host memory, [count][planes][steps]; bins at minradius + i*(maxradius-minradius)/steps. */
+QTRK_API QTrkLUTModel* QTrkLUTModelCreate(const float* luts, int count, int planes, int steps, float minradius,
+                                          float maxradius, int width, int height, int ovs, bool splineInterp);
+QTRK_API void QTrkLUTModelFree(QTrkLUTModel* m);
+/* Forward: mu [n][height][width] and/or the Jacobian d mu / d theta [n][5][height][width]
+ * (either may be NULL).  lutIndex [n] selects each ROI's LUT (NULL: LUT 0). */
+QTRK_API int QTrkLUTModelForward(QTrkLUTModel* m, const float* theta, const int* lutIndex, int n, float* mu,
+                                 float* jacobian, void* stream);
+/* Backward (adjoint): gradTheta[i][k] = sum_px gradMu[i][px] * d mu[i][px] / d theta_k. */
+QTRK_API int QTrkLUTModelBackward(QTrkLUTModel* m, const float* theta, const int* lutIndex, int n,
+                                  const float* gradMu, float* gradTheta, void* stream);
+/* Fits theta to photon counts data [n][height][width] from thetaInit with `iterations`
+ * Fisher-scoring Newton steps (FP64 5x5 solves); writes theta [n][5] (may alias thetaInit)
+ * and, at the fitted theta, the Fisher matrix [n][25] (FP64), the CRLB sqrt(diag(F^-1)) [n][5]
+ * and the negative log-likelihood sum(mu - data*ln(mu)) [n]; those three may be NULL. */
+QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float* thetaInit, const int* lutIndex,
+                                int n, int iterations, float* theta, double* fisher, float* crlb, float* nll,
+                                void* stream);
+/* Waits for `stream` (NULL: the default stream). */
+QTRK_API int QTrkLUTModelSync(QTrkLUTModel* m, void* stream);
+
 #endif

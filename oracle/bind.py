@@ -77,6 +77,8 @@ class Oracle:
             "oq_bspline_weights": (None, [P, fl]),
             "oq_generate_image_from_lut": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl,
                                                   C.c_int, C.c_int]),
+            "oq_fisher_ref": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl, fl, fl, fl,
+                                     fl]),
             "oq_erf": (fl, [fl]),
             "oq_gaussian_spot": (None, [P, C.c_int, C.c_int, fl, fl, fl, fl, fl]),
             "oq_gauss2d_mle": (None, [P, C.c_int, C.c_int, fl, fl, fl, C.c_int, fl, P]),
@@ -174,6 +176,13 @@ class Oracle:
         self.L.oq_generate_image_from_lut(_p(img), w, h, _p(lut), lut.shape[0], lut.shape[1], minr, maxr, x, y,
                                           z, 1 if spline else 0, ovs)
         return img
+
+    def fisher_ref(self, w, h, lut, minr, maxr, pos, delta, max_value):
+        """SampleFisherMatrix::Compute: 3x3 (x, y, z) Fisher matrix by central differences."""
+        lut = np.ascontiguousarray(lut, np.float32)
+        F = np.zeros(9, self.fl)
+        self.L.oq_fisher_ref(_p(F), w, h, _p(lut), lut.shape[0], lut.shape[1], minr, maxr, *pos, *delta, max_value)
+        return F.reshape(3, 3)
 
     def erf(self, x):
         return self.L.oq_erf(x)

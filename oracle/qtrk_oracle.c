@@ -965,6 +965,44 @@ void oq_generate_image_from_lut(fl* img, int w, int h, const float* zlut, int pl
 		}
 }
 
+/* SampleFisherMatrix::Compute, cputrack/FisherMatrix.h:36-104: mean of the
+ * image as ImageData::mean (a float sum, cputrack/utils.h:103-108); derivative
+ * images (p - m) * (1/(2 delta)) (:14-19); elements sum d1*d2/(1e-5 + mu) in
+ * double, y-major (:22-30); the float Matrix3X3 scaled by maxValue / mean. */
+void oq_fisher_ref(fl* F, int w, int h, const float* zlut, int planes, int steps, fl minr, fl maxr, fl px, fl py,
+                   fl pz, fl dx, fl dy, fl dz, fl maxValue)
+{
+	const int n = w * h;
+	fl* buf = (fl*)malloc(sizeof(fl) * (size_t)n * 7);
+	fl *mu = buf, *d[3] = { buf + n, buf + 2 * n, buf + 3 * n }, *m = buf + 4 * n;
+	oq_generate_image_from_lut(mu, w, h, zlut, planes, steps, minr, maxr, px, py, pz, 1, 4);
+	fl s = 0;
+	for (int i = 0; i < n; i++) s += mu[i];
+	const fl maxImg = s / n;
+	const fl delta[3] = { dx, dy, dz };
+	for (int a = 0; a < 3; a++) {
+		const fl qp[3] = { px + (a == 0 ? dx : 0), py + (a == 1 ? dy : 0), pz + (a == 2 ? dz : 0) };
+		const fl qm[3] = { px - (a == 0 ? dx : 0), py - (a == 1 ? dy : 0), pz - (a == 2 ? dz : 0) };
+		oq_generate_image_from_lut(d[a], w, h, zlut, planes, steps, minr, maxr, qp[0], qp[1], qp[2], 1, 4);
+		oq_generate_image_from_lut(m, w, h, zlut, planes, steps, minr, maxr, qm[0], qm[1], qm[2], 1, 4);
+		const fl inv2d = FC(1.0) / (2 * delta[a]);
+		for (int i = 0; i < n; i++) d[a][i] = (d[a][i] - m[i]) * inv2d;
+	}
+	for (int a = 0; a < 3; a++)
+		for (int b = 0; b < 3; b++) {
+			double sum = 0;
+			for (int y = 0; y < h; y++)
+				for (int x = 0; x < w; x++) {
+					const int i = y * w + x;
+					sum += (double)d[a][i] * (double)d[b][i] / ((double)1e-5f + (double)mu[i]);
+				}
+			F[a * 3 + b] = (fl)sum;
+		}
+	const fl scale = maxValue / maxImg;
+	for (int i = 0; i < 9; i++) F[i] *= scale;
+	free(buf);
+}
+
 /* erf as the reference build binds it: a float argument resolves to the C
  * library's erff (the A&S template in cputrack/utils.h:395-417 loses overload
  * resolution to the non-template float overload). */

@@ -156,6 +156,11 @@ void oq_generate_image_from_lut(fl* img, int w, int h, const float* zlut, int pl
                                 fl minr, fl maxr, fl px, fl py, fl pz, int spline, int ovs);
 fl oq_erf(fl x);
 void oq_gaussian_spot(fl* img, int w, int h, fl px, fl py, fl sigma, fl I0, fl bg);
+/* SampleFisherMatrix::Compute (cputrack/FisherMatrix.h:36-104): 3x3 (x, y, z)
+ * Fisher matrix by central differences of GenerateImageFromLUT (spline, ovs 4),
+ * written row-major into F[9]. */
+void oq_fisher_ref(fl* F, int w, int h, const float* zlut, int planes, int steps, fl minr, fl maxr, fl px, fl py,
+                   fl pz, fl dx, fl dy, fl dz, fl maxValue);
 void oq_gauss2d_mle(const fl* img, int w, int h, fl mean, fl x0, fl y0, int iters, fl sigma,
                     fl* out4);
 

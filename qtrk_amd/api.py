@@ -176,6 +176,13 @@ SIGNATURES = {
     "QTrkLUTMatchRun": (C.c_int, [_P, C.c_int, C.c_int]),
     "QTrkLUTMatchSync": (C.c_float, [_P]),
     "QTrkLUTMatchDownload": (C.c_int, [_P, _FP, _IP, _IP, C.c_int]),
+    "QTrkLUTModelCreate": (_P, [_FP, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                C.c_bool]),
+    "QTrkLUTModelFree": (None, [_P]),
+    "QTrkLUTModelForward": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P]),
+    "QTrkLUTModelBackward": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P]),
+    "QTrkLUTModelFitMLE": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
+    "QTrkLUTModelSync": (C.c_int, [_P, _P]),
 }
 
 MATCH_FP32, MATCH_BF16, MATCH_FP16 = 0, 1, 2
@@ -530,3 +537,104 @@ class LUTMatch:
         if self.h:
             lib().QTrkLUTMatchFree(self.h)
             self.h = None
+
+
+def _dptr(t):
+    """Device pointer of a torch CUDA tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class LUTModel:
+    """Bead-image model rendered from radial Z-LUTs: forward, adjoint and Poisson MLE with Fisher/CRLB
+    (include/qtrk_amd.h).  Arrays are torch CUDA tensors; calls run on torch's current stream."""
+
+    def __init__(self, luts, minr, maxr, width, height, ovs=4, spline=True):
+        luts = np.ascontiguousarray(luts, np.float32)
+        if luts.ndim == 2:
+            luts = luts[None]
+        self.count, self.planes, self.steps = luts.shape
+        self.width, self.height = width, height
+        self.h = lib().QTrkLUTModelCreate(_fptr(luts), self.count, self.planes, self.steps, minr, maxr, width,
+                                          height, ovs, bool(spline))
+        check()
+
+    @staticmethod
+    def _stream():
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def _prep(self, theta, lut_index):
+        import torch
+        theta = theta.to(device="cuda", dtype=torch.float32).contiguous()
+        idx = None if lut_index is None else torch.as_tensor(lut_index, dtype=torch.int32, device="cuda").contiguous()
+        return theta, idx
+
+    def forward(self, theta, lut_index=None, jacobian=False):
+        """mu [n][H][W] (and d mu / d theta [n][5][H][W]) for theta [n][5] = (x, y, z, I, bg)."""
+        import torch
+        theta, idx = self._prep(theta, lut_index)
+        n = theta.shape[0]
+        mu = torch.empty((n, self.height, self.width), device="cuda")
+        jac = torch.empty((n, 5, self.height, self.width), device="cuda") if jacobian else None
+        if lib().QTrkLUTModelForward(self.h, _dptr(theta), _dptr(idx), n, _dptr(mu), _dptr(jac), self._stream()):
+            check()
+        return (mu, jac) if jacobian else mu
+
+    def backward(self, theta, grad_mu, lut_index=None):
+        """sum_px grad_mu * d mu / d theta: [n][5]."""
+        import torch
+        theta, idx = self._prep(theta, lut_index)
+        g = grad_mu.to(device="cuda", dtype=torch.float32).contiguous()
+        out = torch.empty((theta.shape[0], 5), device="cuda")
+        if lib().QTrkLUTModelBackward(self.h, _dptr(theta), _dptr(idx), theta.shape[0], _dptr(g), _dptr(out),
+                                      self._stream()):
+            check()
+        return out
+
+    def fit(self, data, theta0, iterations=10, lut_index=None):
+        """Poisson MLE from theta0: (theta [n][5], Fisher [n][5][5] float64, CRLB [n][5], NLL [n])."""
+        import torch
+        theta0, idx = self._prep(theta0, lut_index)
+        d = data.to(device="cuda", dtype=torch.float32).contiguous()
+        n = theta0.shape[0]
+        th = torch.empty((n, 5), device="cuda")
+        fisher = torch.empty((n, 5, 5), device="cuda", dtype=torch.float64)
+        crlb = torch.empty((n, 5), device="cuda")
+        nll = torch.empty(n, device="cuda")
+        if lib().QTrkLUTModelFitMLE(self.h, _dptr(d), _dptr(theta0), _dptr(idx), n, int(iterations), _dptr(th),
+                                    _dptr(fisher), _dptr(crlb), _dptr(nll), self._stream()):
+            check()
+        return th, fisher, crlb, nll
+
+    def render(self, theta, lut_index=None):
+        """mu as a differentiable function of theta (torch autograd through the adjoint kernel)."""
+        return _LUTRender.apply(theta, self, lut_index)
+
+    def free(self):
+        if self.h:
+            lib().QTrkLUTModelFree(self.h)
+            self.h = None
+
+
+def _make_render_fn():
+    try:
+        import torch
+    except ImportError:  # pragma: no cover
+        return None
+
+    class _Render(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, model, lut_index):
+            ctx.model, ctx.lut_index = model, lut_index
+            ctx.save_for_backward(theta)
+            return model.forward(theta, lut_index)
+
+        @staticmethod
+        def backward(ctx, grad_mu):
+            (theta,) = ctx.saved_tensors
+            return ctx.model.backward(theta, grad_mu, ctx.lut_index).to(theta.dtype), None, None
+
+    return _Render
+
+
+_LUTRender = _make_render_fn()

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqtrk.so")
 
-SOURCES = ["localize.hip", "lutmatch.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp", "tracker_common.cpp"]
+SOURCES = ["localize.hip", "lutmatch.hip", "lutmodel.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp",
+           "lutmodel_host.cpp", "tracker_common.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -1,0 +1,414 @@
+// lutmodel.hip -- the LUT image model (forward), its adjoint (backward) and
+// the Poisson MLE with Fisher/CRLB, for gfx950.  One workgroup (256 lanes)
+// per ROI: the z-blended LUT row zi[] and its z-derivative dzi[] are built once
+// per ROI (per fit iteration) in LDS, then lanes take pixels.
+//
+// The forward pass follows GenerateImageFromLUT's float arithmetic operation
+// for operation (this file is built with -ffp-contract=off and IEEE sqrt):
+// B-spline weights as ComputeBSplineWeights (cputrack/CubicBSpline.h:15-24),
+// zi[r] summed over the 4 planes in order, sub-samples summed ox-major, the
+// sum divided by ovs*ovs (cputrack/utils.cpp:325-390).
+#include "lutmodel.hpp"
+
+#include <cfloat>
+
+namespace qtrk {
+
+constexpr int kModelBlock = 256;
+
+struct ModelLds {
+	int zi, dzi, data, red, th, total;
+};
+__host__ __device__ inline int m16(int x) { return (x + 15) & ~15; }
+__host__ __device__ inline ModelLds model_lds(const LutModelParams& P, bool fit)
+{
+	ModelLds L;
+	int o = 0;
+	L.zi = o;    o += m16(P.steps * 4);
+	L.dzi = o;   o += m16(P.steps * 4);
+	L.data = o;  o += fit ? m16(P.width * P.height * 4) : 0;
+	L.red = o;   o += m16(5 * 24 * 8);  // 4 wavefront partials + the totals
+	L.th = o;    o += m16(8 * 4);
+	L.total = o;
+	return L;
+}
+size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit) { return (size_t)model_lds(p, fit).total; }
+
+// zi[r] and dzi[r] for depth z: the cubic B-spline blend of planes iz-1..iz+2
+// and its derivative d/dz (or the linear blend of planes iz, iz+1 and its slope).
+__device__ void build_z_row(const LutModelParams& P, const float* lut, float z, float* zi, float* dzi)
+{
+	const int S = P.steps;
+	// (int)z as the reference computes it, defined for every float (NaN -> 0)
+	const int izt = z != z ? 0 : (int)fminf(fmaxf(z, -2.0e9f), 2.0e9f);
+	if (P.spline) {
+		int iz = izt;
+		if (iz > P.planes - 3) iz = P.planes - 3;
+		if (iz < 1) iz = 1;
+		const float t = z - iz;
+		const float t2 = t * t, t3 = t2 * t, omt = 1 - t;
+		const float w0 = 1.0f / 6 * omt * omt * omt;
+		const float w1 = 1.0f / 6 * (3 * t3 - 6 * t2 + 4);
+		const float w2 = 1.0f / 6 * (-3 * t3 + 3 * t2 + 3 * t + 1);
+		const float w3 = 1.0f / 6 * t3;
+		const float d0 = -omt * omt, d1 = 3 * t2 - 4 * t, d2 = -3 * t2 + 2 * t + 1, d3 = t2;
+		const float* L0 = lut + (size_t)(iz - 1) * S;
+		for (int r = threadIdx.x; r < S; r += kModelBlock) {
+			const float a = L0[r], b = L0[S + r], c = L0[2 * S + r], d = L0[3 * S + r];
+			float v = 0;
+			v += w0 * a;
+			v += w1 * b;
+			v += w2 * c;
+			v += w3 * d;
+			zi[r] = v;
+			dzi[r] = 0.5f * (d0 * a + d1 * b + d2 * c + d3 * d);
+		}
+	} else {
+		const int iz = izt;
+		for (int r = threadIdx.x; r < S; r += kModelBlock) {
+			if (iz < 0) {
+				zi[r] = lut[r];
+				dzi[r] = 0;
+			} else if (iz >= P.planes - 1) {
+				zi[r] = lut[(size_t)(P.planes - 1) * S + r];
+				dzi[r] = 0;
+			} else {
+				const float a = lut[(size_t)iz * S + r], b = lut[(size_t)(iz + 1) * S + r];
+				zi[r] = a + (b - a) * (z - iz);
+				dzi[r] = b - a;
+			}
+		}
+	}
+}
+
+// g and its derivatives with respect to the bead position at pixel (x, y).
+struct PixelG {
+	float g, gx, gy, gz;
+};
+template <bool DERIV>
+__device__ __forceinline__ PixelG pixel_g(const LutModelParams& P, int x, int y, float px, float py, float radialcov,
+                                          float step, const float* zi, const float* dzi)
+{
+	float s = 0, sx = 0, sy = 0, sz = 0;
+	const float rmax = (float)(P.steps - 2);
+	for (int ox = 0; ox < P.ovs; ox++) {
+		for (int oy = 0; oy < P.ovs; oy++) {
+			const float X = x + (ox + 0.5f) * step - px - 0.5f;
+			const float Y = y + (oy + 0.5f) * step - py - 0.5f;
+			const float pr = sqrtf(X * X + Y * Y);
+			const float ru = (pr - P.minr) * radialcov;
+			const bool inside = ru >= 0 && ru <= rmax;  // the slope in r is zero where r is clamped
+			// clamp to [0, steps-2]; a NaN radius (NaN position) maps to bin 0
+			const float r = fminf(fmaxf(ru, 0.0f), rmax);
+			const int i = (int)r;
+			const float fr = r - i;
+			const float a = zi[i], b = zi[i + 1];
+			s += a + (b - a) * fr;
+			if (DERIV) {
+				const float da = dzi[i], db = dzi[i + 1];
+				sz += da + (db - da) * fr;
+				if (inside && pr > 0) {
+					const float k = (b - a) * radialcov / pr;
+					sx -= k * X;
+					sy -= k * Y;
+				}
+			}
+		}
+	}
+	const float inv = 1.0f / (float)(P.ovs * P.ovs);
+	PixelG o;
+	o.g = s / (float)(P.ovs * P.ovs);
+	o.gx = sx * inv;
+	o.gy = sy * inv;
+	o.gz = sz * inv;
+	return o;
+}
+
+// ------------------------------------------------------------ forward / backward
+
+template <bool JAC>
+__global__ __launch_bounds__(kModelBlock) void lutmodel_forward_kernel(LutModelParams P)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	const int roi = blockIdx.x;
+	const ModelLds L = model_lds(P, false);
+	float* zi = (float*)(smem + L.zi);
+	float* dzi = (float*)(smem + L.dzi);
+	const float* th = P.theta + (size_t)roi * kModelParams;
+	const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3], bg = th[4];
+	const int li = P.lut_index ? P.lut_index[roi] : 0;
+	build_z_row(P, P.luts + (size_t)li * P.planes * P.steps, z0, zi, dzi);
+	__syncthreads();
+	const float radialcov = P.steps / (P.maxr - P.minr);
+	const float step = 1.0f / P.ovs;
+	const int npx = P.width * P.height;
+	for (int p = threadIdx.x; p < npx; p += kModelBlock) {
+		const int y = p / P.width, x = p - y * P.width;
+		const PixelG G = pixel_g<JAC>(P, x, y, x0, y0, radialcov, step, zi, dzi);
+		if (P.mu) P.mu[(size_t)roi * npx + p] = bg + I * G.g;
+		if (JAC) {
+			float* J = P.jac + (size_t)roi * kModelParams * npx + p;
+			J[0] = I * G.gx;
+			J[npx] = I * G.gy;
+			J[2 * npx] = I * G.gz;
+			J[3 * npx] = G.g;
+			J[4 * npx] = 1.0f;
+		}
+	}
+}
+
+// Sum of K per-lane float partials over the workgroup in double; the result is
+// in red[0..K) for every lane after the call.
+template <int K>
+__device__ __forceinline__ void block_sum_d(const float (&v)[K], double* red)
+{
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	double d[K];
+#pragma unroll
+	for (int k = 0; k < K; k++) {
+		d[k] = (double)v[k];
+		for (int off = 32; off > 0; off >>= 1) d[k] += __shfl_xor(d[k], off);
+	}
+	__syncthreads();  // red may still be read from the previous call
+	if (lane == 0)
+#pragma unroll
+		for (int k = 0; k < K; k++) red[wid * 24 + k] = d[k];
+	__syncthreads();
+	if (threadIdx.x < K) {
+		double s = 0;
+		for (int w = 0; w < kModelBlock / 64; w++) s += red[w * 24 + threadIdx.x];
+		red[96 + threadIdx.x] = s;
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(kModelBlock) void lutmodel_backward_kernel(LutModelParams P)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	const int roi = blockIdx.x;
+	const ModelLds L = model_lds(P, false);
+	float* zi = (float*)(smem + L.zi);
+	float* dzi = (float*)(smem + L.dzi);
+	double* red = (double*)(smem + L.red);
+	const float* th = P.theta + (size_t)roi * kModelParams;
+	const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3];
+	const int li = P.lut_index ? P.lut_index[roi] : 0;
+	build_z_row(P, P.luts + (size_t)li * P.planes * P.steps, z0, zi, dzi);
+	__syncthreads();
+	const float radialcov = P.steps / (P.maxr - P.minr);
+	const float step = 1.0f / P.ovs;
+	const int npx = P.width * P.height;
+	float acc[kModelParams] = { 0, 0, 0, 0, 0 };
+	for (int p = threadIdx.x; p < npx; p += kModelBlock) {
+		const int y = p / P.width, x = p - y * P.width;
+		const PixelG G = pixel_g<true>(P, x, y, x0, y0, radialcov, step, zi, dzi);
+		const float gm = P.gmu[(size_t)roi * npx + p];
+		acc[0] += gm * I * G.gx;
+		acc[1] += gm * I * G.gy;
+		acc[2] += gm * I * G.gz;
+		acc[3] += gm * G.g;
+		acc[4] += gm;
+	}
+	block_sum_d<kModelParams>(acc, red);
+	if (threadIdx.x < kModelParams) P.grad[(size_t)roi * kModelParams + threadIdx.x] = (float)red[96 + threadIdx.x];
+}
+
+// ------------------------------------------------------------ maximum likelihood
+
+// Cholesky factorisation A = L L^T of a symmetric positive definite 5x5
+// matrix, L stored in the lower triangle of a; false if A is not positive
+// definite.  Fully unrolled so the matrix stays in registers.
+__device__ __forceinline__ bool chol5(double (&a)[5][5])
+{
+#pragma unroll
+	for (int j = 0; j < 5; j++) {
+		double s = a[j][j];
+#pragma unroll
+		for (int k = 0; k < j; k++) s -= a[j][k] * a[j][k];
+		if (!(s > 0)) return false;
+		const double d = sqrt(s);
+		a[j][j] = d;
+#pragma unroll
+		for (int i = j + 1; i < 5; i++) {
+			double t = a[i][j];
+#pragma unroll
+			for (int k = 0; k < j; k++) t -= a[i][k] * a[j][k];
+			a[i][j] = t / d;
+		}
+	}
+	return true;
+}
+
+// Solves L L^T x = b in place.
+__device__ __forceinline__ void chol5_solve(const double (&L)[5][5], double (&b)[5])
+{
+#pragma unroll
+	for (int i = 0; i < 5; i++) {
+		double s = b[i];
+#pragma unroll
+		for (int k = 0; k < i; k++) s -= L[i][k] * b[k];
+		b[i] = s / L[i][i];
+	}
+#pragma unroll
+	for (int i = 4; i >= 0; i--) {
+		double s = b[i];
+#pragma unroll
+		for (int k = i + 1; k < 5; k++) s -= L[k][i] * b[k];
+		b[i] = s / L[i][i];
+	}
+}
+
+// Index of (i, j), i <= j, in the packed upper triangle of a 5x5 matrix.
+__host__ __device__ constexpr int tri5(int i, int j) { return i * 5 - i * (i - 1) / 2 + (j - i); }
+
+// Fisher-scoring Newton iterations on the Poisson negative log-likelihood
+// sum(mu - d ln mu): gradient sum (1 - d/mu) J, Fisher matrix sum J J^T / mu
+// (J = d mu / d theta), step F^-1 (-gradient) solved in FP64.  Steps are
+// limited to 1 px in x/y and 2 planes in z per iteration; I and bg that would
+// turn non-positive are halved instead; z stays within the LUT.  After the
+// last update one more pass evaluates F, its FP64 inverse (the CRLB) and the
+// negative log-likelihood at the fitted parameters.
+__global__ __launch_bounds__(kModelBlock) void lutmodel_fit_kernel(LutModelParams P)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	const int roi = blockIdx.x;
+	const ModelLds L = model_lds(P, true);
+	float* zi = (float*)(smem + L.zi);
+	float* dzi = (float*)(smem + L.dzi);
+	float* dat = (float*)(smem + L.data);
+	double* red = (double*)(smem + L.red);
+	float* th = (float*)(smem + L.th);
+	const int npx = P.width * P.height;
+	for (int p = threadIdx.x; p < npx; p += kModelBlock) dat[p] = P.data[(size_t)roi * npx + p];
+	if (threadIdx.x < kModelParams) th[threadIdx.x] = P.theta[(size_t)roi * kModelParams + threadIdx.x];
+	const int li = P.lut_index ? P.lut_index[roi] : 0;
+	const float* lut = P.luts + (size_t)li * P.planes * P.steps;
+	const float radialcov = P.steps / (P.maxr - P.minr);
+	const float step = 1.0f / P.ovs;
+	__syncthreads();
+
+	for (int it = 0; it <= P.iters; it++) {
+		const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3], bg = th[4];
+		build_z_row(P, lut, z0, zi, dzi);
+		__syncthreads();
+		const bool last = it == P.iters;
+		float v[21];
+#pragma unroll
+		for (int k = 0; k < 21; k++) v[k] = 0;
+		for (int p = threadIdx.x; p < npx; p += kModelBlock) {
+			const int y = p / P.width, x = p - y * P.width;
+			const PixelG G = pixel_g<true>(P, x, y, x0, y0, radialcov, step, zi, dzi);
+			float mu = bg + I * G.g;
+			mu = mu > 1e-6f ? mu : 1e-6f;
+			const float d = dat[p];
+			const float J[5] = { I * G.gx, I * G.gy, I * G.gz, G.g, 1.0f };
+			const float gw = 1.0f - d / mu, fw = 1.0f / mu;
+#pragma unroll
+			for (int i = 0; i < 5; i++) {
+				v[i] += gw * J[i];
+#pragma unroll
+				for (int j = i; j < 5; j++) v[5 + tri5(i, j)] += fw * J[i] * J[j];
+			}
+			if (last) v[20] += mu - d * logf(mu);
+		}
+		block_sum_d<21>(v, red);
+		if (threadIdx.x == 0) {
+			const double* s = red + 96;
+			double F[5][5], g[5];
+			for (int i = 0; i < 5; i++) {
+				g[i] = -s[i];
+				for (int j = i; j < 5; j++) F[i][j] = F[j][i] = s[5 + tri5(i, j)];
+			}
+			double Lc[5][5];
+#pragma unroll
+			for (int i = 0; i < 5; i++)
+#pragma unroll
+				for (int j = 0; j < 5; j++) Lc[i][j] = F[i][j];
+			const bool spd = chol5(Lc);
+			if (!last) {
+				bool finite = spd;
+				if (spd) {
+					chol5_solve(Lc, g);
+#pragma unroll
+					for (int k = 0; k < 5; k++) finite = finite && isfinite(g[k]);
+				}
+				if (finite) {
+					const float lim[3] = { 1.0f, 1.0f, 2.0f };
+					for (int k = 0; k < 3; k++) {
+						const float dk = (float)g[k];
+						th[k] += dk > lim[k] ? lim[k] : (dk < -lim[k] ? -lim[k] : dk);
+					}
+					const float zmax = (float)(P.planes - 1);
+					th[2] = th[2] < 0 ? 0 : (th[2] > zmax ? zmax : th[2]);
+					const float In = th[3] + (float)g[3], bn = th[4] + (float)g[4];
+					th[3] = In > 0 ? In : th[3] * 0.5f;
+					th[4] = bn > 0 ? bn : th[4] * 0.5f;
+				}
+			} else {
+				float* to = P.theta_out + (size_t)roi * kModelParams;
+				for (int k = 0; k < 5; k++) to[k] = th[k];
+				if (P.fisher)
+					for (int i = 0; i < 5; i++)
+						for (int j = 0; j < 5; j++) P.fisher[(size_t)roi * 25 + i * 5 + j] = F[i][j];
+				if (P.nll) P.nll[roi] = (float)s[20];
+				if (P.crlb) {
+					// column k of F^-1 solves F c = e_k; the CRLB is the square root of its k-th entry
+#pragma unroll
+					for (int k = 0; k < 5; k++) {
+						double e[5] = { 0, 0, 0, 0, 0 };
+						e[k] = 1;
+						if (spd) chol5_solve(Lc, e);
+						P.crlb[(size_t)roi * kModelParams + k] = spd && e[k] > 0 ? (float)sqrt(e[k]) : __builtin_nanf("");
+					}
+				}
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------ launches
+
+static hipError_t set_lds(const void* k, size_t lds)
+{
+	if (lds <= 64 * 1024) return hipSuccess;
+	return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+hipError_t launch_lutmodel_forward(const LutModelParams& p, hipStream_t stream)
+{
+	if (p.n <= 0) return hipSuccess;
+	const size_t lds = lutmodel_lds_bytes(p, false);
+	if (p.jac) {
+		hipError_t e = set_lds((const void*)lutmodel_forward_kernel<true>, lds);
+		if (e != hipSuccess) return e;
+		lutmodel_forward_kernel<true><<<p.n, kModelBlock, lds, stream>>>(p);
+	} else {
+		hipError_t e = set_lds((const void*)lutmodel_forward_kernel<false>, lds);
+		if (e != hipSuccess) return e;
+		lutmodel_forward_kernel<false><<<p.n, kModelBlock, lds, stream>>>(p);
+	}
+	return hipGetLastError();
+}
+
+hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream)
+{
+	if (p.n <= 0) return hipSuccess;
+	const size_t lds = lutmodel_lds_bytes(p, false);
+	hipError_t e = set_lds((const void*)lutmodel_backward_kernel, lds);
+	if (e != hipSuccess) return e;
+	lutmodel_backward_kernel<<<p.n, kModelBlock, lds, stream>>>(p);
+	return hipGetLastError();
+}
+
+hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream)
+{
+	if (p.n <= 0) return hipSuccess;
+	const size_t lds = lutmodel_lds_bytes(p, true);
+	hipError_t e = set_lds((const void*)lutmodel_fit_kernel, lds);
+	if (e != hipSuccess) return e;
+	lutmodel_fit_kernel<<<p.n, kModelBlock, lds, stream>>>(p);
+	return hipGetLastError();
+}
+
+}  // namespace qtrk

@@ -1,0 +1,54 @@
+// lutmodel.hpp -- differentiable bead-image model rendered from a radial Z-LUT,
+// its hand-written adjoint, and a Poisson maximum-likelihood fit of
+// theta = (x, y, z, I, bg) with per-ROI 5x5 Fisher matrix and CRLB.
+//
+// Model: mu(px) = bg + I * g(px; x, y, z), where g is the reference's
+// GenerateImageFromLUT (cputrack/utils.cpp:325-390): the mean over ovs x ovs
+// sub-samples of the z-interpolated LUT row (cubic B-spline over planes
+// iz-1..iz+2 with iz = clamp((int)z, 1, planes-3), cputrack/CubicBSpline.h:15-24,
+// or linear), itself interpolated linearly in r = (|p - c| - minr) * steps /
+// (maxr - minr) clamped to [0, steps-2], pixel centres at integer coordinates.
+// The forward pass reproduces that arithmetic operation for operation.
+//
+// Derivatives (the adjoint): d g / d x = mean over sub-samples of
+// (zi[i+1] - zi[i]) * steps/(maxr-minr) * (-(X)/|p - c|), zero where r is
+// clamped; likewise for y; d g / d z = the same interpolation of dzi/dz, the
+// derivative of the B-spline blend (ComputeBSplineDerivatives,
+// cputrack/CubicBSpline.h:41-50); d mu / d I = g; d mu / d bg = 1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace qtrk {
+
+constexpr int kModelParams = 5;  // x, y, z, I, bg
+
+struct LutModelParams {
+	const float* luts;       // [count][planes][steps]
+	int count, planes, steps;
+	float minr, maxr;
+	int width, height, ovs, spline;
+	int n;
+	const float* theta;      // [n][5] model parameters (input of forward/backward, start of the fit)
+	const int* lut_index;    // [n] LUT of each ROI, or null (LUT 0)
+	// forward
+	float* mu;               // [n][height][width] or null
+	float* jac;              // [n][5][height][width] or null
+	// backward: grad[k] = sum_px gmu * d mu / d theta_k
+	const float* gmu;        // [n][height][width]
+	float* grad;             // [n][5]
+	// fit
+	const float* data;       // [n][height][width] photon counts
+	float* theta_out;        // [n][5] fitted parameters (may alias theta)
+	int iters;               // Fisher-scoring Newton iterations
+	double* fisher;          // [n][25] Fisher matrix at the fitted parameters, or null
+	float* crlb;             // [n][5] sqrt(diag(F^-1)), or null
+	float* nll;              // [n] negative log-likelihood (without the data-only log-factorial term), or null
+};
+
+hipError_t launch_lutmodel_forward(const LutModelParams& p, hipStream_t stream);
+hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream);
+hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream);
+size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit);
+
+}  // namespace qtrk

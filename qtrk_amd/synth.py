@@ -44,6 +44,29 @@ def analytic_lut(planes, steps, minr, maxr, period, dperiod=0.04, width0=6.0, dw
     return p.astype(np.float32)
 
 
+def model_lut(planes=200, steps=48, maxr=24.0):
+    """Nonnegative radial LUT for the bead-image model (bins r_i = i * maxr / steps, minr = 0):
+    exp(-r^2 / (2 w(z)^2)) (1 + cos(2 pi r / T(z))) / 2 with w = 3 + 0.02 z px and T = 5 + 0.015 z px,
+    each plane divided by its disc integral sum_i p_i 2 pi r_i dr, so that I in mu = bg + I g is
+    (close to) the total signal photons.  [planes][steps] float32."""
+    r = np.arange(steps, dtype=np.float64) * maxr / steps
+    z = np.arange(planes, dtype=np.float64)[:, None]
+    w = 3.0 + 0.02 * z
+    T = 5.0 + 0.015 * z
+    p = np.exp(-r[None, :] ** 2 / (2 * w * w)) * (1 + np.cos(2 * np.pi * r[None, :] / T)) / 2
+    p /= (p * 2 * np.pi * np.maximum(r[None, :], 0.5 * maxr / steps) * (maxr / steps)).sum(axis=1, keepdims=True)
+    return p.astype(np.float32)
+
+
+def model_params(n, rng, size=48, planes=200):
+    """theta = (x, y, z, I, bg) of the differentiation workload: centre +-3 px, z in [20, 180],
+    I in [2,000, 20,000], bg in [5, 50].  [n][5] float32."""
+    c = size / 2
+    return np.stack([c + rng.uniform(-3, 3, n), c + rng.uniform(-3, 3, n),
+                     rng.uniform(20, min(180, planes - 20), n), rng.uniform(2000, 20000, n),
+                     rng.uniform(5, 50, n)], axis=1).astype(np.float32)
+
+
 def normalize_profiles(lut):
     """Mean 0 / RMS 1 per profile (the normalisation the tracker applies to measured profiles)."""
     x = lut.astype(np.float64)
