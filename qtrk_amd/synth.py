@@ -45,16 +45,16 @@ def analytic_lut(planes, steps, minr, maxr, period, dperiod=0.04, width0=6.0, dw
 
 
 def model_lut(planes=200, steps=48, maxr=24.0):
-    """Nonnegative radial LUT for the bead-image model (bins r_i = i * maxr / steps, minr = 0):
-    exp(-r^2 / (2 w(z)^2)) (1 + cos(2 pi r / T(z))) / 2 with w = 3 + 0.02 z px and T = 5 + 0.015 z px,
-    each plane divided by its disc integral sum_i p_i 2 pi r_i dr, so that I in mu = bg + I g is
-    (close to) the total signal photons.  [planes][steps] float32."""
+    """Nonnegative radial LUT for the bead-image model (bins r_i = i * maxr / steps, minr = 0): the
+    realtime100 ring family made nonnegative, exp(-r^2 / (2 w(z)^2)) (1 + cos(2 pi r / T(z))) / 2 with
+    w = 6 + 0.05 z px and T = 8 + 0.04 z px.  Every plane peaks at 1 (r = 0), so I in mu = bg + I g is
+    the peak signal in photons per pixel, as the reference's rendered LUT images are scaled.
+    [planes][steps] float32."""
     r = np.arange(steps, dtype=np.float64) * maxr / steps
     z = np.arange(planes, dtype=np.float64)[:, None]
-    w = 3.0 + 0.02 * z
-    T = 5.0 + 0.015 * z
+    w = 6.0 + 0.05 * z
+    T = 8.0 + 0.04 * z
     p = np.exp(-r[None, :] ** 2 / (2 * w * w)) * (1 + np.cos(2 * np.pi * r[None, :] / T)) / 2
-    p /= (p * 2 * np.pi * np.maximum(r[None, :], 0.5 * maxr / steps) * (maxr / steps)).sum(axis=1, keepdims=True)
     return p.astype(np.float32)
 
 

@@ -84,17 +84,18 @@ def test_jacobian_and_adjoint_vs_finite_differences(oracle64, lut, gpu):
         grad = m.backward(torch.from_numpy(th), gmu).cpu().numpy()
         for i in range(len(th)):
             # h = 1e-3 differences straddle kinks of the linear-in-r interpolation (up to ~0.3% in
-            # x/y); h = 1e-6 in FP64 resolves the analytic derivative to the GPU's float precision
+            # x/y); h = 1e-6 in FP64 resolves the analytic derivative to its float precision, which
+            # the cancelling x/y-weighted sub-sample sums limit to a few 1e-4 of the norm
             Jf = fd_jacobian(oracle64, lut, th[i])
             Jt = fd_jacobian(oracle64, lut, th[i], h=1e-6)
             for k in range(5):
                 rel = np.linalg.norm(J[i, k] - Jf[k]) / np.linalg.norm(Jf[k])
                 rel_t = np.linalg.norm(J[i, k] - Jt[k]) / np.linalg.norm(Jt[k])
-                assert rel < 1e-2 and rel_t < 1e-4, (i, k, rel, rel_t)
+                assert rel < 1e-2 and rel_t < 5e-4, (i, k, rel, rel_t)
             # adjoint = J^T gmu, and the directional derivative of <gmu, mu(theta)>
-            gf = np.einsum("khw,hw->k", Jf, gmu[i].numpy().astype(np.float64))
+            gf = np.einsum("khw,hw->k", Jt, gmu[i].numpy().astype(np.float64))
             assert np.allclose(grad[i], np.einsum("khw,hw->k", J[i], gmu[i].numpy()), rtol=1e-4, atol=1e-3)
-            assert np.allclose(grad[i], gf, rtol=5e-3, atol=1e-2 * np.abs(gf).max())
+            assert np.allclose(grad[i], gf, rtol=1e-3, atol=1e-4 * np.abs(Jt).sum((1, 2)).max())
     finally:
         m.free()
 
@@ -139,9 +140,9 @@ def test_fisher_crlb_vs_fp64(oracle64, lut, gpu):
         g = oracle64.generate_image_from_lut(W, H, lut, 0.0, MAXR, *map(float, t[:3]))
         t[3], t[4] = mv / g.mean(), 0.0
         _, F0, _, _ = m.fit(m.forward(torch.from_numpy(t[None])), torch.from_numpy(t[None]), iterations=0)
-        # SampleFisherMatrix adds 1e-5 to an image it expects to peak near 1; this LUT integrates to 1
-        # (peak ~0.01), and the maxValue / mean(mu) scaling makes the result independent of the LUT's
-        # scale otherwise, so the reference sees the LUT x 1000
+        # SampleFisherMatrix adds 1e-5 to the image, which matters at this LUT's dark ring minima;
+        # its maxValue / mean(mu) scaling makes the result independent of the LUT's scale otherwise,
+        # so the reference sees the LUT x 1000, where the 1e-5 term is negligible
         Fr = oracle64.fisher_ref(W, H, lut * 1000, 0.0, MAXR, tuple(map(float, t[:3])), (1e-3, 1e-3, 1e-3), mv)
         F0 = F0.cpu().numpy()[0, :3, :3]
         scale = np.sqrt(np.outer(np.diag(Fr), np.diag(Fr)))
