@@ -144,7 +144,8 @@ QTRK_API int QTrkLUTModelForward(QTrkLUTModel* m, const float* theta, const int*
 QTRK_API int QTrkLUTModelBackward(QTrkLUTModel* m, const float* theta, const int* lutIndex, int n,
                                   const float* gradMu, float* gradTheta, void* stream);
 /* Fits theta to photon counts data [n][height][width] from thetaInit with `iterations`
- * Fisher-scoring Newton steps (FP64 5x5 solves); writes theta [n][5] (may alias thetaInit)
+ * Fisher-scoring Newton steps (FP64 5x5 solves); a start with I <= 0 first gets I and bg by
+ * linear least squares at its (x, y, z).  Writes theta [n][5] (may alias thetaInit)
  * and, at the fitted theta, the Fisher matrix [n][25] (FP64), the CRLB sqrt(diag(F^-1)) [n][5]
  * and the negative log-likelihood sum(mu - data*ln(mu)) [n]; those three may be NULL. */
 QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float* thetaInit, const int* lutIndex,
@@ -152,5 +153,52 @@ QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float*
                                 void* stream);
 /* Waits for `stream` (NULL: the default stream). */
 QTRK_API int QTrkLUTModelSync(QTrkLUTModel* m, void* stream);
+
+/*
+ * Multi-tracker pipeline: the full chain of several trackers (each with its
+ * own ROI size, settings, LUT and pixel type) served by one hipGraph launch.
+ * Per tracker: COM -> QI -> radial profile (the tracker's kernel and settings)
+ * -> grouped Z-LUT match of the batch's profiles against the tracker's LUT
+ * (matchPrecision as in QTrkLUTMatch) -> optional LUT-model MLE on the ROI
+ * pixels (mleIterations Fisher-scoring steps from the QI x/y and matched z,
+ * image model from modelLUTs as in QTrkLUTModel, 4x4 sub-samples, B-spline).
+ * The chains run as parallel branches of the graph.
+ *
+ * Order of use: create; add trackers (each with its LUT already set); set
+ * every tracker's jobs (the fixed ROI layout of a batch: frame/bead per ROI;
+ * the grouping by bead is derived from it); upload representative ROIs;
+ * finalize (autotune = true times the launch-shape candidates of each stage on
+ * those ROIs and keeps the fastest; then the graph is captured); then per
+ * batch: fill the ROI buffers (upload or QTrkPipelineDeviceROIs), launch,
+ * sync, download.  Trackers must not change their settings or LUT after
+ * finalize.  No counterpart in the reference (one tracker, one queue).
+ */
+typedef struct QTrkPipeline QTrkPipeline;
+
+QTRK_API QTrkPipeline* QTrkPipelineCreate(void);
+QTRK_API void QTrkPipelineFree(QTrkPipeline* p);
+/* Returns the tracker's slot (>= 0), or -1.  modelLUTs: host [modelCount][modelPlanes][modelSteps]
+ * (one LUT per bead, or one for all), or NULL for no MLE stage. */
+QTRK_API int QTrkPipelineAddTracker(QTrkPipeline* p, QueuedTracker* qtrk, int roisPerBatch, QTRK_PixelDataType pdt,
+                                    int matchPrecision, const float* modelLUTs, int modelCount, int modelPlanes,
+                                    int modelSteps, float modelMinRadius, float modelMaxRadius, int mleIterations);
+QTRK_API int QTrkPipelineSetJobs(QTrkPipeline* p, int slot, const LocalizationJob* jobs, int n);
+QTRK_API void* QTrkPipelineDeviceROIs(QTrkPipeline* p, int slot);
+QTRK_API int QTrkPipelineUploadROIs(QTrkPipeline* p, int slot, const void* rois, int n);
+QTRK_API int QTrkPipelineFinalize(QTrkPipeline* p, bool autotune);
+/* One graph launch over every tracker's batch (asynchronous). */
+QTRK_API int QTrkPipelineLaunch(QTrkPipeline* p);
+/* Waits; event-timed milliseconds of the last launch, negative on error. */
+QTRK_API float QTrkPipelineSync(QTrkPipeline* p);
+/* results: the localization records with z from the grouped match; theta [n][5] and crlb [n][5]
+ * from the MLE stage.  Any output may be NULL. */
+QTRK_API int QTrkPipelineDownload(QTrkPipeline* p, int slot, LocalizationResult* results, float* theta, float* crlb,
+                                  int n);
+/* Camera-paced run: `batches` graph launches, batch i submitted at i * periodMs after the first;
+ * latencyMs[i] = host time from its submission until its completion is observed, gpuMs[i]
+ * (may be NULL) = its event-timed GPU time. */
+QTRK_API int QTrkPipelineStream(QTrkPipeline* p, int batches, float periodMs, float* latencyMs, float* gpuMs);
+/* Per-tracker sizes and the launch shapes chosen (with the candidates' times). */
+QTRK_API int QTrkPipelineGetReport(QTrkPipeline* p, char* dst, int maxLen);
 
 #endif

@@ -183,6 +183,19 @@ SIGNATURES = {
     "QTrkLUTModelBackward": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P]),
     "QTrkLUTModelFitMLE": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "QTrkLUTModelSync": (C.c_int, [_P, _P]),
+    "QTrkPipelineCreate": (_P, []),
+    "QTrkPipelineFree": (None, [_P]),
+    "QTrkPipelineAddTracker": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.c_int,
+                                         C.c_float, C.c_float, C.c_int]),
+    "QTrkPipelineSetJobs": (C.c_int, [_P, C.c_int, _P, C.c_int]),
+    "QTrkPipelineDeviceROIs": (_P, [_P, C.c_int]),
+    "QTrkPipelineUploadROIs": (C.c_int, [_P, C.c_int, _P, C.c_int]),
+    "QTrkPipelineFinalize": (C.c_int, [_P, C.c_bool]),
+    "QTrkPipelineLaunch": (C.c_int, [_P]),
+    "QTrkPipelineSync": (C.c_float, [_P]),
+    "QTrkPipelineDownload": (C.c_int, [_P, C.c_int, _P, _FP, _FP, C.c_int]),
+    "QTrkPipelineStream": (C.c_int, [_P, C.c_int, C.c_float, _FP, _FP]),
+    "QTrkPipelineGetReport": (C.c_int, [_P, C.c_char_p, C.c_int]),
 }
 
 MATCH_FP32, MATCH_BF16, MATCH_FP16 = 0, 1, 2
@@ -638,3 +651,84 @@ def _make_render_fn():
 
 
 _LUTRender = _make_render_fn()
+
+
+class Pipeline:
+    """Several trackers' COM -> QI -> grouped Z-LUT match -> LUT-model MLE chains in one hipGraph
+    (include/qtrk_amd.h)."""
+
+    def __init__(self):
+        self.h = lib().QTrkPipelineCreate()
+        check()
+        self.slots = []
+
+    def add(self, trk, rois_per_batch, pdt, precision=MATCH_BF16, model_luts=None, model_minr=0.0,
+            model_maxr=0.0, iterations=5):
+        ml = None
+        cnt = planes = steps = 0
+        if model_luts is not None:
+            ml = np.ascontiguousarray(model_luts, np.float32)
+            if ml.ndim == 2:
+                ml = ml[None]
+            cnt, planes, steps = ml.shape
+        slot = lib().QTrkPipelineAddTracker(self.h, trk.h, int(rois_per_batch), int(pdt), int(precision),
+                                            None if ml is None else _fptr(ml), cnt, planes, steps, model_minr,
+                                            model_maxr, int(iterations))
+        if slot < 0:
+            check()
+        self.slots.append((trk, rois_per_batch, ml is not None))
+        return slot
+
+    def set_jobs(self, slot, jobs):
+        jobs = np.ascontiguousarray(jobs, JOB_DTYPE)
+        if lib().QTrkPipelineSetJobs(self.h, slot, _ptr(jobs), len(jobs)) != 0:
+            check()
+
+    def upload(self, slot, rois):
+        rois = np.ascontiguousarray(rois)
+        if lib().QTrkPipelineUploadROIs(self.h, slot, _ptr(rois), len(rois)) != 0:
+            check()
+
+    def finalize(self, autotune=True):
+        if lib().QTrkPipelineFinalize(self.h, bool(autotune)) != 0:
+            check()
+
+    def launch(self):
+        if lib().QTrkPipelineLaunch(self.h) != 0:
+            check()
+
+    def sync(self):
+        ms = lib().QTrkPipelineSync(self.h)
+        if ms < 0:
+            check()
+        return ms
+
+    def download(self, slot):
+        """(results with the matched z, MLE theta [n][5] or None, CRLB [n][5] or None)."""
+        n, mle = self.slots[slot][1], self.slots[slot][2]
+        res = np.zeros(n, RESULT_DTYPE)
+        th = np.zeros((n, 5), np.float32) if mle else None
+        cr = np.zeros((n, 5), np.float32) if mle else None
+        if lib().QTrkPipelineDownload(self.h, slot, _ptr(res), None if th is None else _fptr(th),
+                                      None if cr is None else _fptr(cr), n) != 0:
+            check()
+        return res, th, cr
+
+    def stream(self, batches, period_ms):
+        """Camera-paced launches: (latency ms per batch, GPU ms per batch)."""
+        lat = np.zeros(batches, np.float32)
+        gpu = np.zeros(batches, np.float32)
+        if lib().QTrkPipelineStream(self.h, int(batches), float(period_ms), _fptr(lat), _fptr(gpu)) != 0:
+            check()
+        return lat, gpu
+
+    def report(self):
+        buf = C.create_string_buffer(8192)
+        if lib().QTrkPipelineGetReport(self.h, buf, 8192) != 0:
+            check()
+        return buf.value.decode()
+
+    def free(self):
+        if self.h:
+            lib().QTrkPipelineFree(self.h)
+            self.h = None

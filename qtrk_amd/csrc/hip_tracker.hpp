@@ -9,6 +9,7 @@
 #pragma once
 
 #include "QueuedTracker.h"
+#include "devbuf.hpp"
 #include "localize.hpp"
 
 #include <hip/hip_runtime.h>
@@ -20,24 +21,6 @@
 #include <vector>
 
 namespace qtrk {
-
-struct DeviceBuffer {
-	void* p = nullptr;
-	size_t bytes = 0;
-	void ensure(size_t n);
-	void release();
-	template <typename T> T* as() const { return (T*)p; }
-	~DeviceBuffer() { release(); }
-};
-
-struct PinnedBuffer {
-	void* p = nullptr;
-	size_t bytes = 0;
-	void ensure(size_t n);
-	void release();
-	template <typename T> T* as() const { return (T*)p; }
-	~PinnedBuffer() { release(); }
-};
 
 class QueuedHIPTracker : public QueuedTracker {
 public:
@@ -75,6 +58,12 @@ public:
 	int Device() const { return dev; }
 	int LocalizeMode() const { return localizeMode; }
 	void WaitLaunched();
+	// the radial Z-LUT [ZCount()][ZPlanes()][zlut_radialsteps] and weights in GPU memory (null when unset)
+	const float* DeviceZLUT() const { return zcount > 0 ? d_zlut.as<float>() : nullptr; }
+	const float* DeviceZCmp() const { return has_zcmp ? d_zcmp.as<float>() : nullptr; }
+	int ZCount() const { return zcount; }
+	int ZPlanes() const { return zplanes; }
+	bool HasCalibration() const { return has_off || has_gain; }
 
 private:
 	struct Slot {

@@ -193,7 +193,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
 	int bead, chunk;
 	xcd_map(blockIdx.x, P.chunks, bead, chunk);
 	if (bead >= P.beads) return;
-	const int s0 = P.start[bead] + chunk * kMatchFrameTile;
+	const int s0 = P.start[bead] + chunk * FT * 32;
 	const int rem = P.start[bead + 1] - s0;
 	const int nf = rem < FT * 32 ? rem : FT * 32;
 	if (nf <= 0) return;
@@ -613,7 +613,7 @@ static hipError_t launch_mfma_prec(const LutMatchParams& q, int FT, int grid, hi
 	return FT == 1 ? launch_mfma<PREC, 1, 16>(q, grid, stream) : launch_mfma<PREC, 2, 16>(q, grid, stream);
 }
 
-hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream)
+hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream, int frame_tile)
 {
 	if (p.beads <= 0 || max_per_bead <= 0) return hipSuccess;
 	LutMatchParams q = p;
@@ -636,18 +636,20 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 		return hipGetLastError();
 	}
 	if (p.res > kMatchMaxRes) return hipErrorInvalidValue;
-	const int FT = max_per_bead > 32 ? 2 : 1;
-	q.chunks = (max_per_bead + kMatchFrameTile - 1) / kMatchFrameTile;
+	if (frame_tile != 0 && frame_tile != 32 && frame_tile != 64) return hipErrorInvalidValue;
+	const int FT = frame_tile ? frame_tile / 32 : (max_per_bead > 32 ? 2 : 1);
+	q.chunks = (max_per_bead + FT * 32 - 1) / (FT * 32);
 	const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
 	if (prec == kMatchBF16) return launch_mfma_prec<kMatchBF16>(q, FT, grid, stream);
 	if (prec == kMatchFP16) return launch_mfma_prec<kMatchFP16>(q, FT, grid, stream);
 	return hipErrorInvalidValue;
 }
 
-size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead)
+size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile)
 {
 	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes).total;
-	return (size_t)mfma_lds(max_per_bead > 32 ? 2 : 1, res, ksteps).total;
+	const int FT = frame_tile ? frame_tile / 32 : (max_per_bead > 32 ? 2 : 1);
+	return (size_t)mfma_lds(FT, res, ksteps).total;
 }
 
 }  // namespace qtrk

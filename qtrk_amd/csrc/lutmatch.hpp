@@ -22,6 +22,8 @@
 //     result (best plane and z) is identical to the reference's.
 #pragma once
 
+#include "devbuf.hpp"
+
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -30,7 +32,6 @@ namespace qtrk {
 enum : int { kMatchFP32 = 0, kMatchBF16 = 1, kMatchFP16 = 2 };
 constexpr int kMatchTopK = 8;
 constexpr int kMatchMaxRes = 256;     // radial steps supported by the matrix-core path
-constexpr int kMatchFrameTile = 64;   // profiles per workgroup, matrix-core path
 constexpr int kMatchExactTile = 16;   // profiles per workgroup, exact path
 
 struct LutMatchParams {
@@ -61,9 +62,27 @@ struct LutMatchParams {
 hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, float* norm, float* mean,
                                    float* raw_mean, hipStream_t stream);
 // One grouped match of all beads' profiles (prec: kMatchFP32 / BF16 / FP16);
-// max_per_bead bounds start[b+1] - start[b].
-hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream);
+// max_per_bead bounds start[b+1] - start[b].  frame_tile: profiles per
+// matrix-core workgroup, 32 or 64 (0: 64 when a bead has more than 32).
+hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream,
+                           int frame_tile = 0);
 // Dynamic LDS bytes of that launch.
-size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead);
+size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile = 0);
+
+// Device-resident LUT of a grouped match in every precision's format: the FP32
+// LUT and weights, and per precision (built on first use) the tile means,
+// residual fragments and plane norms of the matrix-core path.
+struct LutMatchTables {
+	int beads = 0, planes = 0, res = 0, ptiles = 0, ksteps = 0, mres = 0, idx_bits = 0;
+	DeviceBuffer lut, w, norm, mean, raw_mean, frag[3];
+	bool have_w = false, prepared[3] = {};
+
+	LutMatchTables(int beads, int planes, int res);
+	// luts [beads][planes][res] and weights [res] (or null), host or device memory
+	void set(const float* luts, const float* weights, bool device_src, hipStream_t stream);
+	void prepare(int prec, hipStream_t stream);
+	// parameters with the LUT fields filled (profiles, grouping and outputs left null)
+	LutMatchParams params(int prec) const;
+};
 
 }  // namespace qtrk

@@ -10,30 +10,75 @@
 
 using namespace qtrk;
 
+LutMatchTables::LutMatchTables(int beads_, int planes_, int res_)
+{
+	if (beads_ <= 0 || planes_ <= 0 || res_ <= 0) fail("beads, planes and radialsteps must be > 0");
+	if (planes_ > 8192) fail("at most 8192 planes are supported");
+	beads = beads_;
+	planes = planes_;
+	res = res_;
+	ptiles = (planes + 31) / 32;
+	ksteps = (res + 15) / 16;
+	mres = (res + 7) & ~7;
+	idx_bits = 0;
+	while ((1 << idx_bits) < ptiles * 32) idx_bits++;
+	lut.ensure((size_t)beads * planes * res * sizeof(float));
+	w.ensure((size_t)res * sizeof(float));
+}
+
+void LutMatchTables::set(const float* luts, const float* weights, bool device_src, hipStream_t stream)
+{
+	const hipMemcpyKind k = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+	HIP_CHECK(hipMemcpyAsync(lut.p, luts, (size_t)beads * planes * res * sizeof(float), k, stream));
+	have_w = weights != nullptr;
+	if (weights) HIP_CHECK(hipMemcpyAsync(w.p, weights, (size_t)res * sizeof(float), k, stream));
+	HIP_CHECK(hipStreamSynchronize(stream));
+	prepared[1] = prepared[2] = false;
+}
+
+void LutMatchTables::prepare(int prec, hipStream_t stream)
+{
+	if (prec == kMatchFP32 || prepared[prec]) return;
+	if (res > kMatchMaxRes) fail("the BF16/FP16 match supports at most 256 radial steps");
+	frag[prec].ensure((size_t)beads * ptiles * ksteps * 64 * 16);
+	norm.ensure((size_t)beads * ptiles * 32 * sizeof(float));
+	mean.ensure((size_t)beads * ptiles * mres * sizeof(float));
+	raw_mean.ensure((size_t)beads * ptiles * mres * sizeof(float));
+	HIP_CHECK(launch_lutmatch_prepare(params(prec), prec, frag[prec].as<uint4>(), norm.as<float>(), mean.as<float>(),
+	                                  raw_mean.as<float>(), stream));
+	prepared[prec] = true;
+}
+
+LutMatchParams LutMatchTables::params(int prec) const
+{
+	LutMatchParams p = {};
+	p.beads = beads;
+	p.planes = planes;
+	p.res = res;
+	p.ptiles = ptiles;
+	p.ksteps = ksteps;
+	p.idx_bits = idx_bits;
+	p.lut = lut.as<float>();
+	p.frag = prec > 0 ? frag[prec].as<uint4>() : nullptr;
+	p.norm = norm.as<float>();
+	p.mean = mean.as<float>();
+	p.mres = mres;
+	p.w = have_w ? w.as<float>() : nullptr;
+	return p;
+}
+
 struct QTrkLUTMatch {
 	int dev = 0;
-	int beads = 0, planes = 0, res = 0, ptiles = 0, ksteps = 0, idx_bits = 0;
-	DeviceBuffer lut, w, norm, mean, raw_mean, frag[3], prof, start, z, best, cand;
-	bool have_lut = false, have_w = false, prepared[3] = {};
+	std::unique_ptr<LutMatchTables> tab;
+	DeviceBuffer prof, start, z, best, cand;
+	bool have_lut = false;
 	int frames = 0;
 	hipStream_t stream = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 
 	LutMatchParams params(int prec) const
 	{
-		LutMatchParams p = {};
-		p.beads = beads;
-		p.planes = planes;
-		p.res = res;
-		p.ptiles = ptiles;
-		p.ksteps = ksteps;
-		p.idx_bits = idx_bits;
-		p.lut = lut.as<float>();
-		p.frag = prec > 0 ? frag[prec].as<uint4>() : nullptr;
-		p.norm = norm.as<float>();
-		p.mean = mean.as<float>();
-		p.mres = (res + 7) & ~7;
-		p.w = have_w ? w.as<float>() : nullptr;
+		LutMatchParams p = tab->params(prec);
 		p.prof = prof.as<float>();
 		p.start = start.as<int>();
 		p.rowidx = nullptr;
@@ -46,14 +91,14 @@ struct QTrkLUTMatch {
 	void set_frames(int f)
 	{
 		if (f <= 0) fail("framesPerBead must be > 0");
-		const size_t n = (size_t)beads * f;
-		prof.ensure(n * res * sizeof(float));
+		const size_t n = (size_t)tab->beads * f;
+		prof.ensure(n * tab->res * sizeof(float));
 		z.ensure(n * sizeof(float));
 		best.ensure(n * sizeof(int));
 		cand.ensure(n * kMatchTopK * sizeof(int));
 		if (f != frames) {
-			std::vector<int> s(beads + 1);
-			for (int b = 0; b <= beads; b++) s[b] = b * f;
+			std::vector<int> s(tab->beads + 1);
+			for (int b = 0; b <= tab->beads; b++) s[b] = b * f;
 			HIP_CHECK(hipMemcpyAsync(start.p, s.data(), s.size() * sizeof(int), hipMemcpyHostToDevice, stream));
 			HIP_CHECK(hipStreamSynchronize(stream));
 			frames = f;
@@ -65,20 +110,9 @@ QTRK_API QTrkLUTMatch* QTrkLUTMatchCreate(int beads, int planes, int radialsteps
 {
 	QTrkLUTMatch* m = nullptr;
 	GUARD({
-		if (beads <= 0 || planes <= 0 || radialsteps <= 0) fail("beads, planes and radialsteps must be > 0");
-		if (planes > 8192) fail("at most 8192 planes are supported");
 		std::unique_ptr<QTrkLUTMatch> o(new QTrkLUTMatch());
 		HIP_CHECK(hipGetDevice(&o->dev));
-		o->beads = beads;
-		o->planes = planes;
-		o->res = radialsteps;
-		o->ptiles = (planes + 31) / 32;
-		o->ksteps = (radialsteps + 15) / 16;
-		int bits = 0;
-		while ((1 << bits) < o->ptiles * 32) bits++;
-		o->idx_bits = bits;
-		o->lut.ensure((size_t)beads * planes * radialsteps * sizeof(float));
-		o->w.ensure((size_t)radialsteps * sizeof(float));
+		o->tab.reset(new LutMatchTables(beads, planes, radialsteps));
 		o->start.ensure((size_t)(beads + 1) * sizeof(int));
 		HIP_CHECK(hipStreamCreateWithFlags(&o->stream, hipStreamNonBlocking));
 		HIP_CHECK(hipEventCreate(&o->e0));
@@ -104,15 +138,8 @@ QTRK_API int QTrkLUTMatchSetLUT(QTrkLUTMatch* m, const float* luts, const float*
 	return GUARD({
 		if (!m || !luts) fail("null argument");
 		HIP_CHECK(hipSetDevice(m->dev));
-		HIP_CHECK(hipMemcpyAsync(m->lut.p, luts, (size_t)m->beads * m->planes * m->res * sizeof(float),
-		                         hipMemcpyHostToDevice, m->stream));
-		m->have_w = radialweights != nullptr;
-		if (radialweights)
-			HIP_CHECK(hipMemcpyAsync(m->w.p, radialweights, (size_t)m->res * sizeof(float), hipMemcpyHostToDevice,
-			                         m->stream));
-		HIP_CHECK(hipStreamSynchronize(m->stream));
+		m->tab->set(luts, radialweights, false, m->stream);
 		m->have_lut = true;
-		m->prepared[1] = m->prepared[2] = false;
 	}) ? 0 : -1;
 }
 
@@ -134,7 +161,7 @@ QTRK_API int QTrkLUTMatchUploadProfiles(QTrkLUTMatch* m, const float* profiles, 
 		if (!m || !profiles) fail("null argument");
 		HIP_CHECK(hipSetDevice(m->dev));
 		m->set_frames(framesPerBead);
-		HIP_CHECK(hipMemcpyAsync(m->prof.p, profiles, (size_t)m->beads * framesPerBead * m->res * sizeof(float),
+		HIP_CHECK(hipMemcpyAsync(m->prof.p, profiles, (size_t)m->tab->beads * framesPerBead * m->tab->res * sizeof(float),
 		                         hipMemcpyHostToDevice, m->stream));
 		HIP_CHECK(hipStreamSynchronize(m->stream));
 	}) ? 0 : -1;
@@ -149,22 +176,12 @@ QTRK_API int QTrkLUTMatchRun(QTrkLUTMatch* m, int precision, int repeats)
 		if (m->frames <= 0) fail("no profiles set");
 		if (repeats < 1) repeats = 1;
 		HIP_CHECK(hipSetDevice(m->dev));
+		const LutMatchTables& t = *m->tab;
 		if (precision == QTRK_MATCH_FP32) {
-			if (lutmatch_lds_bytes(precision, m->res, m->planes, m->ksteps, m->frames) > 160 * 1024)
+			if (lutmatch_lds_bytes(precision, t.res, t.planes, t.ksteps, m->frames) > 160 * 1024)
 				fail("too many planes for the FP32 match (at most about 1,200)");
 		} else {
-			if (m->res > kMatchMaxRes) fail("the BF16/FP16 match supports at most 256 radial steps");
-			if (!m->prepared[precision]) {
-				m->frag[precision].ensure((size_t)m->beads * m->ptiles * m->ksteps * 64 * 16);
-				m->norm.ensure((size_t)m->beads * m->ptiles * 32 * sizeof(float));
-				const size_t mres = (size_t)((m->res + 7) & ~7);
-				m->mean.ensure((size_t)m->beads * m->ptiles * mres * sizeof(float));
-				m->raw_mean.ensure((size_t)m->beads * m->ptiles * mres * sizeof(float));
-				HIP_CHECK(launch_lutmatch_prepare(m->params(precision), precision, m->frag[precision].as<uint4>(),
-				                                  m->norm.as<float>(), m->mean.as<float>(), m->raw_mean.as<float>(),
-				                                  m->stream));
-				m->prepared[precision] = true;
-			}
+			m->tab->prepare(precision, m->stream);
 		}
 		const LutMatchParams p = m->params(precision);
 		HIP_CHECK(hipEventRecord(m->e0, m->stream));
@@ -189,7 +206,7 @@ QTRK_API int QTrkLUTMatchDownload(QTrkLUTMatch* m, float* z, int* bestPlane, int
 {
 	return GUARD({
 		if (!m) fail("null argument");
-		if (n < 0 || (size_t)n > (size_t)m->beads * m->frames) fail("n exceeds beads * framesPerBead");
+		if (n < 0 || (size_t)n > (size_t)m->tab->beads * m->frames) fail("n exceeds beads * framesPerBead");
 		HIP_CHECK(hipSetDevice(m->dev));
 		HIP_CHECK(hipStreamSynchronize(m->stream));
 		if (z) HIP_CHECK(hipMemcpy(z, m->z.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));

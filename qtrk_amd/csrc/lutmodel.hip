@@ -11,10 +11,11 @@
 #include "lutmodel.hpp"
 
 #include <cfloat>
+#include <cstdint>
 
 namespace qtrk {
 
-constexpr int kModelBlock = 256;
+constexpr int kModelBlock = 256;  // lanes per ROI of the forward/backward passes; the fit takes 64-256
 
 struct ModelLds {
 	int zi, dzi, data, red, th, total;
@@ -53,7 +54,7 @@ __device__ void build_z_row(const LutModelParams& P, const float* lut, float z, 
 		const float w3 = 1.0f / 6 * t3;
 		const float d0 = -omt * omt, d1 = 3 * t2 - 4 * t, d2 = -3 * t2 + 2 * t + 1, d3 = t2;
 		const float* L0 = lut + (size_t)(iz - 1) * S;
-		for (int r = threadIdx.x; r < S; r += kModelBlock) {
+		for (int r = threadIdx.x; r < S; r += blockDim.x) {
 			const float a = L0[r], b = L0[S + r], c = L0[2 * S + r], d = L0[3 * S + r];
 			float v = 0;
 			v += w0 * a;
@@ -65,7 +66,7 @@ __device__ void build_z_row(const LutModelParams& P, const float* lut, float z, 
 		}
 	} else {
 		const int iz = izt;
-		for (int r = threadIdx.x; r < S; r += kModelBlock) {
+		for (int r = threadIdx.x; r < S; r += blockDim.x) {
 			if (iz < 0) {
 				zi[r] = lut[r];
 				dzi[r] = 0;
@@ -79,6 +80,13 @@ __device__ void build_z_row(const LutModelParams& P, const float* lut, float z, 
 			}
 		}
 	}
+}
+
+__device__ __forceinline__ int lut_of(const LutModelParams& P, int roi)
+{
+	if (!P.lut_index) return 0;
+	const int li = P.lut_index[(size_t)roi * P.lut_index_stride];
+	return li >= 0 && li < P.count ? li : 0;
 }
 
 // g and its derivatives with respect to the bead position at pixel (x, y).
@@ -136,8 +144,7 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_forward_kernel(LutModelP
 	float* dzi = (float*)(smem + L.dzi);
 	const float* th = P.theta + (size_t)roi * kModelParams;
 	const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3], bg = th[4];
-	const int li = P.lut_index ? P.lut_index[roi] : 0;
-	build_z_row(P, P.luts + (size_t)li * P.planes * P.steps, z0, zi, dzi);
+	build_z_row(P, P.luts + (size_t)lut_of(P, roi) * P.planes * P.steps, z0, zi, dzi);
 	__syncthreads();
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;
@@ -176,7 +183,7 @@ __device__ __forceinline__ void block_sum_d(const float (&v)[K], double* red)
 	__syncthreads();
 	if (threadIdx.x < K) {
 		double s = 0;
-		for (int w = 0; w < kModelBlock / 64; w++) s += red[w * 24 + threadIdx.x];
+		for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w * 24 + threadIdx.x];
 		red[96 + threadIdx.x] = s;
 	}
 	__syncthreads();
@@ -192,8 +199,7 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_backward_kernel(LutModel
 	double* red = (double*)(smem + L.red);
 	const float* th = P.theta + (size_t)roi * kModelParams;
 	const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3];
-	const int li = P.lut_index ? P.lut_index[roi] : 0;
-	build_z_row(P, P.luts + (size_t)li * P.planes * P.steps, z0, zi, dzi);
+	build_z_row(P, P.luts + (size_t)lut_of(P, roi) * P.planes * P.steps, z0, zi, dzi);
 	__syncthreads();
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;
@@ -268,10 +274,11 @@ __host__ __device__ constexpr int tri5(int i, int j) { return i * 5 - i * (i - 1
 // turn non-positive are halved instead; z stays within the LUT.  After the
 // last update one more pass evaluates F, its FP64 inverse (the CRLB) and the
 // negative log-likelihood at the fitted parameters.
-__global__ __launch_bounds__(kModelBlock) void lutmodel_fit_kernel(LutModelParams P)
+__global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
 	const int roi = blockIdx.x;
+	const int nt = blockDim.x;
 	const ModelLds L = model_lds(P, true);
 	float* zi = (float*)(smem + L.zi);
 	float* dzi = (float*)(smem + L.dzi);
@@ -279,13 +286,46 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_fit_kernel(LutModelParam
 	double* red = (double*)(smem + L.red);
 	float* th = (float*)(smem + L.th);
 	const int npx = P.width * P.height;
-	for (int p = threadIdx.x; p < npx; p += kModelBlock) dat[p] = P.data[(size_t)roi * npx + p];
+	const size_t d0 = (size_t)roi * npx;
+	if (P.data_pdt == 1)
+		for (int p = threadIdx.x; p < npx; p += nt) dat[p] = (float)((const uint16_t*)P.data)[d0 + p];
+	else if (P.data_pdt == 0)
+		for (int p = threadIdx.x; p < npx; p += nt) dat[p] = (float)((const uint8_t*)P.data)[d0 + p];
+	else
+		for (int p = threadIdx.x; p < npx; p += nt) dat[p] = ((const float*)P.data)[d0 + p];
 	if (threadIdx.x < kModelParams) th[threadIdx.x] = P.theta[(size_t)roi * kModelParams + threadIdx.x];
-	const int li = P.lut_index ? P.lut_index[roi] : 0;
-	const float* lut = P.luts + (size_t)li * P.planes * P.steps;
+	const float* lut = P.luts + (size_t)lut_of(P, roi) * P.planes * P.steps;
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;
 	__syncthreads();
+
+	if (!(th[3] > 0)) {
+		// start: I and bg by linear least squares of d = bg + I g at the start position
+		build_z_row(P, lut, th[2], zi, dzi);
+		__syncthreads();
+		float v[4] = { 0, 0, 0, 0 };
+		for (int p = threadIdx.x; p < npx; p += nt) {
+			const int y = p / P.width, x = p - y * P.width;
+			const PixelG G = pixel_g<false>(P, x, y, th[0], th[1], radialcov, step, zi, dzi);
+			const float d = dat[p];
+			v[0] += G.g;
+			v[1] += G.g * G.g;
+			v[2] += G.g * d;
+			v[3] += d;
+		}
+		block_sum_d<4>(v, red);
+		if (threadIdx.x == 0) {
+			const double sg = red[96], sgg = red[97], sgd = red[98], sd = red[99], N = (double)npx;
+			const double det = sgg * N - sg * sg;
+			double I = det > 0 ? (N * sgd - sg * sd) / det : 0.0;
+			double bg = det > 0 ? (sgg * sd - sg * sgd) / det : sd / N;
+			if (!(bg > 1e-3)) bg = 1e-3;
+			if (!(I > 0)) I = sg > 0 ? fmax((sd - bg * N) / sg, 1.0) : 1.0;
+			th[3] = (float)I;
+			th[4] = (float)bg;
+		}
+		__syncthreads();
+	}
 
 	for (int it = 0; it <= P.iters; it++) {
 		const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3], bg = th[4];
@@ -295,7 +335,7 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_fit_kernel(LutModelParam
 		float v[21];
 #pragma unroll
 		for (int k = 0; k < 21; k++) v[k] = 0;
-		for (int p = threadIdx.x; p < npx; p += kModelBlock) {
+		for (int p = threadIdx.x; p < npx; p += nt) {
 			const int y = p / P.width, x = p - y * P.width;
 			const PixelG G = pixel_g<true>(P, x, y, x0, y0, radialcov, step, zi, dzi);
 			float mu = bg + I * G.g;
@@ -401,13 +441,14 @@ hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream)
 	return hipGetLastError();
 }
 
-hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream)
+hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int block)
 {
 	if (p.n <= 0) return hipSuccess;
+	if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
 	const size_t lds = lutmodel_lds_bytes(p, true);
 	hipError_t e = set_lds((const void*)lutmodel_fit_kernel, lds);
 	if (e != hipSuccess) return e;
-	lutmodel_fit_kernel<<<p.n, kModelBlock, lds, stream>>>(p);
+	lutmodel_fit_kernel<<<p.n, block, lds, stream>>>(p);
 	return hipGetLastError();
 }
 

@@ -30,7 +30,8 @@ struct LutModelParams {
 	int width, height, ovs, spline;
 	int n;
 	const float* theta;      // [n][5] model parameters (input of forward/backward, start of the fit)
-	const int* lut_index;    // [n] LUT of each ROI, or null (LUT 0)
+	const int* lut_index;    // LUT of ROI i at lut_index[i * lut_index_stride], or null (LUT 0)
+	int lut_index_stride;    // in ints (1 for a plain array, 6 for the zlutIndex of 24-byte job records)
 	// forward
 	float* mu;               // [n][height][width] or null
 	float* jac;              // [n][5][height][width] or null
@@ -38,9 +39,11 @@ struct LutModelParams {
 	const float* gmu;        // [n][height][width]
 	float* grad;             // [n][5]
 	// fit
-	const float* data;       // [n][height][width] photon counts
+	const void* data;        // [n][height][width] photon counts
+	int data_pdt;            // pixel type of data: 0 u8, 1 u16, 2 f32
 	float* theta_out;        // [n][5] fitted parameters (may alias theta)
-	int iters;               // Fisher-scoring Newton iterations
+	int iters;               // Fisher-scoring Newton iterations; a start with I <= 0 gets I and bg
+	                         // by linear least squares at its (x, y, z) first
 	double* fisher;          // [n][25] Fisher matrix at the fitted parameters, or null
 	float* crlb;             // [n][5] sqrt(diag(F^-1)), or null
 	float* nll;              // [n] negative log-likelihood (without the data-only log-factorial term), or null
@@ -48,7 +51,8 @@ struct LutModelParams {
 
 hipError_t launch_lutmodel_forward(const LutModelParams& p, hipStream_t stream);
 hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream);
-hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream);
+// block: lanes per ROI (64, 128 or 256)
+hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int block = 256);
 size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit);
 
 }  // namespace qtrk

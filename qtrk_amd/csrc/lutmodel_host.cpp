@@ -23,6 +23,7 @@ struct QTrkLUTModel {
 		p.luts = luts.as<float>();
 		p.theta = theta;
 		p.lut_index = lutIndex;
+		p.lut_index_stride = 1;
 		p.n = n;
 		return p;
 	}
@@ -112,6 +113,7 @@ QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float*
 		HIP_CHECK(hipSetDevice(m->dev));
 		LutModelParams p = m->params(thetaInit, lutIndex, n);
 		p.data = data;
+		p.data_pdt = 2;
 		p.theta_out = theta;
 		p.iters = iterations;
 		p.fisher = fisher;

@@ -67,6 +67,45 @@ def model_params(n, rng, size=48, planes=200):
                      rng.uniform(5, 50, n)], axis=1).astype(np.float32)
 
 
+def scaled_model_lut(planes, steps, size):
+    """model_lut's family for a size x size ROI: widths and ring periods scaled by size/48 and the
+    family's z range spread over `planes` planes; bins at i * (size/2) / steps."""
+    k = size / 48.0
+    r = np.arange(steps, dtype=np.float64) * (size / 2) / steps
+    z = np.arange(planes, dtype=np.float64)[:, None] * 200.0 / planes
+    w = k * (6.0 + 0.05 * z)
+    T = k * (8.0 + 0.04 * z)
+    return (np.exp(-r[None, :] ** 2 / (2 * w * w)) * (1 + np.cos(2 * np.pi * r[None, :] / T)) / 2).astype(np.float32)
+
+
+def batching_tracker(size, n, beads, planes, steps, rng, render=True):
+    """One tracker of the batching workload: size x size uint16 ROIs (n = beads x frames, frame-major),
+    per-tracker image LUT of `planes` x `steps` (scaled_model_lut), bead centre +-size/16 px, z in the
+    middle 80% of the planes, I (peak photons/px) 2,000-20,000, bg 5-50, Poisson noise.  Returns
+    dict(settings, model_lut, lut (normalised profiles for the tracker's compare), rois, jobs, truth)."""
+    R = size / 2
+    st = api.default_settings(width=size, height=size, qi_minradius=1.0, qi_roi_coverage=1.0,
+                              qi_radial_coverage=1.4, qi_angular_coverage=1.5, qi_angstep_factor=1.0,
+                              qi_iterations=5, zlut_minradius=0.0, zlut_roi_coverage=1.0,
+                              zlut_radial_coverage=steps / R, zlut_angular_coverage=0.7)
+    cfg = api.computed_config(st)
+    assert cfg.zlut_radialsteps == steps, (cfg.zlut_radialsteps, steps)
+    model = scaled_model_lut(planes, steps, size)
+    bead = (np.arange(n) % beads).astype(np.int32)
+    off = size / 16
+    truth = np.stack([R + rng.uniform(-off, off, n), R + rng.uniform(-off, off, n),
+                      rng.uniform(0.1 * planes, 0.9 * planes, n), rng.uniform(2000, 20000, n),
+                      rng.uniform(5, 50, n)], axis=1).astype(np.float32)
+    rois = None
+    if render:
+        g = api.generate_images_from_lut(model[None], truth[:, :3], np.zeros(n, np.int32), size, size, 0.0, R)
+        lam = truth[:, 4, None, None] + truth[:, 3, None, None] * g.astype(np.float64)
+        rois = np.clip(rng.poisson(lam), 0, 65535).astype(np.uint16)
+    jobs = api.make_jobs(n, frame=(np.arange(n) // beads).astype(np.uint32), zlut_index=bead)
+    lut = np.repeat(normalize_profiles(model)[None], beads, axis=0)
+    return dict(settings=st, cfg=cfg, model_lut=model, lut=lut, rois=rois, jobs=jobs, truth=truth)
+
+
 def normalize_profiles(lut):
     """Mean 0 / RMS 1 per profile (the normalisation the tracker applies to measured profiles)."""
     x = lut.astype(np.float64)
