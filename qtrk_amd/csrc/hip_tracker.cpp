@@ -293,6 +293,7 @@ QueuedHIPTracker::Slot* QueuedHIPTracker::AcquireSlot(std::unique_lock<std::mute
 			if (s->state == Slot::Free) {
 				s->state = Slot::Filling;
 				s->n = 0;
+				s->frame = false;
 				return s;
 			}
 		// Queue full: block like the reference (cputrack/QueuedCPUTracker.cpp:437-440)
@@ -340,13 +341,22 @@ void QueuedHIPTracker::ScheduleLocalization(void* data, int pitch, QTRK_PixelDat
 
 void QueuedHIPTracker::Launch(Slot* s)
 {
-	const size_t rb = (size_t)cfg.width * cfg.height * PDT_BytesPerPixel((QTRK_PixelDataType)s->pdt);
+	const int bpp = PDT_BytesPerPixel((QTRK_PixelDataType)s->pdt);
+	const size_t rb = (size_t)cfg.width * cfg.height * bpp;
 	HIP_CHECK(hipEventRecord(s->ev_start, s->stream));
-	HIP_CHECK(hipMemcpyAsync(s->d_rois.p, s->h_rois.p, rb * s->n, hipMemcpyHostToDevice, s->stream));
-	HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
 	LocalizeParams P;
-	FillParams(P, s->pdt, s->n, s->d_rois.p, nullptr, cfg.width * PDT_BytesPerPixel((QTRK_PixelDataType)s->pdt),
-	           s->d_jobs.as<DevJob>(), s->d_results.as<DevResult>());
+	if (s->frame) {
+		HIP_CHECK(hipMemcpyAsync(s->d_rois.p, s->h_rois.p, s->frame_bytes, hipMemcpyHostToDevice, s->stream));
+		HIP_CHECK(hipMemcpyAsync(s->d_off.p, s->h_off.p, sizeof(int64_t) * s->n, hipMemcpyHostToDevice, s->stream));
+		HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
+		FillParams(P, s->pdt, s->n, s->d_rois.p, s->d_off.as<int64_t>(), s->frame_pitch, s->d_jobs.as<DevJob>(),
+		           s->d_results.as<DevResult>());
+	} else {
+		HIP_CHECK(hipMemcpyAsync(s->d_rois.p, s->h_rois.p, rb * s->n, hipMemcpyHostToDevice, s->stream));
+		HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
+		FillParams(P, s->pdt, s->n, s->d_rois.p, nullptr, cfg.width * bpp, s->d_jobs.as<DevJob>(),
+		           s->d_results.as<DevResult>());
+	}
 	HIP_CHECK(hipEventRecord(s->ev_k0, s->stream));
 	HIP_CHECK(launch_localize(P, s->stream));
 	HIP_CHECK(hipEventRecord(s->ev_k1, s->stream));
@@ -354,6 +364,68 @@ void QueuedHIPTracker::Launch(Slot* s)
 	HIP_CHECK(hipEventRecord(s->ev_done, s->stream));
 	s->seq = ++seqCounter;
 	s->state = Slot::Launched;
+}
+
+// QueuedTracker::ScheduleFrame (cputrack/QueuedTracker.cpp:91-111) with the
+// extraction on the GPU: ROIs outside the frame are skipped by the reference's
+// rule and job.zlutIndex = i + jobInfo->zlutIndex as there; then, per
+// batch_size ROIs, the frame rows those ROIs cover are copied once into a batch
+// slot and the kernel reads each ROI at its byte offset in them.  Returns the
+// number of ROIs queued.
+int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int height, ROIPosition* positions,
+                                    int numROI, QTRK_PixelDataType pdt, const LocalizationJob* jobInfo)
+{
+	if ((int)pdt < 0 || (int)pdt > 2) fail("invalid pixel data type");
+	if (numROI <= 0) return 0;
+	if (!imgptr || !positions || !jobInfo) fail("null argument");
+	const int bpp = PDT_BytesPerPixel(pdt);
+	if (pitch < width * bpp) fail("pitch is smaller than width * bytes per pixel");
+	std::vector<int> sel;
+	for (int i = 0; i < numROI; i++) {
+		const ROIPosition& p = positions[i];
+		if (p.x < 0 || p.y < 0 || p.x + cfg.width > width || p.y + cfg.height > height) continue;
+		sel.push_back(i);
+	}
+	if (sel.empty()) return 0;
+	const uint8_t* img = (const uint8_t*)imgptr;
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	if (filling && filling->n > 0) {  // ROIs queued before the frame go first
+		Launch(filling);
+		filling = nullptr;
+	}
+	for (size_t c0 = 0; c0 < sel.size(); c0 += (size_t)batchSize) {
+		const size_t c1 = std::min(sel.size(), c0 + (size_t)batchSize);
+		int y0 = height, y1 = 0;
+		for (size_t k = c0; k < c1; k++) {
+			y0 = std::min(y0, positions[sel[k]].y);
+			y1 = std::max(y1, positions[sel[k]].y + cfg.height);
+		}
+		// rows y0 .. y1-1; the last one only as far as the frame's width
+		const size_t bytes = (size_t)pitch * (y1 - y0 - 1) + (size_t)width * bpp;
+		Slot* s = AcquireSlot(lk);
+		s->frame = true;
+		s->pdt = (int)pdt;
+		s->frame_pitch = pitch;
+		s->frame_bytes = bytes;
+		s->n = (int)(c1 - c0);
+		s->h_rois.ensure(bytes);
+		s->d_rois.ensure(bytes);
+		s->h_off.ensure(sizeof(int64_t) * (size_t)batchSize);
+		s->d_off.ensure(sizeof(int64_t) * (size_t)batchSize);
+		memcpy(s->h_rois.p, img + (size_t)pitch * y0, bytes);
+		int64_t* off = s->h_off.as<int64_t>();
+		DevJob* jobs = s->h_jobs.as<DevJob>();
+		for (size_t k = c0; k < c1; k++) {
+			const int i = sel[k];
+			off[k - c0] = (int64_t)pitch * (positions[i].y - y0) + (int64_t)positions[i].x * bpp;
+			LocalizationJob job = *jobInfo;
+			job.zlutIndex = i + jobInfo->zlutIndex;
+			memcpy(&jobs[k - c0], &job, sizeof(DevJob));
+		}
+		Launch(s);
+	}
+	return (int)sel.size();
 }
 
 void QueuedHIPTracker::Harvest()

@@ -31,6 +31,8 @@ public:
 	void SetPixelCalibrationImages(float* offset, float* gain) override;
 	void SetPixelCalibrationFactors(float offsetFactor, float gainFactor) override;
 	void ScheduleLocalization(void* data, int pitch, QTRK_PixelDataType pdt, const LocalizationJob* jobInfo) override;
+	int ScheduleFrame(void* imgptr, int pitch, int width, int height, ROIPosition* positions, int numROI,
+	                  QTRK_PixelDataType pdt, const LocalizationJob* jobInfo) override;
 	void ClearResults() override;
 	void Flush() override;
 	void SetRadialZLUT(float* data, int count, int planes) override;
@@ -72,6 +74,13 @@ private:
 		uint64_t seq = 0;
 		PinnedBuffer h_rois, h_jobs, h_results;
 		DeviceBuffer d_rois, d_jobs, d_results;
+		// frame mode (ScheduleFrame): h_rois holds rows of a camera frame, `pitch` bytes
+		// apart, and ROI i starts at byte h_off[i] of it
+		bool frame = false;
+		int frame_pitch = 0;
+		size_t frame_bytes = 0;
+		PinnedBuffer h_off;
+		DeviceBuffer d_off;
 		hipStream_t stream = nullptr;
 		hipEvent_t ev_start = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
 	};

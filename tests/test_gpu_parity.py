@@ -259,6 +259,54 @@ def test_schedule_frame_extraction(gpu):
     assert r["z"][0] == ref["z"][0]
 
 
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32], ids=["u8", "u16", "f32"])
+def test_schedule_frame_chunks_pitch_and_formats(gpu, dtype):
+    """The frame is uploaded once per batch_size ROIs and the kernel reads each ROI at its offset in
+    it: every result equals the per-ROI path, with a padded frame pitch, ROIs outside the frame
+    skipped, zlutIndex = i + the job's zlutIndex, and more ROIs than one batch."""
+    n, base = 40, 5
+    w = synth.make_workload("realtime100", n=n)
+    rois = w["rois"]
+    if dtype == np.uint8:
+        rois = np.clip(rois // 64, 0, 255).astype(np.uint8)
+    rois = rois.astype(dtype)
+    H, W = 720, 1100
+    big = np.zeros((H, W + 37), dtype)  # rows padded past the frame's width
+    frame = big[:, :W]
+    pos = []
+    for i in range(n):
+        x, y = 3 + (i % 10) * 109, 7 + (i // 10) * 175
+        frame[y:y + 100, x:x + 100] = rois[i]
+        pos.append((x, y))
+    pos = pos[:20] + [(W - 99, 0), (-2, 40)] + pos[20:] + [(0, H - 99)]  # 3 outside, interleaved
+    keep = [i for i, p in enumerate(pos) if p[0] >= 0 and p[1] >= 0 and p[0] + 100 <= W and p[1] + 100 <= H]
+    nb = base + len(pos)
+    luts = np.concatenate([w["luts"]] * (nb // len(w["luts"]) + 1))[:nb]
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_config_value("batch_size", "16")
+    trk.set_radial_zlut(luts)
+    trk.set_localization_mode(w["mode"])
+    job = np.zeros(1, api.JOB_DTYPE)[0]
+    job["frame"] = 9
+    job["zlutIndex"] = base
+    assert frame.strides[0] == (W + 37) * np.dtype(dtype).itemsize
+    assert trk.schedule_frame(frame, pos, job) == n
+    trk.flush()
+    trk.wait_for(n)
+    r = trk.fetch_results(n)
+    r = r[np.argsort(r["zlutIndex"])]
+    assert list(r["zlutIndex"]) == [base + i for i in keep]
+    jobs = api.make_jobs(n, frame=np.full(n, 9), zlut_index=np.array([base + i for i in keep]))
+    trk2 = api.QTrkInstance(w["settings"])
+    trk2.set_radial_zlut(luts)
+    trk2.set_localization_mode(w["mode"])
+    ref, _, _ = _batch(trk2, rois, jobs, diag=False)
+    trk2.free()
+    trk.free()
+    for k in ("frame", "zlutIndex", "x", "y", "z", "comx", "comy", "error", "imageMean"):
+        assert np.array_equal(r[k], ref[k]), k
+
+
 def test_build_lut_matches_oracle(gpu, oracle):
     w = synth.make_workload("plain64", n=8)
     cfg = w["cfg"]
