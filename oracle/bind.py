@@ -80,6 +80,10 @@ class Oracle:
             "oq_fisher_ref": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl, fl, fl, fl,
                                      fl]),
             "oq_erf": (fl, [fl]),
+            "oq_lut_compare_weighted": (fl, [P, P, C.c_int, C.c_int, fl, fl, fl, P, P]),
+            "oq_xcor1d_img": (None, [P, C.c_int, C.c_int, fl, fl, C.c_int, C.c_int, C.c_int, P]),
+            "oq_quadrant_align_img": (None, [P, C.c_int, C.c_int, fl, P, C.c_int, C.c_int, fl, fl, C.c_int, fl, fl,
+                                             fl, P]),
             "oq_gaussian_spot": (None, [P, C.c_int, C.c_int, fl, fl, fl, fl, fl]),
             "oq_gauss2d_mle": (None, [P, C.c_int, C.c_int, fl, fl, fl, C.c_int, fl, P]),
         }
@@ -164,6 +168,33 @@ class Oracle:
         z = self.L.oq_lut_compare(_p(prof), _p(lut), planes, res, None if w is None else _p(w), _p(sc),
                                   C.byref(best))
         return z, best.value, sc
+
+    def lut_compare_weighted(self, prof, lut, minr, maxr, z_estim):
+        """LUTProfileCompareAdjustedWeights: (z, best plane, scores)."""
+        prof = np.ascontiguousarray(prof, self.fl)
+        lut = np.ascontiguousarray(lut, np.float32)
+        planes, res = lut.shape
+        sc = np.zeros(planes, np.float64)
+        best = C.c_int()
+        z = self.L.oq_lut_compare_weighted(_p(prof), _p(lut), planes, res, minr, maxr, z_estim, _p(sc),
+                                           C.byref(best))
+        return z, best.value, sc
+
+    def xcor1d(self, img, cx, cy, iterations, profile_width, length):
+        """ComputeXCorInterpolated from (cx, cy): (x, y, boundary)."""
+        img = np.ascontiguousarray(img, self.fl)
+        out = np.zeros(3, self.fl)
+        self.L.oq_xcor1d_img(_p(img), img.shape[1], img.shape[0], cx, cy, iterations, profile_width, length, _p(out))
+        return float(out[0]), float(out[1]), bool(out[2])
+
+    def quadrant_align(self, img, mean, lut, minr, maxr, angsteps, x, y, z):
+        """QuadrantAlign of (x, y) against the LUT row at z: (x, y, boundary)."""
+        img = np.ascontiguousarray(img, self.fl)
+        lut = np.ascontiguousarray(lut, np.float32)
+        out = np.zeros(3, self.fl)
+        self.L.oq_quadrant_align_img(_p(img), img.shape[1], img.shape[0], mean, _p(lut), lut.shape[0], lut.shape[1],
+                                     minr, maxr, angsteps, x, y, z, _p(out))
+        return float(out[0]), float(out[1]), bool(out[2])
 
     def bspline_weights(self, t):
         w = np.zeros(4, self.fl)

@@ -9,6 +9,7 @@
 #include "qtrk_oracle.h"
 
 #include <math.h>
+#include <limits.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -209,6 +210,58 @@ static void bfly4(const OFFT* p, ocpx* F, size_t fs, size_t m)
 	}
 }
 
+/* Radix-3 butterfly, cputrack/kissfft.h:154-187: HALF_OF is a product with the double 0.5, the
+ * same float as a product with 0.5f. */
+static void bfly3(const OFFT* p, ocpx* F, size_t fs, size_t m)
+{
+	ocpx epi3 = p->tw[fs * m];
+	for (size_t k = 0; k < m; k++) {
+		ocpx s1 = cmul(F[k + m], p->tw[k * fs]);
+		ocpx s2 = cmul(F[k + 2 * m], p->tw[k * fs * 2]);
+		ocpx s3 = cadd(s1, s2);
+		ocpx s0 = csub(s1, s2);
+		ocpx fm = { F[k].re - s3.re * FC(0.5), F[k].im - s3.im * FC(0.5) };
+		s0.re = s0.re * epi3.im;
+		s0.im = s0.im * epi3.im;
+		F[k] = cadd(F[k], s3);
+		ocpx f2 = { fm.re + s0.im, fm.im - s0.re };
+		ocpx rot = { -s0.im, s0.re };
+		F[k + 2 * m] = f2;
+		F[k + m] = cadd(fm, rot);
+	}
+}
+
+/* Radix-5 butterfly, cputrack/kissfft.h:189-253. */
+static void bfly5(const OFFT* p, ocpx* F, size_t fs, size_t m)
+{
+	ocpx ya = p->tw[fs * m], yb = p->tw[fs * 2 * m];
+	for (size_t u = 0; u < m; u++) {
+		ocpx* f0 = F + u;
+		ocpx* f1 = f0 + m;
+		ocpx* f2 = f0 + 2 * m;
+		ocpx* f3 = f0 + 3 * m;
+		ocpx* f4 = f0 + 4 * m;
+		ocpx s0 = *f0;
+		ocpx s1 = cmul(*f1, p->tw[u * fs]);
+		ocpx s2 = cmul(*f2, p->tw[2 * u * fs]);
+		ocpx s3 = cmul(*f3, p->tw[3 * u * fs]);
+		ocpx s4 = cmul(*f4, p->tw[4 * u * fs]);
+		ocpx s7 = cadd(s1, s4), s10 = csub(s1, s4), s8 = cadd(s2, s3), s9 = csub(s2, s3);
+		*f0 = cadd(*f0, s7);
+		*f0 = cadd(*f0, s8);
+		ocpx t5 = { s7.re * ya.re + s8.re * yb.re, s7.im * ya.re + s8.im * yb.re };
+		ocpx s5 = cadd(s0, t5);
+		ocpx s6 = { s10.im * ya.im + s9.im * yb.im, -(s10.re * ya.im) - s9.re * yb.im };
+		*f1 = csub(s5, s6);
+		*f4 = cadd(s5, s6);
+		ocpx t11 = { s7.re * yb.re + s8.re * ya.re, s7.im * yb.re + s8.im * ya.re };
+		ocpx s11 = cadd(s0, t11);
+		ocpx s12 = { -(s10.im * yb.im) + s9.im * ya.im, s10.re * yb.im - s9.re * ya.im };
+		*f2 = cadd(s11, s12);
+		*f3 = csub(s11, s12);
+	}
+}
+
 /* Generic radix-p DFT butterfly, cputrack/kissfft.h:256-290. */
 static void bfly_generic(const OFFT* p, ocpx* F, size_t fs, int m, int r)
 {
@@ -243,7 +296,9 @@ static void kf_work(const OFFT* p, int stage, ocpx* Fout, const ocpx* f, size_t 
 	}
 	Fout = beg;
 	if (r == 2) bfly2(p, Fout, fstride, m);
+	else if (r == 3) bfly3(p, Fout, fstride, m);
 	else if (r == 4) bfly4(p, Fout, fstride, m);
+	else if (r == 5) bfly5(p, Fout, fstride, m);
 	else bfly_generic(p, Fout, fstride, m, r);
 }
 
@@ -443,23 +498,28 @@ static void ensure_fft(OFFT* p, int n, int inverse)
 }
 
 /* CPUTracker::ComputeQI, cputrack/cpu_tracker.cpp:285-360 */
+/* the quadrant direction table of ComputeQI, cputrack/cpu_tracker.cpp:294-300 */
+static void ensure_qdirs(OWorker* wk, int A)
+{
+	if (A == wk->ndirs) return;
+	free(wk->qdirs);
+	wk->qdirs = (fl*)malloc(sizeof(fl) * 2 * (size_t)(A > 0 ? A : 1));
+	wk->ndirs = A;
+	for (int j = 0; j < A; j++) {
+#ifdef ORACLE_FP64
+		fl ang = 0.5 * 3.141593 * (j + 0.5) / (double)A;
+#else
+		float ang = (float)(0.5 * (double)3.141593f * (double)(j + 0.5f) / (double)(float)A);
+#endif
+		wk->qdirs[2 * j] = FCOS(ang);
+		wk->qdirs[2 * j + 1] = FSIN(ang);
+	}
+}
+
 void oq_qi(OWorker* wk, fl cx, fl cy, int iterations, int nr, int A, float factor, fl minr, fl maxr,
            const fl* weights, fl* outx, fl* outy, int* boundary, OTrace* tr)
 {
-	if (A != wk->ndirs) {
-		free(wk->qdirs);
-		wk->qdirs = (fl*)malloc(sizeof(fl) * 2 * (size_t)(A > 0 ? A : 1));
-		wk->ndirs = A;
-		for (int j = 0; j < A; j++) {
-#ifdef ORACLE_FP64
-			fl ang = 0.5 * 3.141593 * (j + 0.5) / (double)A;
-#else
-			float ang = (float)(0.5 * (double)3.141593f * (double)(j + 0.5f) / (double)(float)A);
-#endif
-			wk->qdirs[2 * j] = FCOS(ang);
-			wk->qdirs[2 * j + 1] = FSIN(ang);
-		}
-	}
+	ensure_qdirs(wk, A);
 	ensure_fft(&wk->fwd, nr * 2, 0);
 	ensure_fft(&wk->inv, nr * 2, 1);
 	fl* buf = (fl*)alloca(sizeof(fl) * nr * 4);
@@ -597,6 +657,186 @@ void oq_set_image(OWorker* wk, const void* data, int pitch, int pdt)
 
 /* QueuedCPUTracker::ProcessJob, cputrack/QueuedCPUTracker.cpp:273-361 (COM,
  * QI and Z-LUT modes; bias correction is applied by the caller). */
+/* Truncation toward zero saturated to the int range, NaN giving 0: the reference's (int) casts where
+ * their result is defined, and a defined value where it is not. */
+static int trunc_sat_c(fl v)
+{
+	if (!(v == v)) return 0;
+	if (v >= FC(2147483648.0)) return INT_MAX;
+	if (v <= FC(-2147483648.0)) return INT_MIN;
+	return (int)v;
+}
+
+/* CPUTracker::LUTProfileCompareAdjustedWeights, cputrack/cpu_tracker.cpp:871-901: radius weights
+ * |L[zi+1] - L[zi]| * r_i around the plane of z_estim, double difference and sum.  A z_estim below 0
+ * (or NaN) uses plane 0 here; the reference would index before the LUT. */
+fl oq_lut_compare_weighted(const fl* prof, const float* lut, int planes, int res, fl minr, fl maxr, fl z_estim,
+                           double* scores, int* best)
+{
+	int zi = trunc_sat_c(z_estim);
+	if (zi > planes - 2) zi = planes - 2;
+	if (zi < 0) zi = 0;
+	const float* z0 = lut + (size_t)zi * res;
+	const float* z1 = lut + (size_t)(zi + 1 < planes ? zi + 1 : zi) * res;
+	double* w = (double*)alloca(sizeof(double) * res);
+	for (int i = 0; i < res; i++) {
+		fl r = minr + (maxr - minr) * i / (float)res;
+		w[i] = fabs((double)z1[i] - (double)z0[i]) * (double)r;
+	}
+	for (int k = 0; k < planes; k++) {
+		double sum = 0;
+		for (int r = 0; r < res; r++) {
+			double d = (double)prof[r] - (double)lut[(size_t)k * res + r];
+			d = -d * d;
+			d *= w[r];
+			sum += d;
+		}
+		scores[k] = sum;
+	}
+	return (fl)oq_max_interp_zlut(scores, planes, best);
+}
+
+/* CPUTracker::QuadrantAlign_ComputeOffset, cputrack/cpu_tracker.cpp:409-438 */
+static fl qa_offset(OWorker* wk, ocpx* profile, const ocpx* zfft, int nr)
+{
+	int N = nr * 2;
+	ocpx* fo = (ocpx*)alloca(sizeof(ocpx) * N);
+	fl* ac = (fl*)alloca(sizeof(fl) * N);
+	oq_fft_transform(&wk->qafwd, profile, fo);
+	for (int x = 0; x < N; x++) {
+		ocpx cj = { zfft[x].re, -zfft[x].im };
+		fo[x] = cmul(fo[x], cj);
+	}
+	oq_fft_transform(&wk->qainv, fo, profile);
+	for (int x = 0; x < N; x++) ac[x] = profile[(x + nr) % N].re;
+	int im;
+	fl maxPos = oq_max_interp_qi(ac, N, &im);
+	return (maxPos - nr) * (FC(3.14159265359) / 4);
+}
+
+/* CPUTracker::QuadrantAlign, cputrack/cpu_tracker.cpp:449-512: the quadrant profiles at (x, y) on the
+ * Z-LUT's radii, each normalised, correlated against the bead's LUT row interpolated at z; the boundary
+ * flag ends cleared, as there. */
+void oq_quadrant_align(OWorker* wk, const float* lut, int planes, int res, fl minr, fl maxr, int angsteps,
+                       fl* px, fl* py, fl pz, int* boundary)
+{
+	int N = 2 * res;
+	ensure_qdirs(wk, angsteps);
+	ensure_fft(&wk->qafwd, N, 0);
+	ensure_fft(&wk->qainv, N, 1);
+	ocpx* concat = (ocpx*)alloca(sizeof(ocpx) * N);
+	ocpx* zfft = (ocpx*)alloca(sizeof(ocpx) * N);
+	memset(concat, 0, sizeof(ocpx) * N);
+	int iz = trunc_sat_c(pz);
+	int izc = iz < -1 ? -1 : (iz > planes ? planes : iz);
+	int zp0 = izc < 0 ? 0 : (izc > planes - 1 ? planes - 1 : izc);
+	int zp1 = izc + 1 < 0 ? 0 : (izc + 1 > planes - 1 ? planes - 1 : izc + 1);
+	const float* l0 = lut + (size_t)zp0 * res;
+	const float* l1 = lut + (size_t)zp1 * res;
+	float frac = (float)(pz - iz);
+	for (int r = 0; r < res; r++) {
+		float v = l0[r] + (l1[r] - l0[r]) * frac;
+		concat[res - r - 1].re = concat[res + r].re = (fl)(double)v;
+	}
+	oq_fft_transform(&wk->qafwd, concat, zfft);
+	fl* buf = (fl*)alloca(sizeof(fl) * res * 4);
+	fl *q0 = buf, *q1 = buf + res, *q2 = buf + 2 * res, *q3 = buf + 3 * res;
+	for (int q = 0; q < 4; q++) {
+		oq_quadrant_profile(wk, buf + q * res, res, angsteps, q, minr, maxr, *px, *py, 0);
+		oq_normalize_profile(buf + q * res, res);
+	}
+	fl ppl = (maxr - minr) / res;
+	for (int r = 0; r < res; r++) {
+		concat[res - r - 1].re = q1[r] + q2[r]; concat[res - r - 1].im = 0;
+		concat[res + r].re = q0[r] + q3[r]; concat[res + r].im = 0;
+	}
+	fl offX = qa_offset(wk, concat, zfft, res);
+	for (int r = 0; r < res; r++) {
+		concat[res + r].re = q0[r] + q1[r]; concat[res + r].im = 0;
+		concat[res - r - 1].re = q2[r] + q3[r]; concat[res - r - 1].im = 0;
+	}
+	fl offY = qa_offset(wk, concat, zfft, res);
+	*px = *px + offX * ppl;
+	*py = *py + offY * ppl;
+	*boundary = 0;
+}
+
+/* XCor1DBuffer::XCorFFTHelper, cputrack/cpu_tracker.cpp:140-158 */
+static void xcor_helper(OWorker* wk, const ocpx* prof, const ocpx* rev, fl* result, int n)
+{
+	ocpx* fo = (ocpx*)alloca(sizeof(ocpx) * n);
+	ocpx* fr = (ocpx*)alloca(sizeof(ocpx) * n);
+	oq_fft_transform(&wk->xfwd, prof, fo);
+	oq_fft_transform(&wk->xfwd, rev, fr);
+	for (int x = 0; x < n; x++) {
+		ocpx cj = { fr[x].re, -fr[x].im };
+		fo[x] = cmul(fo[x], cj);
+	}
+	oq_fft_transform(&wk->xinv, fo, fr);
+	for (int x = 0; x < n; x++) result[x] = fr[(x + n / 2) % n].re;
+}
+
+/* CPUTracker::ComputeXCorInterpolated, cputrack/cpu_tracker.cpp:190-270 (XCorScale = 1): per iteration
+ * an x line profile averaged over profileWidth rows and a y line profile averaged over profileWidth
+ * columns, each autocorrelated against its reverse, peak-fitted, and half the offset applied. */
+void oq_xcor1d(OWorker* wk, fl cx, fl cy, int iterations, int profileWidth, int xcorw, fl* outx, fl* outy,
+               int* boundary)
+{
+	fl px = cx, py = cy;
+	if (xcorw < profileWidth) profileWidth = xcorw;
+	*outx = cx;
+	*outy = cy;
+	if (xcorw > wk->w || xcorw > wk->h) { *boundary = 1; return; }
+	ensure_fft(&wk->xfwd, xcorw, 0);
+	ensure_fft(&wk->xinv, xcorw, 1);
+	ocpx* prof = (ocpx*)alloca(sizeof(ocpx) * xcorw);
+	ocpx* rev = (ocpx*)alloca(sizeof(ocpx) * xcorw);
+	fl* ac = (fl*)alloca(sizeof(fl) * xcorw);
+	const fl half = FC(1.0) * xcorw / 2;
+	*boundary = 0;
+	for (int k = 0; k < iterations; k++) {
+		*boundary = check_bounds(wk->w, wk->h, px, py, half);
+		fl xmin = px - half, ymin = py - half;
+		for (int x = 0; x < xcorw; x++) {
+			fl s = 0;
+			int n = 0;
+			for (int y = 0; y < profileWidth; y++) {
+				fl xp = x * FC(1.0) + xmin;
+				fl yp = py + FC(1.0) * (y - profileWidth / 2);
+				int out;
+				s += oq_interpolate(wk->img, wk->w, wk->h, xp, yp, &out);
+				n += out ? 0 : 1;
+			}
+			if (n > 0) s /= n; else s = 0;
+			prof[x].re = s; prof[x].im = 0;
+			rev[xcorw - x - 1].re = s; rev[xcorw - x - 1].im = 0;
+		}
+		xcor_helper(wk, prof, rev, ac, xcorw);
+		int im;
+		fl offX = oq_max_interp_qi(ac, xcorw, &im) - (fl)xcorw / 2;
+		for (int y = 0; y < xcorw; y++) {
+			fl s = 0;
+			int n = 0;
+			for (int x = 0; x < profileWidth; x++) {
+				fl xp = px + FC(1.0) * (x - profileWidth / 2);
+				fl yp = y * FC(1.0) + ymin;
+				int out;
+				s += oq_interpolate(wk->img, wk->w, wk->h, xp, yp, &out);
+				n += out ? 0 : 1;
+			}
+			if (n > 0) s /= n; else s = 0;
+			prof[y].re = s; prof[y].im = 0;
+			rev[xcorw - y - 1].re = s; rev[xcorw - y - 1].im = 0;
+		}
+		xcor_helper(wk, prof, rev, ac, xcorw);
+		fl offY = oq_max_interp_qi(ac, xcorw, &im) - (fl)xcorw / 2;
+		px += (offX - 1) * FC(1.0) * FC(0.5);
+		py += (offY - 1) * FC(1.0) * FC(0.5);
+	}
+	*outx = px;
+	*outy = py;
+}
+
 void oq_process_job(const OTracker* t, OWorker* wk, const void* roi, int pitch, int pdt,
                     const OJob* job, OResult* out, OTrace* tr)
 {
@@ -624,7 +864,11 @@ void oq_process_job(const OTracker* t, OWorker* wk, const void* roi, int pitch, 
 	if (tr) { tr->comx = comx; tr->comy = comy; tr->mean = wk->mean; tr->stdev = wk->stdev; }
 	int boundary = 0;
 	fl px = 0, py = 0, pz = 0;
-	if (mode & M_QI) {
+	if (mode & M_XCOR1D) {
+		out->comx = (float)comx; out->comy = (float)comy;
+		oq_xcor1d(wk, comx, comy, c->xc1_iterations, c->xc1_profileWidth, c->xc1_profileLength, &px, &py,
+		          &boundary);
+	} else if (mode & M_QI) {
 		out->comx = (float)comx; out->comy = (float)comy;
 		oq_qi(wk, comx, comy, c->qi_iterations, c->qi_radialsteps, c->qi_angstepspq, c->qi_angstep_factor,
 		      (fl)c->qi_minradius, (fl)c->qi_maxradius, t->qi_window, &px, &py, &boundary, tr);
@@ -638,17 +882,32 @@ void oq_process_job(const OTracker* t, OWorker* wk, const void* roi, int pitch, 
 		px = comx; py = comy;
 	}
 	if (mode & M_LOCALIZEZ) {
+		/* QueuedCPUTracker::ProcessJob, cputrack/QueuedCPUTracker.cpp:319-348: 5 passes with
+		 * LT_ZLUTAlign, each profiling at the current position, quadrant-aligning from the second
+		 * pass on, and scoring the pass's profile; then the weighted compare */
 		int res = c->zlut_radialsteps;
 		fl* prof = tr && tr->profile ? tr->profile : (fl*)alloca(sizeof(fl) * res);
-		oq_radial_profile(wk->img, wk->w, wk->h, prof, res, c->zlut_angularsteps, (fl)c->zlut_minradius,
-		                  (fl)c->zlut_maxradius, px, py, wk->mean, (mode & M_NORMPROF) != 0);
-		boundary = check_bounds(wk->w, wk->h, px, py, (fl)c->zlut_maxradius);
-		if (t->zluts && job->zlutIndex >= 0 && job->zlutIndex < t->zlut_count) {
-			double* sc = tr && tr->scores ? tr->scores : (double*)alloca(sizeof(double) * t->zlut_planes);
+		const int has_lut = t->zluts && job->zlutIndex >= 0 && job->zlutIndex < t->zlut_count;
+		const float* lut = has_lut ? t->zluts + (size_t)job->zlutIndex * t->zlut_planes * res : 0;
+		double* sc = tr && tr->scores ? tr->scores : (double*)alloca(sizeof(double) * (t->zlut_planes > 0 ? t->zlut_planes : 1));
+		const int passes = (mode & M_ZLUTALIGN) ? 5 : 1;
+		for (int pass = 0; pass < passes; pass++) {
+			oq_radial_profile(wk->img, wk->w, wk->h, prof, res, c->zlut_angularsteps, (fl)c->zlut_minradius,
+			                  (fl)c->zlut_maxradius, px, py, wk->mean, (mode & M_NORMPROF) != 0);
+			boundary = check_bounds(wk->w, wk->h, px, py, (fl)c->zlut_maxradius);
+			if (pass > 0 && has_lut)
+				oq_quadrant_align(wk, lut, t->zlut_planes, res, (fl)c->zlut_minradius, (fl)c->zlut_maxradius,
+				                  c->qi_angstepspq, &px, &py, pz, &boundary);
+			if (has_lut) {
+				int best = 0;
+				pz = oq_lut_compare(prof, lut, t->zlut_planes, res, t->zcmp, sc, &best);
+				if (tr) tr->best_plane = best;
+			}
+		}
+		if ((mode & M_ZWEIGHTED) && has_lut) {
 			int best = 0;
-			const float* lut = t->zluts + (size_t)job->zlutIndex * t->zlut_planes * res;
-			pz = oq_lut_compare(prof, lut, t->zlut_planes, res, t->zcmp, sc, &best);
-			if (tr) tr->best_plane = best;
+			pz = oq_lut_compare_weighted(prof, lut, t->zlut_planes, res, (fl)c->zlut_minradius,
+			                             (fl)c->zlut_maxradius, pz, sc, &best);
 		}
 	}
 	out->x = (float)px; out->y = (float)py; out->z = (float)pz;
@@ -710,6 +969,34 @@ void oq_localize_trace(const OConfig* cfg, int mode, const float* zluts, int zco
 	free(prof);
 	oq_worker_free(&wk);
 	free(t.qi_window);
+}
+
+/* Image-level entry points of the optional modes for ctypes callers: `img` is a w*h float image,
+ * `mean` the value undersampled quadrant bins fall back to.  out3 = (x, y, boundary). */
+void oq_xcor1d_img(const fl* img, int w, int h, fl cx, fl cy, int iterations, int profileWidth, int xcorw,
+                   fl* out3)
+{
+	OWorker wk;
+	memset(&wk, 0, sizeof(wk));
+	wk.w = w; wk.h = h; wk.img = (fl*)img;
+	int b = 0;
+	oq_xcor1d(&wk, cx, cy, iterations, profileWidth, xcorw, &out3[0], &out3[1], &b);
+	out3[2] = (fl)b;
+	wk.img = 0;
+	oq_worker_free(&wk);
+}
+
+void oq_quadrant_align_img(const fl* img, int w, int h, fl mean, const float* lut, int planes, int res, fl minr,
+                           fl maxr, int angsteps, fl x, fl y, fl z, fl* out3)
+{
+	OWorker wk;
+	memset(&wk, 0, sizeof(wk));
+	wk.w = w; wk.h = h; wk.img = (fl*)img; wk.mean = mean;
+	int b = 1;
+	oq_quadrant_align(&wk, lut, planes, res, minr, maxr, angsteps, &x, &y, z, &b);
+	out3[0] = x; out3[1] = y; out3[2] = (fl)b;
+	wk.img = 0;
+	oq_worker_free(&wk);
 }
 
 /* QueuedCPUTracker::BuildLUT, cputrack/QueuedCPUTracker.cpp:580-646: image i of

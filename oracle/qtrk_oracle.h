@@ -16,10 +16,8 @@
  *
  * Each function cites the reference file:line it restates.  How each one is
  * pinned (which test checks it against what) is listed in the test files
- * tests/test_oracle_*.py.  Parity unpinned: the radix-3/5 FFT butterflies
- * (the generic DFT butterfly is used instead; only XCor1D with a
- * non-power-of-two profile length reaches them) and libm ulps of the
- * reference's Windows build (cosf/sinf/expf there vs glibc here).
+ * tests/test_oracle_*.py.  Parity unpinned: libm ulps of the reference's
+ * Windows build (cosf/sinf/expf there vs glibc here).
  */
 #ifndef QTRK_ORACLE_H
 #define QTRK_ORACLE_H
@@ -156,6 +154,17 @@ void oq_generate_image_from_lut(fl* img, int w, int h, const float* zlut, int pl
                                 fl minr, fl maxr, fl px, fl py, fl pz, int spline, int ovs);
 fl oq_erf(fl x);
 void oq_gaussian_spot(fl* img, int w, int h, fl px, fl py, fl sigma, fl I0, fl bg);
+/* LUTProfileCompareAdjustedWeights, QuadrantAlign, ComputeXCorInterpolated (cputrack/cpu_tracker.cpp) */
+fl oq_lut_compare_weighted(const fl* prof, const float* lut, int planes, int res, fl minr, fl maxr, fl z_estim,
+                           double* scores, int* best);
+void oq_quadrant_align(OWorker* wk, const float* lut, int planes, int res, fl minr, fl maxr, int angsteps,
+                       fl* px, fl* py, fl pz, int* boundary);
+void oq_xcor1d(OWorker* wk, fl cx, fl cy, int iterations, int profileWidth, int xcorw, fl* outx, fl* outy,
+               int* boundary);
+void oq_xcor1d_img(const fl* img, int w, int h, fl cx, fl cy, int iterations, int profileWidth, int xcorw,
+                   fl* out3);
+void oq_quadrant_align_img(const fl* img, int w, int h, fl mean, const float* lut, int planes, int res, fl minr,
+                           fl maxr, int angsteps, fl x, fl y, fl z, fl* out3);
 /* SampleFisherMatrix::Compute (cputrack/FisherMatrix.h:36-104): 3x3 (x, y, z)
  * Fisher matrix by central differences of GenerateImageFromLUT (spline, ovs 4),
  * written row-major into F[9]. */
