@@ -71,6 +71,50 @@ static void fft_leaves(const int* radix, const int* rem, int stage, int out, int
 	}
 }
 
+void HostFFT::build(int len)
+{
+	n = len;
+	int rem[kMaxFFTStages];
+	nst = fft_factor(n, radix, rem);
+	int f = 1;
+	for (int s = 0; s < nst; s++) {
+		m[s] = rem[s];
+		fs[s] = f;
+		f *= radix[s];
+	}
+	std::vector<int> leaves(n);
+	fft_leaves(radix, rem, 0, 0, 0, 1, leaves);
+	upload(perm, leaves.data(), leaves.size() * 4);
+	// twiddles exp(+-2 pi i k / n) with the float phase step of kissfft's traits
+	// (cputrack/kissfft.h:11-16); exp(i a) is (cosf a, sinf a)
+	std::vector<float> tf(2 * (size_t)n), ti(2 * (size_t)n);
+	const float phf = -2 * acosf(-1.0f) / n, phi = 2 * acosf(-1.0f) / n;
+	for (int i = 0; i < n; i++) {
+		tf[2 * i] = cosf(i * phf);
+		tf[2 * i + 1] = sinf(i * phf);
+		ti[2 * i] = cosf(i * phi);
+		ti[2 * i + 1] = sinf(i * phi);
+	}
+	upload(tw_fwd, tf.data(), tf.size() * 4);
+	upload(tw_inv, ti.data(), ti.size() * 4);
+}
+
+FFTPlan HostFFT::plan() const
+{
+	FFTPlan p{};
+	p.n = n;
+	p.nst = nst;
+	for (int s = 0; s < kMaxFFTStages; s++) {
+		p.radix[s] = radix[s];
+		p.m[s] = m[s];
+		p.fs[s] = fs[s];
+	}
+	p.perm = perm.as<int>();
+	p.tw_fwd = tw_fwd.as<float2>();
+	p.tw_inv = tw_inv.as<float2>();
+	return p;
+}
+
 QueuedHIPTracker::QueuedHIPTracker(const QTrkComputedConfig& cc)
 {
 	cfg = cc;
@@ -147,31 +191,19 @@ void QueuedHIPTracker::BuildTables()
 	}
 	upload(d_qi_window, wnd.data(), wnd.size() * 4);
 
-	// FFT of length 2*nr: stages, leaf order and twiddles (cputrack/kissfft.h:11-16)
-	fft_n = 2 * nr;
-	if (fft_n > 4 * kBlock / 2) fail("qi_radialsteps > 256 is not supported");
-	int rem[kMaxFFTStages];
-	fft_nst = fft_factor(fft_n, fft_radix, rem);
-	int fs = 1;
-	for (int s = 0; s < fft_nst; s++) {
-		if (fft_radix[s] != 2 && fft_radix[s] != 4) fail("QI FFT length must be a power of two");
-		fft_m[s] = rem[s];
-		fft_fs[s] = fs;
-		fs *= fft_radix[s];
-	}
-	std::vector<int> perm(fft_n);
-	fft_leaves(fft_radix, rem, 0, 0, 0, 1, perm);
-	upload(d_perm, perm.data(), perm.size() * 4);
-	std::vector<float> tf(2 * (size_t)fft_n), ti(2 * (size_t)fft_n);
-	const float phf = -2 * acosf(-1.0f) / fft_n, phi = 2 * acosf(-1.0f) / fft_n;
-	for (int i = 0; i < fft_n; i++) {
-		tf[2 * i] = cosf(i * phf);
-		tf[2 * i + 1] = sinf(i * phf);
-		ti[2 * i] = cosf(i * phi);
-		ti[2 * i + 1] = sinf(i * phi);
-	}
-	upload(d_tw_fwd, tf.data(), tf.size() * 4);
-	upload(d_tw_inv, ti.data(), ti.size() * 4);
+	// FFT of length 2*nr (cputrack/kissfft.h:11-47)
+	if (2 * nr > kMaxFFTLen) fail("qi_radialsteps > 256 is not supported");
+	qi_fft.build(2 * nr);
+	for (int s = 0; s < qi_fft.nst; s++)
+		if (qi_fft.radix[s] != 2 && qi_fft.radix[s] != 4) fail("QI FFT length must be a power of two");
+	// LT_XCor1D's transform of the profile length and LT_ZLUTAlign's of twice the Z-LUT
+	// radial steps, built when the sizes fit; SetLocalizationMode rejects the mode otherwise
+	if (cfg.xc1_profileLength >= 2 && cfg.xc1_profileLength <= kMaxFFTLen) xc_fft.build(cfg.xc1_profileLength);
+	if (cfg.zlut_radialsteps >= 1 && 2 * cfg.zlut_radialsteps <= kMaxFFTLen) qa_fft.build(2 * cfg.zlut_radialsteps);
+	// QuadrantAlign's quadrant profiles take all qi_angstepspq directions
+	// (cputrack/cpu_tracker.cpp:476, 642-657): at least 4 samples, stride trunc(A / samples)
+	qa_steps = A < MIN_RADPROFILE_SMP_COUNT ? MIN_RADPROFILE_SMP_COUNT : A;
+	qa_stride = (int)((float)A / qa_steps);
 
 	// Radial-profile directions, cputrack/utils.cpp:278-282
 	const int ZA = cfg.zlut_angularsteps;
@@ -256,16 +288,14 @@ void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void*
 	P.qi_iter_stride = d_qi_stride.as<int>();
 	P.qi_dirs = d_qi_dirs.as<float2>();
 	P.qi_window = d_qi_window.as<float>();
-	P.fft_n = fft_n;
-	P.fft_nst = fft_nst;
-	for (int s = 0; s < kMaxFFTStages; s++) {
-		P.fft_radix[s] = fft_radix[s];
-		P.fft_m[s] = fft_m[s];
-		P.fft_fs[s] = fft_fs[s];
-	}
-	P.fft_perm = d_perm.as<int>();
-	P.tw_fwd = d_tw_fwd.as<float2>();
-	P.tw_inv = d_tw_inv.as<float2>();
+	P.qi_fft = qi_fft.plan();
+	P.xc_len = cfg.xc1_profileLength;
+	P.xc_width = cfg.xc1_profileWidth;
+	P.xc_iters = cfg.xc1_iterations;
+	P.xc_fft = xc_fft.plan();
+	P.qa_steps = qa_steps;
+	P.qa_stride = qa_stride;
+	P.qa_fft = qa_fft.plan();
 	P.z_nb = cfg.zlut_radialsteps;
 	P.z_A = cfg.zlut_angularsteps;
 	P.z_minr = cfg.zlut_minradius;
@@ -543,11 +573,28 @@ void QueuedHIPTracker::ClearResults()
 // Flush, wait for idle, then switch (cudatrack/QueuedCUDATracker.cu:420-428).
 void QueuedHIPTracker::SetLocalizationMode(LocMode_t mode)
 {
-	const int supported = LT_QI | LT_Gaussian2D | LT_LocalizeZ | LT_NormalizeProfile | LT_ClearFirstFourPixels;
+	const int supported = LT_QI | LT_Gaussian2D | LT_LocalizeZ | LT_NormalizeProfile | LT_ClearFirstFourPixels |
+	                      LT_XCor1D | LT_ZLUTAlign | LT_LocalizeZWeighted;
 	if (mode & ~supported) {
 		std::ostringstream os;
 		os << "localization mode bits 0x" << std::hex << (mode & ~supported) << " are not implemented";
 		fail(os.str());
+	}
+	if ((mode & LT_XCor1D) && xc_fft.n == 0)
+		fail("LT_XCor1D needs xc1_profileLength in 2.." + std::to_string(kMaxFFTLen));
+	if ((mode & LT_XCor1D) && (cfg.xc1_iterations < 0 || cfg.xc1_profileWidth < 0))
+		fail("LT_XCor1D needs xc1_iterations >= 0 and xc1_profileWidth >= 0");
+	if ((mode & LT_ZLUTAlign) && (mode & LT_LocalizeZ) && qa_fft.n == 0)
+		fail("LT_ZLUTAlign needs zlut_radialsteps in 1.." + std::to_string(kMaxFFTLen / 2));
+	{
+		LocalizeParams P{};
+		FillParams(P, 2, 1, nullptr, nullptr, 0, nullptr, nullptr);
+		P.mode = mode;
+		P.zplanes = zplanes;
+		const size_t need = localize_lds_bytes(P);
+		if (need > lds_limit)
+			fail("mode needs " + std::to_string(need) + " bytes of LDS per ROI, more than the device's " +
+			     std::to_string(lds_limit));
 	}
 	Flush();
 	std::unique_lock<std::mutex> lk(mtx);

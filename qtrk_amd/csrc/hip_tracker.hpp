@@ -22,6 +22,15 @@
 
 namespace qtrk {
 
+// A kissfft plan of one length with its leaf order and twiddles in GPU memory.
+struct HostFFT {
+	int n = 0, nst = 0;
+	int radix[kMaxFFTStages] = {}, m[kMaxFFTStages] = {}, fs[kMaxFFTStages] = {};
+	DeviceBuffer perm, tw_fwd, tw_inv;
+	void build(int len);
+	FFTPlan plan() const;
+};
+
 class QueuedHIPTracker : public QueuedTracker {
 public:
 	explicit QueuedHIPTracker(const QTrkComputedConfig& cc);
@@ -99,9 +108,11 @@ private:
 	hipStream_t main_stream = nullptr;
 
 	// per-config device tables
-	DeviceBuffer d_qi_steps, d_qi_stride, d_qi_dirs, d_qi_window, d_perm, d_tw_fwd, d_tw_inv, d_z_dirs;
-	int fft_n = 0, fft_nst = 0;
-	int fft_radix[kMaxFFTStages] = {}, fft_m[kMaxFFTStages] = {}, fft_fs[kMaxFFTStages] = {};
+	DeviceBuffer d_qi_steps, d_qi_stride, d_qi_dirs, d_qi_window, d_z_dirs;
+	HostFFT qi_fft;   // length 2*qi_radialsteps
+	HostFFT xc_fft;   // length xc1_profileLength (LT_XCor1D), when 2..kMaxFFTLen
+	HostFFT qa_fft;   // length 2*zlut_radialsteps (LT_ZLUTAlign), when <= kMaxFFTLen
+	int qa_steps = 0, qa_stride = 0;
 
 	// radial Z-LUT [count][planes][zlut_radialsteps], weights, compare profiles
 	std::vector<float> h_zlut;

@@ -28,15 +28,33 @@
 namespace qtrk {
 
 // LDS map.  The ROI, the radial profile and the reduction scratch live for the
-// whole kernel; the QI tables/buffers, the radial-profile partial sums and the
-// LUT scores are used in disjoint phases and share one region.
+// whole kernel; the QI (or XCor1D) tables/buffers, the radial-profile partial
+// sums, the QuadrantAlign buffers and the LUT scores are used in disjoint phases
+// and share one region.
 struct LdsLayout {
 	int img, zprof, red, st;
-	int qdir, qprof, fft, tw, perm;  // QI phase
-	int zdir, zpart, zcnt;           // radial-profile phase
-	int scores;                      // LUT phase
+	int qdir, qprof, fft, tw, perm;               // QI phase
+	int xprof, xfft, xtw, xperm, xtmp;            // XCor1D phase
+	int zdir, zpart, zcnt;                        // radial-profile phase
+	int adir, aprof, alut, afft, atw, aperm, atmp;  // QuadrantAlign phase
+	int scores, zw;                               // LUT phase (zw: adjusted weights)
 	int total;
 };
+
+// Optional modes the parameters ask for (never in the LUT-build variant).
+__host__ __device__ inline bool mode_xcor(const LocalizeParams& P) { return !P.build && (P.mode & kModeXCor1D); }
+__host__ __device__ inline bool mode_align(const LocalizeParams& P)
+{
+	return !P.build && (P.mode & kModeZLUTAlign) && (P.mode & kModeLocalizeZ);
+}
+__host__ __device__ inline bool mode_zweighted(const LocalizeParams& P)
+{
+	return !P.build && (P.mode & kModeLocalizeZWeighted) && (P.mode & kModeLocalizeZ);
+}
+__host__ __device__ inline bool mode_ext(const LocalizeParams& P)
+{
+	return mode_xcor(P) || mode_align(P) || mode_zweighted(P);
+}
 
 // Bytes per ROI pixel as staged in LDS: the source format when it converts to
 // float exactly and no calibration rewrites it (u8, u16), float otherwise.
@@ -51,11 +69,12 @@ __host__ __device__ inline int z_chunks(const LocalizeParams& P)
 	return c < 1 ? 1 : c;
 }
 
-__host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P)
+// `ext` false leaves the optional-mode phases out (the kernel variant without them).
+__host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ext = true)
 {
-	LdsLayout L;
+	LdsLayout L{};
 	int o = 0;
-	const int N = P.fft_n > 0 ? P.fft_n : 0;
+	const int N = P.qi_fft.n > 0 ? P.qi_fft.n : 0;
 	L.img = o;    o += align16(P.width * P.height * lds_pix_bytes(P));
 	L.zprof = o;  o += align16(P.z_nb * 4);
 	L.red = o;    o += align16(64 * 8);
@@ -68,17 +87,43 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P)
 	L.fft = q;    q += align16(4 * N * 8);
 	L.tw = q;     q += align16(2 * N * 8);
 	L.perm = q;   q += align16(N * 4);
+	// XCor1D phase: x and y profiles, 4 transforms (+ their generic-radix staging)
+	int x = u;
+	if (ext && mode_xcor(P)) {
+		const int Nx = P.xc_fft.n;
+		L.xprof = x;  x += align16(2 * Nx * 4);
+		L.xfft = x;   x += align16(4 * Nx * 8);
+		L.xtw = x;    x += align16(2 * Nx * 8);
+		L.xperm = x;  x += align16(Nx * 4);
+		L.xtmp = x;   x += align16(4 * Nx * 8);
+	}
 	// radial-profile phase
 	int z = u;
 	const int nch = z_chunks(P);
 	L.zdir = z;   z += align16(P.z_A * 8);
 	L.zpart = z;  z += align16(nch * P.z_nb * 8);
 	L.zcnt = z;   z += align16(nch * P.z_nb * 4);
+	// QuadrantAlign phase: directions, quadrant profiles, LUT row, 3 transforms
+	int a = u;
+	if (ext && mode_align(P)) {
+		const int Na = P.qa_fft.n;
+		L.adir = a;   a += align16(4 * P.qi_A * 8);
+		L.aprof = a;  a += align16(4 * P.z_nb * 4);
+		L.alut = a;   a += align16(Na * 4);
+		L.afft = a;   a += align16(3 * Na * 8);
+		L.atw = a;    a += align16(2 * Na * 8);
+		L.aperm = a;  a += align16(Na * 4);
+		L.atmp = a;   a += align16(3 * Na * 8);
+	}
 	// LUT phase
 	L.scores = u;
 	int s = u + align16(P.zplanes * 8);
+	L.zw = s;
+	if (ext && mode_zweighted(P)) s += align16(P.z_nb * 8);
 	o = q > z ? q : z;
 	o = o > s ? o : s;
+	o = o > x ? o : x;
+	o = o > a ? o : a;
 	L.total = o;
 	return L;
 }
@@ -178,45 +223,121 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double* red, int lane,
 }
 
 // kissfft's stages from the deepest (smallest) to the outermost, one lane per
-// butterfly, `nbufs` transforms of length N side by side (one per wavefront).
-__device__ __forceinline__ void fft_stages(float2* base, int nbufs, int N, const LocalizeParams& P,
-                                           const float2* tw, float nii, int wid, int lane)
+// butterfly, `nbufs` transforms of length F.n side by side (one per wavefront).
+// MIXED adds the radix-3, radix-5 and generic butterflies
+// (cputrack/kissfft.h:154-290); without it only radix 4 and 2 occur (QI's
+// power-of-two lengths).  A generic radix-p stage writes each output, a length-p
+// DFT of the group's inputs, into `tmp` and copies the stage back afterwards.
+template <bool MIXED>
+__device__ __forceinline__ void fft_stages(float2* base, float2* tmp, int nbufs, const FFTPlan& F, const float2* tw,
+                                           float nii, int wid, int lane)
 {
+	const int N = F.n;
 #pragma unroll
 	for (int si = 0; si < kMaxFFTStages; si++) {
 		const int s = kMaxFFTStages - 1 - si;
-		if (s >= P.fft_nst) continue;  // uniform: every thread skips together
-		const int r = P.fft_radix[s], m = P.fft_m[s], fs = P.fft_fs[s];
+		if (s >= F.nst) continue;  // uniform: every thread skips together
+		const int r = F.radix[s], m = F.m[s], fs = F.fs[s];
+		const bool generic = MIXED && r != 2 && r != 3 && r != 4 && r != 5;
 		if (wid < nbufs) {
-			float2* F = base + wid * N;
-			const int nbf = N / r;
-			for (int b = lane; b < nbf; b += 64) {
-				const int g = b / m, k = b - g * m;
-				float2* G = F + g * r * m;
-				if (r == 4) {
-					const float2 s0 = cmul(G[k + m], tw[k * fs]);
-					const float2 s1 = cmul(G[k + 2 * m], tw[k * fs * 2]);
-					const float2 s2 = cmul(G[k + 3 * m], tw[k * fs * 3]);
-					float2 f0 = G[k];
-					const float2 s5 = csub(f0, s1);
-					f0 = cadd(f0, s1);
-					const float2 s3 = cadd(s0, s2);
-					float2 s4 = csub(s0, s2);
-					s4 = make_float2(s4.y * nii, -s4.x * nii);
-					G[k + 2 * m] = csub(f0, s3);
-					G[k] = cadd(f0, s3);
-					G[k + m] = cadd(s5, s4);
-					G[k + 3 * m] = csub(s5, s4);
-				} else {
-					const float2 t = cmul(G[m + k], tw[k * fs]);
-					const float2 f0 = G[k];
-					G[m + k] = csub(f0, t);
-					G[k] = cadd(f0, t);
+			float2* X = base + wid * N;
+			if (!generic) {
+				const int nbf = N / r;
+				for (int b = lane; b < nbf; b += 64) {
+					const int g = b / m, k = b - g * m;
+					float2* G = X + g * r * m;
+					if (r == 4) {
+						const float2 s0 = cmul(G[k + m], tw[k * fs]);
+						const float2 s1 = cmul(G[k + 2 * m], tw[k * fs * 2]);
+						const float2 s2 = cmul(G[k + 3 * m], tw[k * fs * 3]);
+						float2 f0 = G[k];
+						const float2 s5 = csub(f0, s1);
+						f0 = cadd(f0, s1);
+						const float2 s3 = cadd(s0, s2);
+						float2 s4 = csub(s0, s2);
+						s4 = make_float2(s4.y * nii, -s4.x * nii);
+						G[k + 2 * m] = csub(f0, s3);
+						G[k] = cadd(f0, s3);
+						G[k + m] = cadd(s5, s4);
+						G[k + 3 * m] = csub(s5, s4);
+					} else if (!MIXED || r == 2) {
+						const float2 t = cmul(G[m + k], tw[k * fs]);
+						const float2 f0 = G[k];
+						G[m + k] = csub(f0, t);
+						G[k] = cadd(f0, t);
+					} else if (r == 3) {
+						// HALF_OF's product with the double 0.5 rounds like a product with 0.5f
+						const float2 e = tw[fs * m];
+						const float2 s1 = cmul(G[k + m], tw[k * fs]);
+						const float2 s2 = cmul(G[k + 2 * m], tw[2 * k * fs]);
+						const float2 s3 = cadd(s1, s2);
+						float2 s0 = csub(s1, s2);
+						const float2 f0 = G[k];
+						const float2 fm = make_float2(f0.x - s3.x * 0.5f, f0.y - s3.y * 0.5f);
+						s0 = make_float2(s0.x * e.y, s0.y * e.y);
+						G[k] = cadd(f0, s3);
+						G[k + 2 * m] = make_float2(fm.x + s0.y, fm.y - s0.x);
+						G[k + m] = cadd(fm, make_float2(-s0.y, s0.x));
+					} else {
+						const float2 ya = tw[fs * m], yb = tw[fs * 2 * m];
+						const float2 s0 = G[k];
+						const float2 s1 = cmul(G[k + m], tw[k * fs]);
+						const float2 s2 = cmul(G[k + 2 * m], tw[2 * k * fs]);
+						const float2 s3 = cmul(G[k + 3 * m], tw[3 * k * fs]);
+						const float2 s4 = cmul(G[k + 4 * m], tw[4 * k * fs]);
+						const float2 s7 = cadd(s1, s4), s10 = csub(s1, s4), s8 = cadd(s2, s3), s9 = csub(s2, s3);
+						G[k] = cadd(cadd(s0, s7), s8);
+						const float2 s5 = cadd(s0, make_float2(s7.x * ya.x + s8.x * yb.x, s7.y * ya.x + s8.y * yb.x));
+						const float2 s6 = make_float2(s10.y * ya.y + s9.y * yb.y, -(s10.x * ya.y) - s9.x * yb.y);
+						G[k + m] = csub(s5, s6);
+						G[k + 4 * m] = cadd(s5, s6);
+						const float2 s11 = cadd(s0, make_float2(s7.x * yb.x + s8.x * ya.x, s7.y * yb.x + s8.y * ya.x));
+						const float2 s12 = make_float2(-(s10.y * yb.y) + s9.y * ya.y, s10.x * yb.y - s9.x * ya.y);
+						G[k + 2 * m] = cadd(s11, s12);
+						G[k + 3 * m] = csub(s11, s12);
+					}
+				}
+			} else {
+				float2* T = tmp + wid * N;
+				const int gl = r * m;
+				for (int t = lane; t < N; t += 64) {
+					const int g = t / gl, kk = t - g * gl, u = kk % m;
+					const float2* G = X + g * gl;
+					float2 acc = G[u];
+					int ti = 0;
+					for (int q = 1; q < r; q++) {
+						ti += fs * kk;
+						if (ti >= N) ti -= N;
+						acc = cadd(acc, cmul(G[u + q * m], tw[ti]));
+					}
+					T[t] = acc;
 				}
 			}
 		}
 		__syncthreads();
+		if (generic) {
+			if (wid < nbufs)
+				for (int t = lane; t < N; t += 64) base[wid * N + t] = tmp[wid * N + t];
+			__syncthreads();
+		}
 	}
+}
+
+// NormalizeRadialProfile (cputrack/utils.cpp:230-261) by one thread: double sums,
+// float mean and deviations, 1/sqrt of the double mean square.
+__device__ __forceinline__ void normalize_profile(float* p, int n)
+{
+	double s = 0.0;
+	for (int i = 0; i < n; i++) s += (double)p[i];
+	const float pm = (float)(s / (double)n);
+	double s2 = 0.0;
+	for (int i = 0; i < n; i++) {
+		const float v = p[i] - pm;
+		p[i] = v;
+		s2 += (double)(float)(v * v);
+	}
+	const double inv = (double)1.0f / sqrt(s2 / (double)n);
+	for (int i = 0; i < n; i++) p[i] = (float)((double)p[i] * inv);
 }
 
 // QI profile concatenations (cputrack/cpu_tracker.cpp:327-343):
@@ -235,11 +356,289 @@ __device__ __forceinline__ float qi_input(const float* qp, int nr, int axis, int
 	return axis == 0 ? q0[r] + q3[r] : q0[r] + q1[r];
 }
 
+// Spectrum products a * conj(b) for two inverse transforms, written in leaf order
+// into buffers 0 and 1: buffer pairs (a0, b0) and (a1, b1).  Every thread reads
+// before any writes (at most 2N / kBlock <= 4 values per thread in registers).
+__device__ __forceinline__ void spectrum_products(float2* fft, const int* perm, int N, int a0, int b0, int a1, int b1,
+                                                  int tid)
+{
+	float2 prod[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		const int t = tid + i * kBlock;
+		if (t < 2 * N) {
+			const int ax = t >= N ? 1 : 0;
+			const int s = perm[t - ax * N];
+			const float2 a = fft[(ax ? a1 : a0) * N + s];
+			const float2 b = fft[(ax ? b1 : b0) * N + s];
+			prod[i] = cmul(a, make_float2(b.x, -b.y));
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int i = 0; i < 4; i++) {
+		const int t = tid + i * kBlock;
+		if (t < 2 * N) fft[t] = prod[i];
+	}
+	__syncthreads();
+}
+
+// The real part of G rotated by `shift` (out[x] = G[(x + shift) % N].x), its first
+// maximum over the wavefront and the 7-point QI peak fit around it
+// (cputrack/LsqQuadraticFit.h:132-176).  The result is valid in lane 0.
+__device__ __forceinline__ float rotated_peak(const float2* G, int N, int shift, int lane)
+{
+	auto at = [&](int x) {
+		int i = x + shift;
+		if (i >= N) i -= N;
+		return G[i].x;
+	};
+	float bv = 0.0f;
+	int bi = 0x7fffffff;
+	for (int x = lane; x < N; x += 64) {
+		const float v = at(x);
+		if (bi == 0x7fffffff || v > bv) { bv = v; bi = x; }
+	}
+	if (bi == 0x7fffffff) bv = -INFINITY;
+	wave_argmax(bv, bi);
+	if (bi == 0x7fffffff) bi = 0;  // only when every value is NaN: keep the window in bounds
+	return lane == 0 ? peak_fit<float>(bi, N, kQIW, at) : 0.0f;
+}
+
+// Best plane (first maximum) of the LUT scores over the workgroup and the FP64
+// 7-point fit around it (cputrack/cpu_tracker.cpp:851-861); valid in thread 0.
+__device__ __forceinline__ float scores_peak(const double* scores, int planes, double* red, float* st, int tid,
+                                             int lane, int wid, int& best)
+{
+	double bv = 0.0;
+	int bi = 0x7fffffff;
+	for (int k = tid; k < planes; k += kBlock) {
+		const double v = scores[k];
+		if (bi == 0x7fffffff || v > bv) { bv = v; bi = k; }
+	}
+	if (bi == 0x7fffffff) bv = -INFINITY;
+	wave_argmax(bv, bi);
+	if (lane == 0) {
+		red[2 * wid] = bv;
+		((int*)st)[4 + wid] = bi;
+	}
+	__syncthreads();
+	float z = 0.0f;
+	if (tid == 0) {
+		double v = red[0];
+		int i = ((int*)st)[4];
+		for (int w = 1; w < kBlock / 64; w++) {
+			const double ov = red[2 * w];
+			const int oi = ((int*)st)[4 + w];
+			if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+		}
+		z = (float)peak_fit<double>(i, planes, kZW, [&](int k) { return scores[k]; });
+		best = i;
+	}
+	return z;
+}
+
+// ComputeXCorInterpolated (cputrack/cpu_tracker.cpp:190-270, XCorScale = 1).  Per
+// iteration an x profile (xc_len samples along x, each the mean of the in-bounds
+// samples over xc_width rows around cy) and a y profile, both taken from the
+// centre at the start of the iteration; each is autocorrelated with its reverse
+// (XCorFFTHelper, :140-158) on four wavefronts like QI, the 7-point fit finds the
+// peak, and the centre moves by (offset - 1) / 2.  A profile longer than the ROI
+// leaves the start in place with the boundary flag set.
 template <typename SPix>
+__device__ __forceinline__ void xcor1d_phase(const LocalizeParams& P, const LdsLayout& L, unsigned char* lds, const SPix* img,
+                             float& cx, float& cy, int& boundary, int tid, int lane, int wid)
+{
+	const int W = P.width, H = P.height, N = P.xc_len;
+	if (N > W || N > H) {
+		boundary = 1;
+		return;
+	}
+	const int pw = P.xc_width < N ? P.xc_width : N;
+	float* xprof = (float*)(lds + L.xprof);
+	float2* fft = (float2*)(lds + L.xfft);
+	float2* tw = (float2*)(lds + L.xtw);
+	int* perm = (int*)(lds + L.xperm);
+	float2* tmp = (float2*)(lds + L.xtmp);
+	float* st = (float*)(lds + L.st);
+	const float Wf = (float)W, Hf = (float)H;
+	const unsigned Wm2 = (unsigned)(W - 2), Hm2 = (unsigned)(H - 2);
+	for (int i = tid; i < N; i += kBlock) {
+		tw[i] = P.xc_fft.tw_fwd[i];
+		tw[N + i] = P.xc_fft.tw_inv[i];
+		perm[i] = P.xc_fft.perm[i];
+	}
+	boundary = 0;
+	const float half = (1.0f * (float)N) / 2;
+	const int ph = pw / 2;
+	for (int k = 0; k < P.xc_iters; k++) {
+		boundary = (cx + half >= Wf) || (cx - half < 0.0f) || (cy + half >= Hf) || (cy - half < 0.0f);
+		const float xmin = cx - half, ymin = cy - half;
+		const bool ok = !isnan(cx) && !isnan(cy);
+		// slot t < N: x profile sample t; t >= N: y profile sample t - N
+		for (int t = tid; t < 2 * N; t += kBlock) {
+			const int axis = t >= N ? 1 : 0, i = t - axis * N;
+			const float along = (float)i * 1.0f + (axis ? ymin : xmin);
+			const float c = axis ? cx : cy;
+			float sum = 0.0f;
+			int cnt = 0;
+			int j = 0;
+			for (; j + 4 <= pw; j += 4) {
+				float x[4], y[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					const float across = c + 1.0f * (float)(j + u - ph);
+					x[u] = axis ? across : along;
+					y[u] = axis ? along : across;
+				}
+				sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+			}
+			for (; j < pw; j++) {
+				const float across = c + 1.0f * (float)(j - ph);
+				const float x[1] = { axis ? across : along }, y[1] = { axis ? along : across };
+				sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+			}
+			xprof[t] = cnt > 0 ? sum / (float)cnt : 0.0f;
+		}
+		__syncthreads();
+		// leaves of the 4 forward transforms: x, reversed x, y, reversed y
+		{
+			float2* F = fft + wid * N;
+			const float* src = xprof + (wid >> 1) * N;
+			for (int j = lane; j < N; j += 64) {
+				int s = perm[j];
+				if (wid & 1) s = N - 1 - s;
+				F[j] = make_float2(src[s], 0.0f);
+			}
+		}
+		__syncthreads();
+		fft_stages<true>(fft, tmp, 4, P.xc_fft, tw, 1.0f, wid, lane);
+		spectrum_products(fft, perm, N, 0, 1, 2, 3, tid);
+		fft_stages<true>(fft, tmp, 2, P.xc_fft, tw + N, -1.0f, wid, lane);
+		if (wid < 2) {
+			const float maxPos = rotated_peak(fft + wid * N, N, N / 2, lane);
+			if (lane == 0) {
+				const float off = maxPos - (float)N / 2;
+				st[wid] = (wid == 0 ? cx : cy) + (off - 1) * 1.0f * 0.5f;
+			}
+		}
+		__syncthreads();
+		cx = st[0];
+		cy = st[1];
+		__syncthreads();
+	}
+}
+
+// QuadrantAlign (cputrack/cpu_tracker.cpp:449-512, offsets :409-438): the bead's LUT
+// rows around z interpolated in float into [rev(L), L], the four quadrant profiles
+// on the Z-LUT's radii (QI's direction table, no radial weights), each normalised,
+// then X = [rev(q1+q2), q0+q3] and Y = [rev(q2+q3), q0+q1] correlated against the
+// LUT row: three forward transforms on three wavefronts, two inverse.  The centre
+// moves by the 7-point peak offset times pi/4 and the Z-LUT's bin width.
+template <typename SPix>
+__device__ __forceinline__ void quadrant_align_phase(const LocalizeParams& P, const LdsLayout& L, unsigned char* lds, const SPix* img,
+                                     float mean, const float* lut, float z, float& cx, float& cy, int tid, int lane,
+                                     int wid)
+{
+	const int W = P.width, H = P.height, nb = P.z_nb, N = P.qa_fft.n, A = P.qi_A;
+	float2* qdir = (float2*)(lds + L.adir);
+	float* qprof = (float*)(lds + L.aprof);
+	float* lutc = (float*)(lds + L.alut);
+	float2* fft = (float2*)(lds + L.afft);
+	float2* tw = (float2*)(lds + L.atw);
+	int* perm = (int*)(lds + L.aperm);
+	float2* tmp = (float2*)(lds + L.atmp);
+	float* st = (float*)(lds + L.st);
+	const float Wf = (float)W, Hf = (float)H;
+	const unsigned Wm2 = (unsigned)(W - 2), Hm2 = (unsigned)(H - 2);
+	// rows clamp((int)z) and clamp((int)z + 1); frac = z - (int)z
+	const int iz = trunc_sat(z);
+	const int izc = iz < -1 ? -1 : (iz > P.zplanes ? P.zplanes : iz);
+	const int zp0 = izc < 0 ? 0 : (izc > P.zplanes - 1 ? P.zplanes - 1 : izc);
+	const int zp1 = izc + 1 < 0 ? 0 : (izc + 1 > P.zplanes - 1 ? P.zplanes - 1 : izc + 1);
+	const float frac = z - (float)iz;
+	const float* l0 = lut + (size_t)zp0 * nb;
+	const float* l1 = lut + (size_t)zp1 * nb;
+	for (int j = tid; j < N; j += kBlock) {
+		const int r = j < nb ? nb - 1 - j : j - nb;
+		lutc[j] = l0[r] + (l1[r] - l0[r]) * frac;
+		tw[j] = P.qa_fft.tw_fwd[j];
+		tw[N + j] = P.qa_fft.tw_inv[j];
+		perm[j] = P.qa_fft.perm[j];
+	}
+	for (int j = tid; j < 4 * A; j += kBlock) {
+		const int q = j / A;
+		const float2 d = P.qi_dirs[j - q * A];
+		const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
+		const float sy = (q < 2) ? 1.0f : -1.0f;
+		qdir[j] = make_float2(sx * d.x, sy * d.y);
+	}
+	__syncthreads();
+	const float rstep = (P.z_maxr - P.z_minr) / (float)nb;
+	const int steps = P.qa_steps, stride = P.qa_stride;
+	const bool ok = !isnan(cx) && !isnan(cy);
+	for (int base = 0; base < 4 * nb; base += kBlock) {
+		const int p = base + tid;
+		if (p < 4 * nb) {
+			const int q = p / nb, ri = p - q * nb;
+			const float2* qd = qdir + q * A;
+			const float r = P.z_minr + rstep * (float)ri;
+			float sum = 0.0f;
+			int cnt = 0;
+			int a = 0;
+			for (; a + 4 <= steps; a += 4) {
+				float x[4], y[4];
+#pragma unroll
+				for (int u = 0; u < 4; u++) {
+					const float2 d = qd[stride * (a + u)];
+					x[u] = cx + d.x * r;
+					y[u] = cy + d.y * r;
+				}
+				sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+			}
+			for (; a < steps; a++) {
+				const float2 d = qd[stride * a];
+				const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+				sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+			}
+			qprof[p] = cnt >= 4 ? sum / (float)cnt : mean;
+		}
+	}
+	__syncthreads();
+	if (tid < 4) normalize_profile(qprof + tid * nb, nb);
+	__syncthreads();
+	if (wid < 3) {
+		float2* F = fft + wid * N;
+		for (int j = lane; j < N; j += 64) {
+			const int s = perm[j];
+			F[j] = make_float2(wid < 2 ? qi_input(qprof, nb, wid, s) : lutc[s], 0.0f);
+		}
+	}
+	__syncthreads();
+	fft_stages<true>(fft, tmp, 3, P.qa_fft, tw, 1.0f, wid, lane);
+	spectrum_products(fft, perm, N, 0, 2, 1, 2, tid);
+	fft_stages<true>(fft, tmp, 2, P.qa_fft, tw + N, -1.0f, wid, lane);
+	const float ppl = (P.z_maxr - P.z_minr) / (float)nb;
+	if (wid < 2) {
+		const float maxPos = rotated_peak(fft + wid * N, N, nb, lane);
+		if (lane == 0) {
+			const float off = (maxPos - (float)nb) * (3.14159265359f / 4);
+			st[wid] = (wid == 0 ? cx : cy) + off * ppl;
+		}
+	}
+	__syncthreads();
+	cx = st[0];
+	cy = st[1];
+	__syncthreads();
+}
+
+// EXT compiles in the optional modes (LT_XCor1D, LT_ZLUTAlign,
+// LT_LocalizeZWeighted); the variant without them is the one the common modes run.
+template <typename SPix, bool EXT>
 __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 {
 	extern __shared__ __align__(16) unsigned char lds[];
-	const LdsLayout L = lds_layout(P);
+	const LdsLayout L = lds_layout(P, EXT);
 	SPix* img = (SPix*)(lds + L.img);
 	float2* qdir = (float2*)(lds + L.qdir);
 	float2* zdir = (float2*)(lds + L.zdir);
@@ -330,19 +729,25 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		}
 	}
 
-	// ---- QI (cputrack/cpu_tracker.cpp:285-406)
+	// ---- XCor1D, which takes precedence over QI (cputrack/QueuedCPUTracker.cpp:299-303)
 	float cx = comx, cy = comy;
 	int boundary = 0;
-	const bool do_qi = (P.mode & kModeQI) || (build && !known);
+	const bool xcor = EXT && mode_xcor(P);
+	if constexpr (EXT) {
+		if (xcor) xcor1d_phase<SPix>(P, L, lds, img, cx, cy, boundary, tid, lane, wid);
+	}
+
+	// ---- QI (cputrack/cpu_tracker.cpp:285-406)
+	const bool do_qi = ((P.mode & kModeQI) && !xcor) || (build && !known);
 	if (do_qi && P.qi_iters > 0) {
-		const int nr = P.qi_nr, N = P.fft_n;
+		const int nr = P.qi_nr, N = P.qi_fft.n;
 		const float maxr = P.qi_maxr, minr = P.qi_minr;
 		const float rstep = (maxr - minr) / (float)nr;
 		const float* window = build ? nullptr : P.qi_window;
 		for (int i = tid; i < N; i += kBlock) {
-			tw[i] = P.tw_fwd[i];
-			tw[N + i] = P.tw_inv[i];
-			perm[i] = P.fft_perm[i];
+			tw[i] = P.qi_fft.tw_fwd[i];
+			tw[N + i] = P.qi_fft.tw_inv[i];
+			perm[i] = P.qi_fft.perm[i];
 		}
 		// direction table per quadrant with the quadrant's signs applied: (mx*dx, my*dy),
 		// exact sign flips of the reference's mx*quadrantDirs[i].x (cputrack/cpu_tracker.cpp:658-659)
@@ -401,7 +806,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				}
 			}
 			__syncthreads();
-			fft_stages(fft, 4, N, P, tw, 1.0f, wid, lane);
+			fft_stages<false>(fft, nullptr, 4, P.qi_fft, tw, 1.0f, wid, lane);
 			// F(p) * conj(F(rev p)) for X and Y, held in registers until every thread has
 			// read the forward spectra, then written in leaf order into buffers 0 and 1
 			// for the inverse transforms (2N / kBlock <= 4 values per thread)
@@ -426,7 +831,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				}
 			}
 			__syncthreads();
-			fft_stages(fft, 2, N, P, tw + N, -1.0f, wid, lane);
+			fft_stages<false>(fft, nullptr, 2, P.qi_fft, tw + N, -1.0f, wid, lane);
 			if (wid < 2) {
 				const float2* G = fft + wid * N;
 				float bv = 0.0f;
@@ -454,7 +859,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	// ---- 2D Gaussian MLE when QI is off (cputrack/cpu_tracker.cpp:514-589, selected at
 	// cputrack/QueuedCPUTracker.cpp:309-313): pixel-integrated Gaussian, independent Newton
 	// steps for x, y, I0 and bg, float model terms summed in double over the ROI.
-	if (!build && !(P.mode & kModeQI) && (P.mode & kModeGauss2D)) {
+	if (!build && !xcor && !(P.mode & kModeQI) && (P.mode & kModeGauss2D)) {
 		const float sigma = P.gauss_sigma;
 		float px = comx, py = comy;
 		float I0 = mean * 0.5f * (float)W * (float)H;
@@ -503,115 +908,136 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		cy = py;
 	}
 
-	// ---- radial profile (cputrack/utils.cpp:275-320) and normalisation (:230-261)
+	// ---- radial profile (cputrack/utils.cpp:275-320) and normalisation (:230-261), LUT
+	// compare; with LT_ZLUTAlign 5 passes, each profiling at the current centre,
+	// quadrant-aligning it from the second pass on and scoring the pass's profile, then
+	// the adjusted-weight compare (cputrack/QueuedCPUTracker.cpp:319-344)
 	float z = 0.0f;
 	const bool lz = (P.mode & kModeLocalizeZ) || build;
 	if (lz && P.z_nb > 0) {
 		const int nb = P.z_nb, nch = z_chunks(P);
 		const float rstep = (P.z_maxr - P.z_minr) / (float)nb;
-		const bool ok = !isnan(cx) && !isnan(cy);
-		for (int j = tid; j < P.z_A; j += kBlock) zdir[j] = P.z_dirs[j];
-		__syncthreads();
-		for (int base = 0; base < nch * nb; base += kBlock) {
-			const int p = base + tid;
-			if (p < nch * nb) {
-				const int ch = p / nb, bin = p - ch * nb;
-				const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
-				const float r = P.z_minr + rstep * (float)bin;
-				double s = 0.0;
-				int c = 0;
-				int a = a0;
-				for (; a + 4 <= a1; a += 4) {
-					float x[4], y[4];
+		const bool has_lut = !build && P.zluts && bead >= 0 && bead < P.zcount && P.zplanes > 0;
+		const float* L0 = has_lut ? P.zluts + (size_t)bead * P.zplanes * nb : nullptr;
+		const int passes = (EXT && mode_align(P) && has_lut) ? 5 : 1;
+		for (int pass = 0; pass < passes; pass++) {
+			const bool ok = !isnan(cx) && !isnan(cy);
+			for (int j = tid; j < P.z_A; j += kBlock) zdir[j] = P.z_dirs[j];
+			__syncthreads();
+			for (int base = 0; base < nch * nb; base += kBlock) {
+				const int p = base + tid;
+				if (p < nch * nb) {
+					const int ch = p / nb, bin = p - ch * nb;
+					const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
+					const float r = P.z_minr + rstep * (float)bin;
+					double s = 0.0;
+					int c = 0;
+					int a = a0;
+					for (; a + 4 <= a1; a += 4) {
+						float x[4], y[4];
 #pragma unroll
-					for (int u = 0; u < 4; u++) {
-						const float2 d = zdir[a + u];
-						x[u] = cx + d.x * r;
-						y[u] = cy + d.y * r;
+						for (int u = 0; u < 4; u++) {
+							const float2 d = zdir[a + u];
+							x[u] = cx + d.x * r;
+							y[u] = cy + d.y * r;
+						}
+						sample_n_acc<SPix, double, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
 					}
-					sample_n_acc<SPix, double, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+					for (; a < a1; a++) {
+						const float2 d = zdir[a];
+						const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+						sample_n_acc<SPix, double, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+					}
+					zpart[p] = s;
+					zcnt[p] = c;
 				}
-				for (; a < a1; a++) {
-					const float2 d = zdir[a];
-					const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
-					sample_n_acc<SPix, double, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+			}
+			__syncthreads();
+			for (int bin = tid; bin < nb; bin += kBlock) {
+				double s = zpart[bin];
+				int c = zcnt[bin];
+				for (int ch = 1; ch < nch; ch++) {
+					s += zpart[ch * nb + bin];
+					c += zcnt[ch * nb + bin];
 				}
-				zpart[p] = s;
-				zcnt[p] = c;
+				zprof[bin] = c > 4 ? (float)(s / (double)c) : mean;
 			}
-		}
-		__syncthreads();
-		for (int bin = tid; bin < nb; bin += kBlock) {
-			double s = zpart[bin];
-			int c = zcnt[bin];
-			for (int ch = 1; ch < nch; ch++) {
-				s += zpart[ch * nb + bin];
-				c += zcnt[ch * nb + bin];
+			__syncthreads();
+			if ((P.mode & kModeNormalizeProfile) && tid == 0) normalize_profile(zprof, nb);
+			__syncthreads();
+			boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
+			           (cy - P.z_maxr < 0.0f);
+			if (P.prof_out && !build)
+				for (int bin = tid; bin < nb; bin += kBlock) P.prof_out[(size_t)roi * nb + bin] = zprof[bin];
+			if constexpr (EXT) {
+				// the centre moves; the profile of this pass stays the one scored, and
+				// QuadrantAlign leaves the boundary flag cleared (cputrack/cpu_tracker.cpp:484)
+				if (pass > 0) {
+					quadrant_align_phase<SPix>(P, L, lds, img, mean, L0, z, cx, cy, tid, lane, wid);
+					boundary = 0;
+				}
 			}
-			zprof[bin] = c > 4 ? (float)(s / (double)c) : mean;
-		}
-		__syncthreads();
-		if ((P.mode & kModeNormalizeProfile) && tid == 0) {
-			double s = 0.0;
-			for (int i = 0; i < nb; i++) s += (double)zprof[i];
-			const float pm = (float)(s / (double)nb);
-			double s2 = 0.0;
-			for (int i = 0; i < nb; i++) {
-				const float v = zprof[i] - pm;
-				zprof[i] = v;
-				s2 += (double)(float)(v * v);
-			}
-			const double inv = (double)1.0f / sqrt(s2 / (double)nb);
-			for (int i = 0; i < nb; i++) zprof[i] = (float)((double)zprof[i] * inv);
-		}
-		__syncthreads();
-		boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
-		           (cy - P.z_maxr < 0.0f);
-		if (P.prof_out && !build)
-			for (int bin = tid; bin < nb; bin += kBlock) P.prof_out[(size_t)roi * nb + bin] = zprof[bin];
 
-		if (build) {
-			float* dst = P.build_lut + ((size_t)roi * P.zplanes + P.build_plane) * nb;
-			for (int bin = tid; bin < nb; bin += kBlock) dst[bin] += zprof[bin];
-		} else if (P.zluts && bead >= 0 && bead < P.zcount && P.zplanes > 0) {
-			// ---- LUT scores, best plane, FP64 peak fit (cputrack/cpu_tracker.cpp:791-868)
-			const float* L0 = P.zluts + (size_t)bead * P.zplanes * nb;
-			for (int k = tid; k < P.zplanes; k += kBlock) {
-				const float* Lk = L0 + (size_t)k * nb;
-				double s = 0.0;
-				for (int r = 0; r < nb; r++) {
-					double d = (double)(zprof[r] - Lk[r]);
-					if (P.zcmp) d *= (double)P.zcmp[r];
-					d = -d * d;
-					s += d;
+			if (build) {
+				float* dst = P.build_lut + ((size_t)roi * P.zplanes + P.build_plane) * nb;
+				for (int bin = tid; bin < nb; bin += kBlock) dst[bin] += zprof[bin];
+			} else if (has_lut) {
+				// ---- LUT scores, best plane, FP64 peak fit (cputrack/cpu_tracker.cpp:791-868)
+				for (int k = tid; k < P.zplanes; k += kBlock) {
+					const float* Lk = L0 + (size_t)k * nb;
+					double s = 0.0;
+					for (int r = 0; r < nb; r++) {
+						double d = (double)(zprof[r] - Lk[r]);
+						if (P.zcmp) d *= (double)P.zcmp[r];
+						d = -d * d;
+						s += d;
+					}
+					scores[k] = s;
+					if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k] = (float)s;
 				}
-				scores[k] = s;
-				if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k] = (float)s;
-			}
-			__syncthreads();
-			double bv = 0.0;
-			int bi = 0x7fffffff;
-			for (int k = tid; k < P.zplanes; k += kBlock) {
-				const double v = scores[k];
-				if (bi == 0x7fffffff || v > bv) { bv = v; bi = k; }
-			}
-			if (bi == 0x7fffffff) bv = -INFINITY;
-			wave_argmax(bv, bi);
-			if (lane == 0) {
-				red[2 * wid] = bv;
-				((int*)st)[4 + wid] = bi;
-			}
-			__syncthreads();
-			if (tid == 0) {
-				double v = red[0];
-				int i = ((int*)st)[4];
-				for (int w = 1; w < kBlock / 64; w++) {
-					const double ov = red[2 * w];
-					const int oi = ((int*)st)[4 + w];
-					if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+				__syncthreads();
+				int best = 0;
+				z = scores_peak(scores, P.zplanes, red, st, tid, lane, wid, best);
+				if (tid == 0 && P.best_out) P.best_out[roi] = best;
+				if constexpr (EXT) {
+					// every thread needs z for the next alignment or the adjusted weights
+					if (tid == 0) st[2] = z;
+					__syncthreads();
+					z = st[2];
 				}
-				z = (float)peak_fit<double>(i, P.zplanes, kZW, [&](int k) { return scores[k]; });
-				if (P.best_out) P.best_out[roi] = i;
+			}
+		}
+		if constexpr (EXT) {
+			if (mode_zweighted(P) && has_lut) {
+				// ---- LUTProfileCompareAdjustedWeights (cputrack/cpu_tracker.cpp:871-901): radius
+				// weights |L[zi+1] - L[zi]| * r_i around the plane of z, double differences.  A z
+				// below 0 (or NaN) uses plane 0; the reference would index before the LUT there.
+				double* zw = (double*)(lds + L.zw);
+				int zi = trunc_sat(z);
+				if (zi > P.zplanes - 2) zi = P.zplanes - 2;
+				if (zi < 0) zi = 0;
+				const int zi1 = zi + 1 < P.zplanes ? zi + 1 : zi;
+				const float* z0 = L0 + (size_t)zi * nb;
+				const float* z1 = L0 + (size_t)zi1 * nb;
+				__syncthreads();  // the last pass's scores are read; zw follows them in LDS
+				for (int r = tid; r < nb; r += kBlock)
+					zw[r] = fabs((double)z1[r] - (double)z0[r]) *
+					        (double)(P.z_minr + (P.z_maxr - P.z_minr) * (float)r / (float)nb);
+				__syncthreads();
+				for (int k = tid; k < P.zplanes; k += kBlock) {
+					const float* Lk = L0 + (size_t)k * nb;
+					double s = 0.0;
+					for (int r = 0; r < nb; r++) {
+						double d = (double)zprof[r] - (double)Lk[r];
+						d = -d * d;
+						d *= zw[r];
+						s += d;
+					}
+					scores[k] = s;
+				}
+				__syncthreads();
+				int best = 0;
+				z = scores_peak(scores, P.zplanes, red, st, tid, lane, wid, best);
 			}
 		}
 	}
@@ -631,20 +1057,28 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	}
 }
 
+template <typename SPix, bool EXT>
+static void launch_variant(const LocalizeParams& p, size_t lds, hipStream_t stream)
+{
+	static bool attr_done = false;
+	if (!attr_done) {
+		hipFuncSetAttribute((const void*)localize_kernel<SPix, EXT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                    160 * 1024);
+		attr_done = true;
+	}
+	hipLaunchKernelGGL((localize_kernel<SPix, EXT>), dim3(p.n), dim3(kBlock), lds, stream, p);
+}
+
 hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
 {
 	if (p.n <= 0) return hipSuccess;
 	const size_t lds = localize_lds_bytes(p);
-	static bool attr_done = false;
-	if (!attr_done) {
-		hipFuncSetAttribute((const void*)localize_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-		hipFuncSetAttribute((const void*)localize_kernel<uint16_t>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-		hipFuncSetAttribute((const void*)localize_kernel<uint8_t>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-		attr_done = true;
-	}
-	if (p.lds_pix == 0) hipLaunchKernelGGL(localize_kernel<uint8_t>, dim3(p.n), dim3(kBlock), lds, stream, p);
-	else if (p.lds_pix == 1) hipLaunchKernelGGL(localize_kernel<uint16_t>, dim3(p.n), dim3(kBlock), lds, stream, p);
-	else hipLaunchKernelGGL(localize_kernel<float>, dim3(p.n), dim3(kBlock), lds, stream, p);
+	const bool ext = mode_ext(p);
+	if (ext && mode_xcor(p) && (p.xc_fft.n < 2 || p.xc_fft.n > kMaxFFTLen)) return hipErrorInvalidValue;
+	if (ext && mode_align(p) && (p.qa_fft.n != 2 * p.z_nb || p.qa_fft.n > kMaxFFTLen)) return hipErrorInvalidValue;
+	if (p.lds_pix == 0) ext ? launch_variant<uint8_t, true>(p, lds, stream) : launch_variant<uint8_t, false>(p, lds, stream);
+	else if (p.lds_pix == 1) ext ? launch_variant<uint16_t, true>(p, lds, stream) : launch_variant<uint16_t, false>(p, lds, stream);
+	else ext ? launch_variant<float, true>(p, lds, stream) : launch_variant<float, false>(p, lds, stream);
 	return hipGetLastError();
 }
 

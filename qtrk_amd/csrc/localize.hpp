@@ -33,6 +33,21 @@ enum : int {
 	kModeNormalizeProfile = 64, kModeClearFirstFour = 128, kModeFourierLUT = 256, kModeLocalizeZWeighted = 512
 };
 
+// kissfft's plan for one transform length (cputrack/kissfft.h:11-47): stage radices
+// ("4s, then 2s, then 3, 5, 7, ..."), remainders, twiddle strides, the input index
+// read at each leaf, and the forward/inverse twiddles in GPU memory.
+struct FFTPlan {
+	int n, nst;
+	int radix[kMaxFFTStages], m[kMaxFFTStages], fs[kMaxFFTStages];
+	const int* perm;            // [n]
+	const float2* tw_fwd;       // [n]
+	const float2* tw_inv;       // [n]
+};
+
+// Largest transform the kernels stage: the spectrum products of two transforms are
+// held in registers, at most 4 per thread.
+constexpr int kMaxFFTLen = 2 * kBlock;
+
 struct LocalizeParams {
 	// ROI source: ROI i starts at src + roi_offset[i] (or src + i*height*pitch
 	// when roi_offset is null); rows are `pitch` bytes apart.
@@ -66,12 +81,16 @@ struct LocalizeParams {
 	const float2* qi_dirs;      // [qi_A]
 	const float* qi_window;     // [qi_nr] or null
 
-	// kissfft-ordered FFT of length fft_n = 2*qi_nr
-	int fft_n, fft_nst;
-	int fft_radix[kMaxFFTStages], fft_m[kMaxFFTStages], fft_fs[kMaxFFTStages];
-	const int* fft_perm;        // [fft_n] leaf order
-	const float2* tw_fwd;       // [fft_n]
-	const float2* tw_inv;       // [fft_n]
+	FFTPlan qi_fft;             // length 2*qi_nr (a power of two)
+
+	// LT_XCor1D (cputrack/cpu_tracker.cpp:190-270): profile length, width and iterations
+	int xc_len, xc_width, xc_iters;
+	FFTPlan xc_fft;             // length xc_len
+
+	// LT_ZLUTAlign (cputrack/cpu_tracker.cpp:409-512): angular samples per quadrant and
+	// their stride through qi_dirs, and the transform of length 2*z_nb
+	int qa_steps, qa_stride;
+	FFTPlan qa_fft;
 
 	// radial profile + Z-LUT
 	int z_nb, z_A;

@@ -125,8 +125,9 @@ def _render(luts, lut_index, pos, cfg, peak, bg, rng, dtype):
     return np.clip(cnt, 0, info.max).astype(dtype)
 
 
-def make_workload(name, n=None, seed=None):
-    """Returns dict(settings, mode, luts, rois, jobs, truth) for a named workload.
+def make_workload(name, n=None, seed=None, **overrides):
+    """Returns dict(settings, mode, luts, rois, jobs, truth) for a named workload; `overrides`
+    replace QTrkSettings fields (the LUTs follow the resulting radial sampling).
 
     plain64:      1 bead, 100 planes x 32 steps, n (default 1000) float32 64x64 ROIs,
                   centre +-4 px, z in [5, 95], peak 10,000, background 20.
@@ -138,7 +139,7 @@ def make_workload(name, n=None, seed=None):
         return _gauss_workload(n, seed)
     idx = {"plain64": 0, "realtime100": 1}[name]
     rng = np.random.Generator(np.random.PCG64(SEED_BASE + idx if seed is None else seed))
-    st = api.default_settings(**SETTINGS[name])
+    st = api.default_settings(**dict(SETTINGS[name], **overrides))
     cfg = api.computed_config(st)
     if name == "plain64":
         n = 1000 if n is None else n

@@ -1,0 +1,139 @@
+"""Parity of the optional localization modes with the CPU oracle, through the C ABI.
+
+LT_XCor1D (cputrack/cpu_tracker.cpp:190-270), LT_ZLUTAlign's five profile /
+QuadrantAlign / compare passes (cputrack/QueuedCPUTracker.cpp:319-344,
+cputrack/cpu_tracker.cpp:409-512) and LT_LocalizeZWeighted
+(cputrack/cpu_tracker.cpp:871-901), alone and combined, on power-of-two and
+mixed-radix transform lengths: 2^k, 4*4*3, 4*3*5, 4*2*7 (generic radix 7),
+2*3*3*5 and 2*3*31 (generic radix 31).
+
+Bars as for the common modes: x/y within 1e-3 px, z within 1e-3 planes, the
+boundary flag, best plane and radial profile identical.  The kernels follow
+the oracle's float rounding operation for operation (the FFTs reproduce
+kissfft's butterflies), so the counts of identical values are printed too.
+"""
+import numpy as np
+import pytest
+
+from qtrk_amd import api, synth
+
+from test_gpu_parity import _assert_parity, _batch, _oracle, _report, _tracker
+
+pytestmark = pytest.mark.gpu
+
+Z = api.LT_LocalizeZ | api.LT_NormalizeProfile
+XCOR_Z = api.LT_XCor1D | Z
+ALIGN = api.LT_QI | Z | api.LT_ZLUTAlign
+WEIGHTED = api.LT_QI | Z | api.LT_LocalizeZWeighted
+ALIGN_WEIGHTED = ALIGN | api.LT_LocalizeZWeighted
+
+
+def _parity(oracle, w, mode, tag):
+    print(f"[{tag}] mode {mode:#x} n {len(w['jobs'])} xc {w['cfg'].xc1_profileLength} "
+          f"zsteps {w['cfg'].zlut_radialsteps}", flush=True)
+    ref, rbest, rprof = _oracle(oracle, w, mode=mode)
+    trk = _tracker(w, mode=mode)
+    try:
+        got, best, prof = _batch(trk, w["rois"], w["jobs"])
+    finally:
+        trk.free()
+    _report(tag, got, ref)
+    _assert_parity(got, ref, best, rbest)
+    if mode & api.LT_LocalizeZ:
+        np.testing.assert_allclose(prof, rprof, atol=1e-5)
+    return got, ref
+
+
+@pytest.mark.parametrize("length", [32, 48, 60, 56])
+def test_xcor1d_parity(gpu, oracle, length):
+    w = synth.make_workload("plain64", n=384, xc1_profileLength=length, xc1_profileWidth=16, xc1_iterations=3)
+    got, ref = _parity(oracle, w, XCOR_Z, f"xcor {length}")
+    # XCor1D localises the beads (the COM start is off by up to a pixel)
+    e = np.hypot(got["x"] - w["truth"][:, 0], got["y"] - w["truth"][:, 1])
+    print(f"[xcor {length}] median |xy - truth| {np.median(e):.4f} px", flush=True)
+    assert np.median(e) < 0.1
+
+
+def test_xcor1d_profile_longer_than_roi(gpu, oracle):
+    """The default 128-sample profile does not fit a 64x64 ROI: the COM is kept, boundary set."""
+    w = synth.make_workload("plain64", n=64)
+    got, _ = _parity(oracle, w, api.LT_XCor1D | api.LT_QI, "xcor 128 on 64px")
+    assert np.array_equal(got["x"], got["comx"]) and np.all(got["error"] == 1)
+
+
+@pytest.mark.parametrize("mode", [ALIGN, WEIGHTED, ALIGN_WEIGHTED], ids=["align", "weighted", "align+weighted"])
+def test_zlut_align_and_weighted_realtime100(gpu, oracle, mode):
+    w = synth.make_workload("realtime100", n=512)
+    got, ref = _parity(oracle, w, mode, f"realtime100 {mode:#x}")
+    if mode & api.LT_ZLUTAlign:
+        assert np.all(got["error"] == 0)  # QuadrantAlign clears the flag
+
+
+@pytest.mark.parametrize("coverage,steps", [(1.25, 32), (1.75, 45), (93.5 / 26, 93)], ids=["n64", "n90", "n186"])
+def test_zlut_align_mixed_radix(gpu, oracle, coverage, steps):
+    w = synth.make_workload("plain64", n=256, zlut_radial_coverage=coverage)
+    assert w["cfg"].zlut_radialsteps == steps
+    _parity(oracle, w, ALIGN_WEIGHTED, f"align {2 * steps}")
+
+
+def test_xcor1d_with_zlut_align(gpu, oracle):
+    """XCor1D start, then the alignment passes (QI's direction table sets the quadrant sampling)."""
+    w = synth.make_workload("plain64", n=256, xc1_profileLength=48, xc1_profileWidth=16, xc1_iterations=2)
+    _parity(oracle, w, api.LT_XCor1D | Z | api.LT_ZLUTAlign | api.LT_LocalizeZWeighted, "xcor+align")
+
+
+def test_optional_mode_edges(gpu, oracle):
+    """Constant, zero, ramp and edge-spot ROIs and beads beyond the LUT through the optional modes."""
+    w = synth.make_workload("plain64", n=32, xc1_profileLength=48, xc1_profileWidth=16)
+    rois = w["rois"].copy()
+    rois[0] = 123.0
+    rois[1] = 0.0
+    rois[2][:, :] = np.arange(64, dtype=np.float32)[None, :]
+    rois[3][30:34, 1:3] = 50000
+    jobs = w["jobs"].copy()
+    jobs["zlutIndex"][4:8] = 7  # one bead only: no compare, no alignment
+    for mode in (XCOR_Z, ALIGN_WEIGHTED):
+        ref, rbest, _ = _oracle(oracle, w, mode=mode, rois=rois, jobs=jobs)
+        trk = _tracker(w, mode=mode)
+        try:
+            got, best, _ = _batch(trk, rois, jobs)
+        finally:
+            trk.free()
+        _report(f"edges {mode:#x}", got, ref)
+        _assert_parity(got, ref, best, rbest)
+        assert np.all(best[4:8] == -1) and np.all(got["z"][4:8] == 0)
+
+
+def test_optional_mode_validation(gpu):
+    w = synth.make_workload("plain64", n=2, xc1_profileLength=600)
+    trk = api.QTrkInstance(w["settings"])
+    try:
+        with pytest.raises(api.QTrkError, match="xc1_profileLength"):
+            trk.set_localization_mode(api.LT_XCor1D)
+        with pytest.raises(api.QTrkError, match="not implemented"):
+            trk.set_localization_mode(api.LT_QI | api.LT_FourierLUT)
+        trk.set_localization_mode(api.LT_QI | api.LT_ZLUTAlign | Z)
+    finally:
+        trk.free()
+    w = synth.make_workload("plain64", n=2, zlut_radial_coverage=10.0)
+    trk = api.QTrkInstance(w["settings"])
+    try:
+        with pytest.raises(api.QTrkError, match="zlut_radialsteps"):
+            trk.set_localization_mode(ALIGN)
+        trk.set_localization_mode(WEIGHTED)
+    finally:
+        trk.free()
+
+
+def test_pipeline_rejects_per_roi_compare_modes(gpu):
+    w = synth.batching_tracker(32, 16, 4, 40, 32, np.random.default_rng(3), render=False)
+    trk = api.QTrkInstance(w["settings"])
+    p = api.Pipeline()
+    try:
+        trk.set_radial_zlut(w["lut"])
+        trk.set_localization_mode(ALIGN)
+        with pytest.raises(api.QTrkError, match="LT_ZLUTAlign"):
+            p.add(trk, 16, api.QTrkU16)
+    finally:
+        p.free()
+        trk.free()
