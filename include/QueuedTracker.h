@@ -94,6 +94,9 @@ public:
 	float ZLUTBiasCorrection(float z, int zlut_planes, int bead);
 	void SetZLUTBiasCorrection(const float* table, int tablePlanes, int beads);
 	bool HasZLUTBiasCorrection() const { return !zbias.empty(); }
+	/* GetZLUTBiasCorrection (cputrack/QueuedTracker.cpp:199-203) as a copy into dst (if not
+	 * NULL) of the [beads][tablePlanes] table; returns false when no table is set. */
+	bool GetZLUTBiasCorrection(float* dst, int* tablePlanes, int* beads) const;
 
 protected:
 	std::vector<float> zbias;

@@ -59,6 +59,34 @@ QTRK_API int QTrkBatchEnableDiagnostics(QTrkBatch* b, bool enable);
 QTRK_API int QTrkBatchDownloadDiagnostics(QTrkBatch* b, int* bestPlane, float* profiles, int n);
 
 /*
+ * Z-bias table: per bead, the z error of the tracker's own localization of
+ * its LUT rendered at biasPlanes depths, inverted on every later result
+ * (QueuedTracker::ZLUTBiasCorrection, cputrack/QueuedTracker.cpp:115-140).
+ *
+ * QTrkComputeZLUTBiasTable replaces QueuedTracker::ComputeZBiasCorrection
+ * (cputrack/QueuedTracker.cpp:143-189), bound in the reference by the LabVIEW
+ * export qtrk_compute_zlut_bias_table (cputrack/lv_queuetrk_api.cpp:457-466):
+ * the renders (GenerateImageFromLUT, smpPerPixel^2 samples per pixel, B-spline
+ * or linear in z, at the ROI centre and z = plane/biasPlanes*planes) and their
+ * localization (COM, 2 QI iterations, the LUT compare weighted by the Z-LUT bin
+ * window) run on the GPU.  The table becomes active; result, when not NULL,
+ * receives it as [numbeads][biasPlanes].  0 on success.
+ *
+ * QTrkSetZLUTBiasTable replaces SetZLUTBiasCorrection (:192-197) /
+ * qtrk_set_zlut_bias_table (cputrack/lv_queuetrk_api.cpp:468-480): table is
+ * [count][biasPlanes] with count equal to the Z-LUT's bead count; NULL clears
+ * it.  QTrkSetRadialZLUT clears it too (cputrack/QueuedCPUTracker.cpp:368-369).
+ *
+ * QTrkGetZLUTBiasTable replaces GetZLUTBiasCorrection (:199-203): writes the
+ * sizes (and the table when dst is not NULL); 1 when a table is set, 0 when
+ * not, -1 on error.
+ */
+QTRK_API int QTrkComputeZLUTBiasTable(QueuedTracker* qtrk, int biasPlanes, float* result, int smpPerPixel,
+                                      bool useSplineInterp);
+QTRK_API int QTrkSetZLUTBiasTable(QueuedTracker* qtrk, const float* table, int biasPlanes, int count);
+QTRK_API int QTrkGetZLUTBiasTable(QueuedTracker* qtrk, float* dst, int* biasPlanes, int* count);
+
+/*
  * Renders n bead images of width x height from radial LUTs, on the host
  * with all cores: image i uses LUT lutIndex[i] (luts is [nluts][planes][steps])
  * at pos[i] = (x, y, z) with GenerateImageFromLUT's model (cputrack/utils.cpp:325-390).

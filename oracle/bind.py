@@ -81,6 +81,8 @@ class Oracle:
                                      fl]),
             "oq_erf": (fl, [fl]),
             "oq_lut_compare_weighted": (fl, [P, P, C.c_int, C.c_int, fl, fl, fl, P, P]),
+            "oq_zbias_apply": (C.c_float, [P, C.c_int, C.c_int, C.c_float]),
+            "oq_compute_zbias_table": (None, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]),
             "oq_xcor1d_img": (None, [P, C.c_int, C.c_int, fl, fl, C.c_int, C.c_int, C.c_int, P]),
             "oq_quadrant_align_img": (None, [P, C.c_int, C.c_int, fl, P, C.c_int, C.c_int, fl, fl, C.c_int, fl, fl,
                                              fl, P]),
@@ -179,6 +181,19 @@ class Oracle:
         z = self.L.oq_lut_compare_weighted(_p(prof), _p(lut), planes, res, minr, maxr, z_estim, _p(sc),
                                            C.byref(best))
         return z, best.value, sc
+
+    def zbias_apply(self, row, planes, z):
+        """ZLUTBiasCorrection of z with one bead's table row."""
+        row = np.ascontiguousarray(row, np.float32)
+        return self.L.oq_zbias_apply(_p(row), len(row), planes, z)
+
+    def compute_zbias_table(self, cfg, luts, bias_planes, ovs=4, spline=True):
+        """ComputeZBiasCorrection: [count][bias_planes] float32."""
+        luts = np.ascontiguousarray(luts, np.float32)
+        out = np.zeros((luts.shape[0], bias_planes), np.float32)
+        self.L.oq_compute_zbias_table(C.byref(cfg), _p(luts), luts.shape[0], luts.shape[1], bias_planes, ovs,
+                                      1 if spline else 0, _p(out))
+        return out
 
     def xcor1d(self, img, cx, cy, iterations, profile_width, length):
         """ComputeXCorInterpolated from (cx, cy): (x, y, boundary)."""

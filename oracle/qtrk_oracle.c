@@ -971,6 +971,78 @@ void oq_localize_trace(const OConfig* cfg, int mode, const float* zluts, int zco
 	free(t.qi_window);
 }
 
+/* (int)x as the reference's x86 build computes it: truncation toward zero, and the "integer
+ * indefinite" INT_MIN for NaN and for values outside the int range (cvttss2si). */
+static int trunc_x86(float x)
+{
+	if (!(x == x) || x >= 2147483648.0f || x < -2147483648.0f) return INT_MIN;
+	return (int)x;
+}
+
+/* QueuedTracker::ZLUTBiasCorrection, cputrack/QueuedTracker.cpp:115-140, with
+ * CImageData::interpolate1D (cputrack/utils.h:60-67, 99): 4 fixed-point steps
+ * pos = z - bias(pos) through the bead's table row of w entries spanning the LUT planes. */
+float oq_zbias_apply(const float* row, int w, int planes, float z)
+{
+	float pos = z;
+	for (int k = 0; k < 4; k++) {
+		float tblpos = pos / (float)planes * w;
+		int fx = trunc_x86(tblpos);
+		float bias;
+		if (fx < 0) bias = row[0];
+		else if (fx >= w - 1) bias = row[w - 1];
+		else bias = (row[fx + 1] - row[fx]) * (tblpos - fx) + row[fx];
+		pos = z - bias;
+	}
+	return pos;
+}
+
+/* QueuedTracker::ComputeZBiasCorrection, cputrack/QueuedTracker.cpp:143-189: each bead's LUT is
+ * rendered (GenerateImageFromLUT, ovs x ovs samples, B-spline or linear in z) at the ROI centre
+ * (width/2, height/2) and z = plane / biasPlanes * planes, then localized by a plain CPUTracker:
+ * COM without background correction (and without ProcessJob's NaN fallback), 2 QI iterations with
+ * the QI bin window, the normalised radial profile at the QI position and the LUT compare weighted
+ * by the Z-LUT bin window (ComputeZ, cputrack/cpu_tracker.h:95-100).  table[bead][plane] = z - true z. */
+void oq_compute_zbias_table(const OConfig* cfg, const float* zluts, int count, int planes, int biasPlanes, int ovs,
+                            int spline, float* table)
+{
+	OTracker t;
+	tracker_setup(&t, cfg, 0, zluts, count, planes, 0);
+	OWorker wk;
+	oq_worker_init(&wk, &t);
+	const int res = cfg->zlut_radialsteps;
+	fl* zwin = (fl*)malloc(sizeof(fl) * (size_t)(res > 0 ? res : 1));
+	float* zw = (float*)malloc(sizeof(float) * (size_t)(res > 0 ? res : 1));
+	fl* prof = (fl*)malloc(sizeof(fl) * (size_t)(res > 0 ? res : 1));
+	double* sc = (double*)malloc(sizeof(double) * (size_t)(planes > 0 ? planes : 1));
+	oq_radial_bin_window(res, zwin);
+	for (int i = 0; i < res; i++) zw[i] = (float)zwin[i];
+	for (int bead = 0; bead < count; bead++) {
+		const float* lut = zluts + (size_t)bead * planes * res;
+		for (int plane = 0; plane < biasPlanes; plane++) {
+			const float pz = (float)plane / (float)biasPlanes * (float)planes;
+			const fl px = (fl)(cfg->width / 2), py = (fl)(cfg->height / 2);
+			oq_generate_image_from_lut(wk.img, wk.w, wk.h, lut, planes, res, (fl)cfg->zlut_minradius,
+			                           (fl)cfg->zlut_maxradius, px, py, (fl)pz, spline, ovs);
+			fl cx, cy, qx, qy;
+			int b = 0, best = 0;
+			oq_com(&wk, 0, &cx, &cy);
+			oq_qi(&wk, cx, cy, 2, cfg->qi_radialsteps, cfg->qi_angstepspq, cfg->qi_angstep_factor,
+			      (fl)cfg->qi_minradius, (fl)cfg->qi_maxradius, t.qi_window, &qx, &qy, &b, 0);
+			oq_radial_profile(wk.img, wk.w, wk.h, prof, res, cfg->zlut_angularsteps, (fl)cfg->zlut_minradius,
+			                  (fl)cfg->zlut_maxradius, qx, qy, wk.mean, 1);
+			const float z = (float)oq_lut_compare(prof, lut, planes, res, zw, sc, &best);
+			table[(size_t)bead * biasPlanes + plane] = z - pz;
+		}
+	}
+	free(zwin);
+	free(zw);
+	free(prof);
+	free(sc);
+	oq_worker_free(&wk);
+	free(t.qi_window);
+}
+
 /* Image-level entry points of the optional modes for ctypes callers: `img` is a w*h float image,
  * `mean` the value undersampled quadrant bins fall back to.  out3 = (x, y, boundary). */
 void oq_xcor1d_img(const fl* img, int w, int h, fl cx, fl cy, int iterations, int profileWidth, int xcorw,

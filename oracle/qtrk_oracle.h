@@ -165,6 +165,11 @@ void oq_xcor1d_img(const fl* img, int w, int h, fl cx, fl cy, int iterations, in
                    fl* out3);
 void oq_quadrant_align_img(const fl* img, int w, int h, fl mean, const float* lut, int planes, int res, fl minr,
                            fl maxr, int angsteps, fl x, fl y, fl z, fl* out3);
+/* ZLUTBiasCorrection and ComputeZBiasCorrection (cputrack/QueuedTracker.cpp:115-189); table is
+ * [count][biasPlanes] */
+float oq_zbias_apply(const float* row, int w, int planes, float z);
+void oq_compute_zbias_table(const OConfig* cfg, const float* zluts, int count, int planes, int biasPlanes, int ovs,
+                            int spline, float* table);
 /* SampleFisherMatrix::Compute (cputrack/FisherMatrix.h:36-104): 3x3 (x, y, z)
  * Fisher matrix by central differences of GenerateImageFromLUT (spline, ovs 4),
  * written row-major into F[9]. */

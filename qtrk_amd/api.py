@@ -154,6 +154,9 @@ SIGNATURES = {
     "QTrkClearLastError": (None, []),
     "QTrkGetDeviceInfo": (C.c_int, [C.c_char_p, C.c_int]),
     "QTrkSetConfigValue": (None, [_P, C.c_char_p, C.c_char_p]),
+    "QTrkComputeZLUTBiasTable": (C.c_int, [_P, C.c_int, _P, C.c_int, C.c_bool]),
+    "QTrkSetZLUTBiasTable": (C.c_int, [_P, _P, C.c_int, C.c_int]),
+    "QTrkGetZLUTBiasTable": (C.c_int, [_P, _P, _P, _P]),
     "QTrkGetConfigValues": (None, [_P, C.c_char_p, C.c_int]),
     "QTrkBatchCreate": (_P, [_P, C.c_int, C.c_int]),
     "QTrkBatchFree": (None, [_P]),
@@ -316,6 +319,31 @@ class QTrkInstance:
         out = np.zeros((c, p, r), np.float32)
         if out.size:
             self._call("QTrkGetRadialZLUT", _ptr(out))
+        return out
+
+    def compute_zlut_bias_table(self, bias_planes, smp_per_pixel=4, spline=True):
+        """QueuedTracker::ComputeZBiasCorrection on the GPU; the table becomes active.
+        Returns it as float32 [beads][bias_planes]."""
+        c, _, _ = self.get_radial_zlut_size()
+        out = np.zeros((c, bias_planes), np.float32)
+        self._call("QTrkComputeZLUTBiasTable", int(bias_planes), _ptr(out), int(smp_per_pixel), bool(spline))
+        return out
+
+    def set_zlut_bias_table(self, table):
+        """Installs a [beads][bias_planes] Z-bias table (None clears it)."""
+        if table is None:
+            self._call("QTrkSetZLUTBiasTable", None, 0, 0)
+            return
+        t = np.ascontiguousarray(table, np.float32)
+        self._call("QTrkSetZLUTBiasTable", _ptr(t), t.shape[1], t.shape[0])
+
+    def get_zlut_bias_table(self):
+        """The active Z-bias table [beads][bias_planes], or None."""
+        w, h = C.c_int(), C.c_int()
+        if not self._call("QTrkGetZLUTBiasTable", None, C.byref(w), C.byref(h)):
+            return None
+        out = np.zeros((h.value, w.value), np.float32)
+        self._call("QTrkGetZLUTBiasTable", _ptr(out), C.byref(w), C.byref(h))
         return out
 
     def set_radial_weights(self, w):

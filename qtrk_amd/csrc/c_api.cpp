@@ -337,6 +337,7 @@ QTRK_API int QTrkBatchDownload(QTrkBatch* b, LocalizationResult* out, int n)
 		if (n < 0 || n > b->maxROIs) fail("n out of range");
 		HIP_CHECK(hipStreamSynchronize(b->stream));
 		HIP_CHECK(hipMemcpy(out, b->results.p, sizeof(DevResult) * n, hipMemcpyDeviceToHost));
+		if (b->trk->LocalizeMode() & LT_LocalizeZ) b->trk->ApplyZBias(out, n);
 	});
 	return ok ? 0 : -1;
 }
@@ -365,4 +366,43 @@ QTRK_API int QTrkBatchDownloadDiagnostics(QTrkBatch* b, int* bestPlane, float* p
 			                    hipMemcpyDeviceToHost));
 	});
 	return ok ? 0 : -1;
+}
+
+// ---------------------------------------------------------------- Z-bias table
+
+QTRK_API int QTrkComputeZLUTBiasTable(QueuedTracker* qtrk, int biasPlanes, float* result, int smpPerPixel,
+                                      bool useSplineInterp)
+{
+	bool ok = GUARD({
+		QueuedHIPTracker* t = dynamic_cast<QueuedHIPTracker*>(qtrk);
+		if (!t) fail("not a MI355X tracker");
+		t->ComputeZBiasCorrection(biasPlanes, result, smpPerPixel, useSplineInterp);
+	});
+	return ok ? 0 : -1;
+}
+
+QTRK_API int QTrkSetZLUTBiasTable(QueuedTracker* qtrk, const float* table, int biasPlanes, int count)
+{
+	bool ok = GUARD({
+		if (!qtrk) fail("null tracker");
+		if (table) {
+			int beads = 0, planes = 0, steps = 0;
+			qtrk->GetRadialZLUTSize(beads, planes, steps);
+			if (count != beads)
+				fail("bias table must be [numbeads][biasPlanes]; the Z-LUT has " + std::to_string(beads) + " beads");
+			if (biasPlanes < 1) fail("biasPlanes must be >= 1");
+		}
+		qtrk->SetZLUTBiasCorrection(table, biasPlanes, count);
+	});
+	return ok ? 0 : -1;
+}
+
+QTRK_API int QTrkGetZLUTBiasTable(QueuedTracker* qtrk, float* dst, int* biasPlanes, int* count)
+{
+	int r = -1;
+	GUARD({
+		if (!qtrk) fail("null tracker");
+		r = qtrk->GetZLUTBiasCorrection(dst, biasPlanes, count) ? 1 : 0;
+	});
+	return r;
 }

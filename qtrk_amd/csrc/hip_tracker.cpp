@@ -1,6 +1,7 @@
 // hip_tracker.cpp -- host runtime of the MI355X tracker.
 #include "hip_tracker.hpp"
 #include "errors.hpp"
+#include "lutmodel.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -42,6 +43,20 @@ static void upload(DeviceBuffer& b, const void* src, size_t bytes)
 {
 	b.ensure(bytes);
 	if (bytes) HIP_CHECK(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+// ComputeRadialBinWindow, cputrack/utils.cpp:665-677
+static std::vector<float> radial_bin_window(int n)
+{
+	std::vector<float> w(n);
+	for (int i = 0; i < n; i++) {
+		float x = i / (float)n;
+		float omx = 1 - x;
+		float fall = 1.0f - expf(-(omx * omx) / 0.05f);
+		float rise = 1.0f - expf(-(x * x) / 0.01f);
+		w[i] = sqrtf(fall * rise * x * 2);
+	}
+	return w;
 }
 
 // kissfft factorisation of n (cputrack/kissfft.h:29-46): 4s first, then 2s, ...
@@ -180,15 +195,7 @@ void QueuedHIPTracker::BuildTables()
 	upload(d_qi_steps, steps.data(), steps.size() * 4);
 	upload(d_qi_stride, stride.data(), stride.size() * 4);
 
-	// Radial bin window, cputrack/utils.cpp:665-677
-	std::vector<float> wnd(nr);
-	for (int i = 0; i < nr; i++) {
-		float x = i / (float)nr;
-		float omx = 1 - x;
-		float fall = 1.0f - expf(-(omx * omx) / 0.05f);
-		float rise = 1.0f - expf(-(x * x) / 0.01f);
-		wnd[i] = sqrtf(fall * rise * x * 2);
-	}
+	const std::vector<float> wnd = radial_bin_window(nr);
 	upload(d_qi_window, wnd.data(), wnd.size() * 4);
 
 	// FFT of length 2*nr (cputrack/kissfft.h:11-47)
@@ -473,19 +480,124 @@ void QueuedHIPTracker::Harvest()
 		batchesDone++;
 		roisDone += s->n;
 		const DevResult* r = s->h_results.as<DevResult>();
-		const bool bias = HasZLUTBiasCorrection() && (localizeMode & LT_LocalizeZ);
+		const bool bias = localizeMode & LT_LocalizeZ;
 		{
 			std::lock_guard<std::mutex> g(res_mtx);
 			for (int i = 0; i < s->n; i++) {
 				LocalizationResult lr;
 				memcpy(&lr, &r[i], sizeof(lr));
-				if (bias) lr.pos.z = ZLUTBiasCorrection(lr.pos.z, zplanes, lr.job.zlutIndex);
+				if (bias) ApplyZBias(&lr, 1);
 				results.push_back(lr);
 			}
 		}
 		s->n = 0;
 		s->state = Slot::Free;
 	}
+}
+
+void QueuedHIPTracker::ApplyZBias(LocalizationResult* r, int n)
+{
+	if (!HasZLUTBiasCorrection()) return;
+	for (int i = 0; i < n; i++) r[i].pos.z = ZLUTBiasCorrection(r[i].pos.z, zplanes, r[i].job.zlutIndex);
+}
+
+void QueuedHIPTracker::ComputeZBiasCorrection(int biasPlanes, float* result, int smpPerPixel, bool spline)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	WaitAllLocked(lk);
+	if (zcount <= 0 || zplanes <= 0) fail("no radial Z-LUT");
+	if (biasPlanes < 1) fail("bias_planes must be >= 1");
+	if (smpPerPixel < 1) fail("smpPerPixel must be >= 1");
+	if (zplanes < (spline ? 4 : 2)) fail("too few Z-LUT planes for the interpolation");
+	if (cfg.zlut_radialsteps < 2 || !(cfg.zlut_maxradius > cfg.zlut_minradius)) fail("invalid Z-LUT radii");
+	const int W = cfg.width, H = cfg.height, nb = cfg.zlut_radialsteps;
+	const int total = zcount * biasPlanes;
+	// renders staged in chunks of at most 512 MB
+	const int chunk = (int)std::max<size_t>(1, std::min<size_t>(total, (512u << 20) / ((size_t)W * H * 4)));
+
+	// The reference localizes each render with a plain CPUTracker: 2 QI iterations with the QI
+	// bin window, COM without background correction, the normalised profile compared with the
+	// Z-LUT bin window as radial weights (cputrack/QueuedTracker.cpp:155-181).
+	const int A = cfg.qi_angstepspq, iters = 2;
+	std::vector<int> steps(iters), stride(iters);
+	float angsteps = A / powf(cfg.qi_angstep_factor, (float)iters);
+	for (int k = 0; k < iters; k++) {
+		int st = (int)angsteps;
+		if (st < MIN_RADPROFILE_SMP_COUNT) st = MIN_RADPROFILE_SMP_COUNT;
+		steps[k] = st;
+		stride[k] = (int)((float)A / st);
+		angsteps *= cfg.qi_angstep_factor;
+	}
+	const std::vector<float> zw = radial_bin_window(nb);
+	DeviceBuffer d_steps, d_stride, d_zw, d_theta, d_idx, d_img, d_jobs, d_res;
+	upload(d_steps, steps.data(), steps.size() * 4);
+	upload(d_stride, stride.data(), stride.size() * 4);
+	upload(d_zw, zw.data(), zw.size() * 4);
+	d_theta.ensure((size_t)chunk * kModelParams * 4);
+	d_idx.ensure((size_t)chunk * 4);
+	d_img.ensure((size_t)chunk * W * H * 4);
+	d_jobs.ensure((size_t)chunk * sizeof(DevJob));
+	d_res.ensure((size_t)chunk * sizeof(DevResult));
+	std::vector<float> theta((size_t)chunk * kModelParams), zs(chunk), table(total);
+	std::vector<int> idx(chunk);
+	std::vector<DevJob> jobs(chunk);
+	std::vector<DevResult> res(chunk);
+	for (int c0 = 0; c0 < total; c0 += chunk) {
+		const int n = std::min(chunk, total - c0);
+		for (int i = 0; i < n; i++) {
+			const int bead = (c0 + i) / biasPlanes, plane = (c0 + i) % biasPlanes;
+			zs[i] = (float)plane / (float)biasPlanes * (float)zplanes;
+			float* th = &theta[(size_t)i * kModelParams];
+			th[0] = (float)(W / 2);
+			th[1] = (float)(H / 2);
+			th[2] = zs[i];
+			th[3] = 1.0f;  // mu = 0 + 1 * g: the rendered LUT image itself
+			th[4] = 0.0f;
+			idx[i] = bead;
+			jobs[i] = DevJob{ 0u, 0u, bead, 0.0f, 0.0f, 0.0f };
+		}
+		HIP_CHECK(hipMemcpyAsync(d_theta.p, theta.data(), (size_t)n * kModelParams * 4, hipMemcpyHostToDevice,
+		                         main_stream));
+		HIP_CHECK(hipMemcpyAsync(d_idx.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, main_stream));
+		HIP_CHECK(hipMemcpyAsync(d_jobs.p, jobs.data(), (size_t)n * sizeof(DevJob), hipMemcpyHostToDevice,
+		                         main_stream));
+		LutModelParams m{};
+		m.luts = d_zlut.as<float>();
+		m.count = zcount;
+		m.planes = zplanes;
+		m.steps = nb;
+		m.minr = cfg.zlut_minradius;
+		m.maxr = cfg.zlut_maxradius;
+		m.width = W;
+		m.height = H;
+		m.ovs = smpPerPixel;
+		m.spline = spline ? 1 : 0;
+		m.n = n;
+		m.theta = d_theta.as<float>();
+		m.lut_index = d_idx.as<int>();
+		m.lut_index_stride = 1;
+		m.mu = d_img.as<float>();
+		HIP_CHECK(launch_lutmodel_forward(m, main_stream));
+		LocalizeParams P;
+		FillParams(P, 2, n, d_img.p, nullptr, W * 4, d_jobs.as<DevJob>(), d_res.as<DevResult>());
+		P.mode = kModeQI | kModeLocalizeZ | kModeNormalizeProfile;
+		P.com_bgcorrection = 0.0f;
+		P.qi_iters = iters;
+		P.qi_iter_steps = d_steps.as<int>();
+		P.qi_iter_stride = d_stride.as<int>();
+		P.zcmp = d_zw.as<float>();
+		P.cmp_out = nullptr;
+		P.calib_offset = nullptr;
+		P.calib_gain = nullptr;
+		HIP_CHECK(launch_localize(P, main_stream));
+		HIP_CHECK(hipMemcpyAsync(res.data(), d_res.p, (size_t)n * sizeof(DevResult), hipMemcpyDeviceToHost,
+		                         main_stream));
+		HIP_CHECK(hipStreamSynchronize(main_stream));
+		for (int i = 0; i < n; i++) table[c0 + i] = res[i].z - zs[i];
+	}
+	SetZLUTBiasCorrection(table.data(), biasPlanes, zcount);
+	if (result) std::copy(table.begin(), table.end(), result);
 }
 
 void QueuedHIPTracker::WaitAllLocked(std::unique_lock<std::mutex>& lk)

@@ -70,6 +70,13 @@ public:
 	int LocalizeMode() const { return localizeMode; }
 	void WaitLaunched();
 	// the radial Z-LUT [ZCount()][ZPlanes()][zlut_radialsteps] and weights in GPU memory (null when unset)
+	// QueuedTracker::ComputeZBiasCorrection (cputrack/QueuedTracker.cpp:143-189) on the GPU:
+	// every bead's LUT rendered at biasPlanes depths and localized; the [ZCount()][biasPlanes]
+	// table becomes the active bias correction and is copied to `result` when not null.
+	void ComputeZBiasCorrection(int biasPlanes, float* result, int smpPerPixel, bool useSplineInterp);
+	// The active Z-bias correction applied to n results' z (as ProcessJob does,
+	// cputrack/QueuedCPUTracker.cpp:346-347).
+	void ApplyZBias(LocalizationResult* r, int n);
 	const float* DeviceZLUT() const { return zcount > 0 ? d_zlut.as<float>() : nullptr; }
 	const float* DeviceZCmp() const { return has_zcmp ? d_zcmp.as<float>() : nullptr; }
 	int ZCount() const { return zcount; }

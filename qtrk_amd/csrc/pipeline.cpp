@@ -360,6 +360,7 @@ void Pipeline::Download(int slot, LocalizationResult* out, float* theta, float* 
 		std::vector<float> z(n);
 		HIP_CHECK(hipMemcpy(z.data(), t.z.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
 		for (int i = 0; i < n; i++) out[i].pos.z = z[i];
+		t.trk->ApplyZBias(out, n);  // the chain always localizes in z
 	}
 	if ((theta || crlb) && !t.mle) fail("this tracker's chain has no MLE stage");
 	if (theta) HIP_CHECK(hipMemcpy(theta, t.theta.p, (size_t)n * kModelParams * sizeof(float), hipMemcpyDeviceToHost));

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -80,7 +81,8 @@ float QueuedTracker::ZLUTBiasCorrection(float z, int zlut_planes, int bead)
 	float pos = z;
 	for (int k = 0; k < 4; k++) {
 		float tblpos = pos / (float)zlut_planes * zbias_w;
-		int fx = (int)tblpos;
+		// (int) as the reference's x86 build truncates: NaN and out-of-range give INT_MIN
+		int fx = (tblpos == tblpos && tblpos < 2147483648.0f && tblpos >= -2147483648.0f) ? (int)tblpos : INT_MIN;
 		float bias;
 		if (fx < 0) bias = row[0];
 		else if (fx >= zbias_w - 1) bias = row[zbias_w - 1];
@@ -88,6 +90,14 @@ float QueuedTracker::ZLUTBiasCorrection(float z, int zlut_planes, int bead)
 		pos = z - bias;
 	}
 	return pos;
+}
+
+bool QueuedTracker::GetZLUTBiasCorrection(float* dst, int* tablePlanes, int* beads) const
+{
+	if (tablePlanes) *tablePlanes = zbias_w;
+	if (beads) *beads = zbias_h;
+	if (dst && !zbias.empty()) std::copy(zbias.begin(), zbias.end(), dst);
+	return !zbias.empty();
 }
 
 void QueuedTracker::SetZLUTBiasCorrection(const float* table, int tablePlanes, int beads)

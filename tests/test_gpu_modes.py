@@ -137,3 +137,70 @@ def test_pipeline_rejects_per_roi_compare_modes(gpu):
     finally:
         p.free()
         trk.free()
+
+
+# ------------------------------------------------------------ Z-bias table
+
+def _bias_workloads():
+    """plain64 (1 bead, 100 planes, 64x64) and 4 beads of realtime100 (200 planes, 100x100)."""
+    a = synth.make_workload("plain64", n=64)
+    b = synth.make_workload("realtime100", n=64)
+    b["luts"] = b["luts"][:4].copy()
+    b["jobs"]["zlutIndex"] %= 4
+    return [a, b]
+
+
+@pytest.mark.parametrize("spline,ovs", [(True, 4), (False, 2)], ids=["spline-ovs4", "linear-ovs2"])
+def test_zbias_table_matches_oracle(gpu, oracle, spline, ovs):
+    """QueuedTracker::ComputeZBiasCorrection (cputrack/QueuedTracker.cpp:143-189) on the GPU: the LUT
+    renders and their localization against the oracle's restatement, entry by entry."""
+    for w in _bias_workloads():
+        planes = w["luts"].shape[1]
+        bias_planes = 2 * planes + 7  # table planes fall between LUT planes too
+        print(f"[zbias] {w['name']} beads {w['luts'].shape[0]} bias_planes {bias_planes}", flush=True)
+        ref = oracle.compute_zbias_table(oracle.config_from_bytes(bytes(w["cfg"])), w["luts"], bias_planes, ovs,
+                                         spline)
+        trk = _tracker(w)
+        try:
+            got = trk.compute_zlut_bias_table(bias_planes, ovs, spline)
+            assert np.array_equal(trk.get_zlut_bias_table(), got)
+        finally:
+            trk.free()
+        same = int((got == ref).sum())
+        print(f"[zbias] {same}/{got.size} identical, max|d| {np.abs(got - ref).max():.2e} planes, "
+              f"bias range [{ref.min():.3f}, {ref.max():.3f}]", flush=True)
+        assert np.abs(got - ref).max() <= 1e-3
+
+
+def test_zbias_correction_applied_to_results(gpu, oracle):
+    """With a table installed every result's z is ZLUTBiasCorrection of the uncorrected z
+    (cputrack/QueuedCPUTracker.cpp:346-347), through the queue and the device-resident batch."""
+    w = synth.make_workload("realtime100", n=256)
+    trk = _tracker(w)
+    try:
+        plain, _, _ = _batch(trk, w["rois"], w["jobs"], diag=False)
+        rng = np.random.default_rng(9)
+        table = rng.uniform(-1.5, 1.5, size=(w["luts"].shape[0], 57)).astype(np.float32)
+        trk.set_zlut_bias_table(table)
+        assert np.array_equal(trk.get_zlut_bias_table(), table)
+        batch, _, _ = _batch(trk, w["rois"], w["jobs"], diag=False)
+        queued = trk.localize_all(w["rois"], w["jobs"])
+        planes = w["luts"].shape[1]
+        exp = np.array([oracle.zbias_apply(table[b], planes, float(z))
+                        for b, z in zip(plain["zlutIndex"], plain["z"])], np.float32)
+        assert np.array_equal(batch["z"], exp)
+        assert np.array_equal(queued["z"], exp)  # localize_all returns job order
+        for k in ("x", "y", "error"):
+            assert np.array_equal(batch[k], plain[k])
+        with pytest.raises(api.QTrkError, match="numbeads"):
+            trk.set_zlut_bias_table(table[:3])
+        trk.set_zlut_bias_table(None)
+        assert trk.get_zlut_bias_table() is None
+        again, _, _ = _batch(trk, w["rois"], w["jobs"], diag=False)
+        assert np.array_equal(again["z"], plain["z"])
+        # a new LUT drops the table (cputrack/QueuedCPUTracker.cpp:368-369)
+        trk.set_zlut_bias_table(table)
+        trk.set_radial_zlut(w["luts"])
+        assert trk.get_zlut_bias_table() is None
+    finally:
+        trk.free()

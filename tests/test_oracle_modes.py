@@ -175,3 +175,64 @@ def test_zlut_align_job(oracle):
     # the weighted compare runs after the passes on the last profile; x/y are unchanged
     wz = oracle.localize(cfg, LT_QI | LT_LOCALIZEZ | LT_NORMPROF | LT_ZLUTALIGN | LT_ZWEIGHTED, luts, rois, jobs)
     assert np.array_equal(wz["x"], aligned["x"]) and np.abs(wz["z"] - t[:, 2]).max() < 1.5
+
+
+# ------------------------------------------------------------ Z-bias table
+
+def test_zbias_apply_closed_forms(oracle):
+    """ZLUTBiasCorrection (cputrack/QueuedTracker.cpp:115-140): 4 fixed-point steps pos = z - bias(pos)."""
+    planes, w = 100, 50
+    assert oracle.zbias_apply(np.zeros(w), planes, 37.3) == np.float32(37.3)
+    # a constant bias is subtracted exactly
+    assert oracle.zbias_apply(np.full(w, 0.75), planes, 37.25) == np.float32(36.5)
+    # bias(pos) = c pos (the table samples planes j * planes / w): pos_4 = z (1 - c + c^2 - c^3 + c^4)
+    c = 0.125
+    row = (c * np.arange(w) * planes / w).astype(np.float32)
+    for z in (10.0, 41.5, 80.25):
+        exp = z * (1 - c + c ** 2 - c ** 3 + c ** 4)
+        assert abs(oracle.zbias_apply(row, planes, z) - exp) < 1e-4 * z
+    # outside the table the end entries hold
+    ramp = np.arange(w, dtype=np.float32)
+    assert oracle.zbias_apply(ramp, planes, -5.0) == -5.0 - ramp[0]
+    assert oracle.zbias_apply(ramp, planes, 500.0) == 500.0 - ramp[-1]
+
+
+def _bias_cfg(oracle, w):
+    """The localization ComputeZBiasCorrection applies to its renders, as a ProcessJob configuration:
+    COM without background correction, 2 QI iterations, the normalised profile."""
+    cfg = oracle.config_from_bytes(bytes(w["cfg"]))
+    cfg.qi_iterations = 2
+    cfg.com_bgcorrection = 0.0
+    return cfg
+
+
+def test_zbias_table_composition_and_round_trip(oracle):
+    """ComputeZBiasCorrection (cputrack/QueuedTracker.cpp:143-189): table = z measured on the bead's
+    own LUT render at the centre minus the true z.  The same measurement through ProcessJob (2 QI
+    iterations, radial weights = the Z-LUT bin window) reproduces every entry, and inverting the
+    table with ZLUTBiasCorrection returns the true z, on and between the table's planes."""
+    w = synth.make_workload("plain64", n=1)
+    lut = w["luts"]
+    planes, bias_planes = lut.shape[1], 200
+    table = oracle.compute_zbias_table(oracle.config_from_bytes(bytes(w["cfg"])), lut, bias_planes)
+    assert table.shape == (1, bias_planes) and np.all(np.isfinite(table))
+    cfg = _bias_cfg(oracle, w)
+    zwin = oracle.bin_window(cfg.zlut_radialsteps).astype(np.float32)
+    jobs = np.zeros(1, dtype=[("frame", "<u4"), ("timestamp", "<u4"), ("zlutIndex", "<i4"),
+                              ("ix", "<f4"), ("iy", "<f4"), ("iz", "<f4")])
+    errs = []
+    for p in (17, 64, 101, 150, 183):
+        zp = np.float32(np.float32(p) / np.float32(bias_planes) * np.float32(planes))
+        img = oracle.generate_image_from_lut(64, 64, lut[0], cfg.zlut_minradius, cfg.zlut_maxradius, 32.0, 32.0,
+                                             float(zp)).astype(np.float32)
+        r = oracle.localize(cfg, synth.MODE_QI_Z, lut, img[None], jobs, zcmp=zwin)
+        assert np.float32(r["z"][0] - zp) == table[0, p]
+        errs.append(abs(oracle.zbias_apply(table[0], planes, float(r["z"][0])) - zp))
+        # between table planes: the correction interpolates the table
+        zq = float(zp) + 0.23
+        img = oracle.generate_image_from_lut(64, 64, lut[0], cfg.zlut_minradius, cfg.zlut_maxradius, 32.0, 32.0,
+                                             zq).astype(np.float32)
+        rq = oracle.localize(cfg, synth.MODE_QI_Z, lut, img[None], jobs, zcmp=zwin)
+        errs.append(abs(oracle.zbias_apply(table[0], planes, float(rq["z"][0])) - zq))
+    print("corrected |z - truth|", np.round(errs, 5))
+    assert max(errs[0::2]) < 1e-3 and max(errs[1::2]) < 0.02
