@@ -137,16 +137,25 @@ typedef struct QTrkComputedConfig {
 } QTrkComputedConfig;
 #endif
 
-/* ResultManager structs -- cputrack/qtrk_c_api.h:210-231 (layout only; the
- * result manager itself is outside this library's scope). */
+/* ResultManager structs -- cputrack/qtrk_c_api.h:210-231 */
 typedef struct ResultManagerConfig {
 	int numBeads, numFrameInfoColumns;
 	vector3f scaling;
-	vector3f offset;
-	int writeInterval;
-	uint maxFramesInMemory;
+	vector3f offset;             /* output is (position - ROI centre + offset) * scaling */
+	int writeInterval;           /* frames between file writes */
+	uint maxFramesInMemory;      /* 0: unlimited */
 	uint8_t binaryOutput;
 } ResultManagerConfig;
+
+typedef struct RMFrameCounters {
+	int startFrame;              /* first frame held in memory */
+	int processedFrames;         /* frames below this have every bead's result (or were given up) */
+	int lastSaveFrame;
+	int capturedFrames;          /* frames with frame info stored */
+	int localizationsDone;
+	int lostFrames;
+	int fileError;
+} RMFrameCounters;
 
 #pragma pack(pop)
 
@@ -215,6 +224,26 @@ QTRK_API void QTrkFreeROIPositions(ROIPosition* data);                      /* q
 QTRK_API ROIPosition* QTrkFindBeads(ImageData* imgData, int smpCornerPosX, int smpCornerPosY, int roi,
                                     float imgRelDist, float acceptance, int* beadcount,
                                     ImageData* sample);                     /* qtrk_c_api.h:205 */
+
+/* ---- result sink -- cputrack/qtrk_c_api.h:233-242, cputrack/qtrk_c_api.cpp:181-241 ---- */
+#ifdef __cplusplus
+class ResultManager;
+#else
+typedef struct ResultManager ResultManager;
+#endif
+QTRK_API ResultManager* RMCreate(const char* file, const char* frameinfo, ResultManagerConfig* cfg,
+                                 const char* semicolonSeparatedNames);     /* qtrk_c_api.h:233 */
+QTRK_API void RMSetTracker(ResultManager* rm, QueuedTracker* qtrk);        /* qtrk_c_api.h:234 */
+QTRK_API void RMDestroy(ResultManager* rm);                                 /* qtrk_c_api.h:235 */
+QTRK_API void RMStoreFrameInfo(ResultManager* rm, int frame, double timestamp, float* cols); /* qtrk_c_api.h:236 */
+QTRK_API int RMGetBeadResults(ResultManager* rm, int start, int numFrames, int bead,
+                              LocalizationResult* results);                /* qtrk_c_api.h:237 */
+QTRK_API void RMGetFrameCounters(ResultManager* rm, RMFrameCounters* dst);  /* qtrk_c_api.h:238 */
+QTRK_API void RMFlush(ResultManager* rm);                                   /* qtrk_c_api.h:239 */
+QTRK_API int RMGetResults(ResultManager* rm, int startFrame, int numFrames,
+                          LocalizationResult* results);                    /* qtrk_c_api.h:240 */
+QTRK_API void RMRemoveBead(ResultManager* rm, int bead);                    /* qtrk_c_api.h:241 */
+QTRK_API void RMGetConfig(ResultManager* rm, ResultManagerConfig* cfg);     /* qtrk_c_api.h:242 */
 
 /* ---- utility exports bound by C# -- cputrack/utils.h:182-219, QTrkDotNet/QTrkDLL.cs:350-367 ---- */
 QTRK_API void ComputeRadialProfile(float* dst, int radialSteps, int angularSteps, float minradius,

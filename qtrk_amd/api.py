@@ -87,6 +87,22 @@ class ImageData(C.Structure):
     _fields_ = [("data", C.POINTER(C.c_float)), ("w", C.c_int), ("h", C.c_int)]
 
 
+class ResultManagerConfig(C.Structure):
+    """cputrack/qtrk_c_api.h:212-220"""
+    _pack_ = 1
+    _fields_ = [("numBeads", C.c_int), ("numFrameInfoColumns", C.c_int), ("scaling", Vector3f),
+                ("offset", Vector3f), ("writeInterval", C.c_int), ("maxFramesInMemory", C.c_uint),
+                ("binaryOutput", C.c_uint8)]
+
+
+class RMFrameCounters(C.Structure):
+    """cputrack/qtrk_c_api.h:221-230"""
+    _pack_ = 1
+    _fields_ = [("startFrame", C.c_int), ("processedFrames", C.c_int), ("lastSaveFrame", C.c_int),
+                ("capturedFrames", C.c_int), ("localizationsDone", C.c_int), ("lostFrames", C.c_int),
+                ("fileError", C.c_int)]
+
+
 # numpy views of the packed records
 JOB_DTYPE = np.dtype([("frame", "<u4"), ("timestamp", "<u4"), ("zlutIndex", "<i4"),
                       ("ix", "<f4"), ("iy", "<f4"), ("iz", "<f4")])
@@ -149,6 +165,17 @@ SIGNATURES = {
     "ApplyGaussianNoise": (None, [C.POINTER(ImageData), C.c_float]),
     "GenerateGaussianSpotImage": (None, [C.POINTER(ImageData), Vector2f, C.c_float, C.c_float, C.c_float]),
     "GetPreciseTime": (C.c_double, []),
+    # result sink: cputrack/qtrk_c_api.h:233-242
+    "RMCreate": (_P, [C.c_char_p, C.c_char_p, C.POINTER(ResultManagerConfig), C.c_char_p]),
+    "RMSetTracker": (None, [_P, _P]),
+    "RMDestroy": (None, [_P]),
+    "RMStoreFrameInfo": (None, [_P, C.c_int, C.c_double, _P]),
+    "RMGetBeadResults": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, _P]),
+    "RMGetFrameCounters": (None, [_P, C.POINTER(RMFrameCounters)]),
+    "RMFlush": (None, [_P]),
+    "RMGetResults": (C.c_int, [_P, C.c_int, C.c_int, _P]),
+    "RMRemoveBead": (None, [_P, C.c_int]),
+    "RMGetConfig": (None, [_P, C.POINTER(ResultManagerConfig)]),
     # include/qtrk_amd.h
     "QTrkGetLastError": (C.c_char_p, []),
     "QTrkClearLastError": (None, []),
@@ -760,3 +787,73 @@ class Pipeline:
         if self.h:
             lib().QTrkPipelineFree(self.h)
             self.h = None
+
+
+class ResultManager:
+    """The result sink after FetchResults (RM* of cputrack/qtrk_c_api.h:233-242): a native thread
+    drains the attached tracker into per-frame tables and appends finished frames to a text or
+    binary trace file."""
+
+    def __init__(self, file, frameinfo, num_beads, info_columns=(), scaling=(1, 1, 1), offset=(0, 0, 0),
+                 write_interval=1, max_frames_in_memory=0, binary=False):
+        cfg = ResultManagerConfig()
+        cfg.numBeads = num_beads
+        cfg.numFrameInfoColumns = len(info_columns)
+        cfg.scaling = Vector3f(*scaling)
+        cfg.offset = Vector3f(*offset)
+        cfg.writeInterval = write_interval
+        cfg.maxFramesInMemory = max_frames_in_memory
+        cfg.binaryOutput = 1 if binary else 0
+        L = lib()
+        L.QTrkClearLastError()
+        self.num_beads, self.ncols = num_beads, len(info_columns)
+        self.h = L.RMCreate((file or "").encode(), (frameinfo or "").encode(), C.byref(cfg),
+                            ";".join(info_columns).encode())
+        check()
+        if not self.h:
+            raise QTrkError("RMCreate returned NULL")
+
+    def _call(self, name, *args):
+        r = getattr(lib(), name)(self.h, *args)
+        check()
+        return r
+
+    def set_tracker(self, trk):
+        self._call("RMSetTracker", trk.h if trk is not None else None)
+
+    def store_frame_info(self, frame, timestamp, cols=()):
+        c = np.ascontiguousarray(cols, np.float32) if self.ncols else np.zeros(1, np.float32)
+        self._call("RMStoreFrameInfo", int(frame), float(timestamp), _ptr(c))
+
+    def counters(self):
+        fc = RMFrameCounters()
+        self._call("RMGetFrameCounters", C.byref(fc))
+        return {k: getattr(fc, k) for k, _ in RMFrameCounters._fields_}
+
+    def config(self):
+        cfg = ResultManagerConfig()
+        self._call("RMGetConfig", C.byref(cfg))
+        return cfg
+
+    def flush(self):
+        self._call("RMFlush")
+
+    def results(self, start_frame, num_frames):
+        out = np.zeros(num_frames * self.num_beads, RESULT_DTYPE)
+        n = self._call("RMGetResults", int(start_frame), int(num_frames), _ptr(out))
+        return n, out.reshape(num_frames, self.num_beads)
+
+    def bead_results(self, start, num_frames, bead):
+        out = np.zeros(max(num_frames, 0), RESULT_DTYPE)
+        n = self._call("RMGetBeadResults", int(start), int(num_frames), int(bead), _ptr(out))
+        return out[:max(n, 0)]
+
+    def remove_bead(self, bead):
+        self._call("RMRemoveBead", int(bead))
+
+    def free(self):
+        if self.h:
+            lib().RMDestroy(self.h)
+            self.h = None
+            check()
+

@@ -24,6 +24,10 @@ LAYOUT = {
     "QTrkComputedConfig": (108, {"zlut_radialsteps": 84, "zlut_angularsteps": 88, "zlut_maxradius": 92,
                                  "qi_radialsteps": 96, "qi_angstepspq": 100, "qi_maxradius": 104}),
     "ROIPosition": (8, {"x": 0, "y": 4}),
+    "ResultManagerConfig": (41, {"numBeads": 0, "numFrameInfoColumns": 4, "scaling": 8, "offset": 20,
+                                 "writeInterval": 32, "maxFramesInMemory": 36, "binaryOutput": 40}),
+    "RMFrameCounters": (28, {"startFrame": 0, "processedFrames": 4, "lastSaveFrame": 8, "capturedFrames": 12,
+                             "localizationsDone": 16, "lostFrames": 20, "fileError": 24}),
 }
 
 
@@ -84,11 +88,8 @@ def test_csharp_binding_resolves():
     imported = re.findall(r'DllImport\(DllName[^\]]*\]\s*(?:\[[^\]]*\]\s*)*public static extern [\w\*\[\]]+ (\w+)\(', cs)
     assert len(imported) > 40
     L = api.lib()
-    # RM* belong to the ResultManager (cputrack/ResultManager.cpp), the result sink after
-    # FetchResults; it is not part of this library yet.  .NET binds P/Invokes lazily, so
-    # the tracker functions work without them.
     missing = [n for n in imported if not hasattr(L, n)]
-    assert [n for n in missing if not n.startswith("RM")] == []
+    assert missing == []
 
 
 def test_default_and_computed_config_match_oracle(oracle):
