@@ -72,7 +72,9 @@ class Oracle:
             "oq_localize_batch": (None, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P]),
             "oq_queued_run": (C.c_double, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, C.c_int, P]),
             "oq_localize_trace": (None, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P, P, P]),
-            "oq_build_lut": (None, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int]),
+            "oq_build_lut": (None, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int, C.c_int]),
+            "oq_power_spectrum_2d": (None, [P, C.c_int, C.c_int]),
+            "oq_fourier_radial_profile": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int]),
             "oq_finalize_lut": (None, [P, C.c_int, C.c_int]),
             "oq_bspline_weights": (None, [P, fl]),
             "oq_generate_image_from_lut": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl,
@@ -274,7 +276,21 @@ class Oracle:
                                  _p(best), _p(prof))
         return out, best, prof
 
-    def build_lut(self, cfg, images, plane, zlut, known_pos=None, normalize=False, calib=(None, None)):
+    def power_spectrum_2d(self, img):
+        """FFT2D::Apply: |FFT2(img)|^2 with the zero frequency at (w/2, h/2)."""
+        out = np.array(img, self.fl, order="C")
+        self.L.oq_power_spectrum_2d(_p(out), out.shape[1], out.shape[0])
+        return out
+
+    def fourier_radial_profile(self, img, nbins, nang):
+        """FourierRadialProfile of a copy of img."""
+        im = np.array(img, self.fl, order="C")
+        out = np.zeros(nbins, self.fl)
+        self.L.oq_fourier_radial_profile(_p(im), im.shape[1], im.shape[0], _p(out), nbins, nang)
+        return out
+
+    def build_lut(self, cfg, images, plane, zlut, known_pos=None, normalize=False, calib=(None, None),
+                  fourier=False):
         """Adds plane `plane` of every bead's LUT from its image, in place (float32 [count][planes][steps])."""
         images = np.ascontiguousarray(images)
         pdt = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[images.dtype]
@@ -284,7 +300,7 @@ class Oracle:
         assert zlut.dtype == np.float32 and zlut.flags.c_contiguous
         self.L.oq_build_lut(C.byref(cfg), _p(images), pdt, len(images), int(plane), None if kp is None else _p(kp),
                             1 if normalize else 0, None if off is None else _p(off),
-                            None if gain is None else _p(gain), _p(zlut), zlut.shape[1])
+                            None if gain is None else _p(gain), _p(zlut), zlut.shape[1], 1 if fourier else 0)
 
     def finalize_lut(self, zlut):
         self.L.oq_finalize_lut(_p(zlut), zlut.shape[0] * zlut.shape[1], zlut.shape[2])

@@ -19,6 +19,7 @@
 #ifdef ORACLE_FP64
 #define FC(x) ((fl)(x))
 #define FSQRT sqrt
+#define FLOG log
 #define FEXP exp
 #define FCOS cos
 #define FSIN sin
@@ -29,6 +30,7 @@
 #else
 #define FC(x) ((fl)(x##f))
 #define FSQRT sqrtf
+#define FLOG logf
 #define FEXP expf
 #define FCOS cosf
 #define FSIN sinf
@@ -892,9 +894,14 @@ void oq_process_job(const OTracker* t, OWorker* wk, const void* roi, int pitch, 
 		double* sc = tr && tr->scores ? tr->scores : (double*)alloca(sizeof(double) * (t->zlut_planes > 0 ? t->zlut_planes : 1));
 		const int passes = (mode & M_ZLUTALIGN) ? 5 : 1;
 		for (int pass = 0; pass < passes; pass++) {
-			oq_radial_profile(wk->img, wk->w, wk->h, prof, res, c->zlut_angularsteps, (fl)c->zlut_minradius,
-			                  (fl)c->zlut_maxradius, px, py, wk->mean, (mode & M_NORMPROF) != 0);
-			boundary = check_bounds(wk->w, wk->h, px, py, (fl)c->zlut_maxradius);
+			if (mode & M_FOURIERLUT) {
+				/* the image itself becomes the power spectrum; the boundary flag is left as it is */
+				oq_fourier_radial_profile(wk->img, wk->w, wk->h, prof, res, c->zlut_angularsteps);
+			} else {
+				oq_radial_profile(wk->img, wk->w, wk->h, prof, res, c->zlut_angularsteps, (fl)c->zlut_minradius,
+				                  (fl)c->zlut_maxradius, px, py, wk->mean, (mode & M_NORMPROF) != 0);
+				boundary = check_bounds(wk->w, wk->h, px, py, (fl)c->zlut_maxradius);
+			}
 			if (pass > 0 && has_lut)
 				oq_quadrant_align(wk, lut, t->zlut_planes, res, (fl)c->zlut_minradius, (fl)c->zlut_maxradius,
 				                  c->qi_angstepspq, &px, &py, pz, &boundary);
@@ -969,6 +976,50 @@ void oq_localize_trace(const OConfig* cfg, int mode, const float* zluts, int zco
 	free(prof);
 	oq_worker_free(&wk);
 	free(t.qi_window);
+}
+
+/* CPUTracker::FFT2D::Apply, cputrack/cpu_tracker.cpp:920-952: kissfft of every row (length w), then
+ * of every column (length h), and the squared magnitude written back over the image with the zero
+ * frequency moved to (w/2, h/2). */
+void oq_power_spectrum_2d(fl* img, int w, int h)
+{
+	OFFT fx, fy;
+	oq_fft_init(&fx, w, 0);
+	oq_fft_init(&fy, h, 0);
+	ocpx* cbuf = (ocpx*)malloc(sizeof(ocpx) * (size_t)w * h);
+	const int m = w > h ? w : h;
+	ocpx* src = (ocpx*)malloc(sizeof(ocpx) * (size_t)m);
+	ocpx* dst = (ocpx*)malloc(sizeof(ocpx) * (size_t)m);
+	for (int y = 0; y < h; y++) {
+		for (int x = 0; x < w; x++) { src[x].re = img[y * w + x]; src[x].im = 0; }
+		oq_fft_transform(&fx, src, cbuf + (size_t)y * w);
+	}
+	for (int x = 0; x < w; x++) {
+		for (int y = 0; y < h; y++) src[y] = cbuf[(size_t)y * w + x];
+		oq_fft_transform(&fy, src, dst);
+		for (int y = 0; y < h; y++) cbuf[(size_t)y * w + x] = dst[y];
+	}
+	for (int y = 0; y < h; y++)
+		for (int x = 0; x < w; x++) {
+			const int dx = (x + w / 2) % w, dy = (y + h / 2) % h;
+			const ocpx v = cbuf[x + (size_t)y * w];
+			img[dx + (size_t)dy * w] = v.re * v.re + v.im * v.im;
+		}
+	free(cbuf);
+	free(src);
+	free(dst);
+	oq_fft_free(&fx);
+	oq_fft_free(&fy);
+}
+
+/* CPUTracker::FourierRadialProfile, cputrack/cpu_tracker.cpp:954-963: the image is replaced by its
+ * power spectrum, whose radial profile about (w/2, h/2) over radii 1 .. 0.3 w (fallback 0, not
+ * normalised) is taken in logarithm. */
+void oq_fourier_radial_profile(fl* img, int w, int h, fl* dst, int nbins, int nang)
+{
+	oq_power_spectrum_2d(img, w, h);
+	oq_radial_profile(img, w, h, dst, nbins, nang, FC(1.0), (fl)(w * FC(0.3)), (fl)(w / 2), (fl)(h / 2), 0, 0);
+	for (int i = 0; i < nbins; i++) dst[i] = FLOG(dst[i]);
 }
 
 /* (int)x as the reference's x86 build computes it: truncation toward zero, and the "integer
@@ -1077,7 +1128,8 @@ void oq_quadrant_align_img(const fl* img, int w, int h, fl mean, const float* lu
  * tracker's mean stays 0); its radial profile is added into plane `plane` of bead
  * i's LUT ([count][planes][zlut_radialsteps]). */
 void oq_build_lut(const OConfig* cfg, const void* images, int pdt, int count, int plane, const float* known_pos,
-                  int normalize, const float* calib_offset, const float* calib_gain, float* zlut, int planes)
+                  int normalize, const float* calib_offset, const float* calib_gain, float* zlut, int planes,
+                  int fourier)
 {
 	OTracker t;
 	tracker_setup(&t, cfg, 0, 0, count, planes, 0);
@@ -1109,8 +1161,11 @@ void oq_build_lut(const OConfig* cfg, const void* images, int pdt, int count, in
 			oq_qi(&wk, cx, cy, cfg->qi_iterations, cfg->qi_radialsteps, cfg->qi_angstepspq, cfg->qi_angstep_factor,
 			      (fl)cfg->qi_minradius, (fl)cfg->qi_maxradius, 0, &px, &py, &bh, 0);
 		}
-		oq_radial_profile(wk.img, wk.w, wk.h, tmp, res, cfg->zlut_angularsteps, (fl)cfg->zlut_minradius,
-		                  (fl)cfg->zlut_maxradius, px, py, wk.mean, normalize);
+		if (fourier) /* BUILDLUT_FOURIER: the position is not used, nor BUILDLUT_NORMALIZE (:630-637) */
+			oq_fourier_radial_profile(wk.img, wk.w, wk.h, tmp, res, cfg->zlut_angularsteps);
+		else
+			oq_radial_profile(wk.img, wk.w, wk.h, tmp, res, cfg->zlut_angularsteps, (fl)cfg->zlut_minradius,
+			                  (fl)cfg->zlut_maxradius, px, py, wk.mean, normalize);
 		float* dst = zlut + ((size_t)i * planes + plane) * res;
 		for (int r = 0; r < res; r++) dst[r] = (float)(dst[r] + tmp[r]);
 	}

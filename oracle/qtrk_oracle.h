@@ -145,7 +145,11 @@ double oq_queued_run(const OConfig* cfg, int mode, const float* zluts, int zcoun
 void oq_localize_trace(const OConfig* cfg, int mode, const float* zluts, int zcount, int zplanes, const float* zcmp,
                        const void* rois, int pdt, const OJob* jobs, int n, OResult* out, int* best, fl* profiles);
 void oq_build_lut(const OConfig* cfg, const void* images, int pdt, int count, int plane, const float* known_pos,
-                  int normalize, const float* calib_offset, const float* calib_gain, float* zlut, int planes);
+                  int normalize, const float* calib_offset, const float* calib_gain, float* zlut, int planes,
+                  int fourier);
+/* FFT2D::Apply and FourierRadialProfile (cputrack/cpu_tracker.cpp:920-963), in place on img */
+void oq_power_spectrum_2d(fl* img, int w, int h);
+void oq_fourier_radial_profile(fl* img, int w, int h, fl* dst, int nbins, int nang);
 void oq_finalize_lut(float* zlut, int rows, int res);
 
 /* Image model and noise-free image generators (cputrack/utils.cpp:325-402, CubicBSpline.h:15-24). */

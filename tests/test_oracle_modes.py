@@ -236,3 +236,53 @@ def test_zbias_table_composition_and_round_trip(oracle):
         errs.append(abs(oracle.zbias_apply(table[0], planes, float(rq["z"][0])) - zq))
     print("corrected |z - truth|", np.round(errs, 5))
     assert max(errs[0::2]) < 1e-3 and max(errs[1::2]) < 0.02
+
+
+# ------------------------------------------------------------ LT_FourierLUT
+
+@pytest.mark.parametrize("w,h", [(64, 64), (100, 100), (96, 96), (62, 58)])
+def test_power_spectrum_matches_numpy(oracle, w, h):
+    """FFT2D::Apply (cputrack/cpu_tracker.cpp:920-952) is |FFT2|^2 with the zero frequency at (w/2, h/2):
+    numpy's fftshift for even and odd sizes."""
+    rng = np.random.default_rng(w * h)
+    img = rng.uniform(0, 100, size=(h, w)).astype(np.float32)
+    ps = oracle.power_spectrum_2d(img)
+    ref = np.fft.fftshift(np.abs(np.fft.fft2(img.astype(np.float64))) ** 2)
+    np.testing.assert_allclose(ps, ref, rtol=1e-4, atol=1e-5 * ref.max())
+    assert ps[h // 2, w // 2] == np.float32(img.astype(np.float64).sum() ** 2) or \
+        abs(ps[h // 2, w // 2] / img.astype(np.float64).sum() ** 2 - 1) < 1e-6  # DC term
+
+
+def test_fourier_profile_composition_and_shift_invariance(oracle):
+    """FourierRadialProfile (:954-963): log of the radial profile of the power spectrum about (w/2, h/2),
+    radii 1 .. 0.3 w, fallback 0, no normalisation; a circular shift of the image leaves it unchanged."""
+    lut = synth.analytic_lut(60, 40, 0.0, 30.0, 8.0)
+    img = (500 + 1000 * oracle.generate_image_from_lut(64, 64, lut, 0.0, 30.0, 31.6, 32.3, 25.0)).astype(np.float32)
+    prof = oracle.fourier_radial_profile(img, 40, 120)
+    ps = oracle.power_spectrum_2d(img)
+    exp = np.log(oracle.radial_profile(ps, 40, 120, 1.0, np.float32(64 * np.float32(0.3)), 32.0, 32.0, 0.0, False))
+    np.testing.assert_allclose(prof, exp, rtol=1e-6)
+    rolled = oracle.fourier_radial_profile(np.roll(img, (5, -7), axis=(0, 1)), 40, 120)
+    np.testing.assert_allclose(rolled, prof, atol=2e-4)
+
+
+def test_fourier_lut_localizes_z_independent_of_position(oracle):
+    """BUILDLUT_FOURIER (cputrack/QueuedCPUTracker.cpp:630-637) + LT_FourierLUT: a Fourier LUT built from
+    centred beads gives the same plane for beads moved by whole pixels (the power spectrum ignores the
+    phase; a sub-pixel move resamples the bead and shifts z by up to a quarter plane)."""
+    cfg = oracle.config(width=64, height=64, zlut_minradius=0.0, zlut_roi_coverage=30.0 / 32, zlut_radial_coverage=1.0)
+    res, planes = cfg.zlut_radialsteps, 60
+    lut = synth.analytic_lut(planes, 40, 0.0, 30.0, 8.0)
+    flut = np.zeros((1, planes, res), np.float32)
+    for p in range(planes):
+        img = (500 + 1000 * oracle.generate_image_from_lut(64, 64, lut, 0.0, 30.0, 32.0, 32.0, float(p)))
+        oracle.build_lut(cfg, img[None].astype(np.float32), p, flut, fourier=True)
+    jobs = np.zeros(4, dtype=[("frame", "<u4"), ("timestamp", "<u4"), ("zlutIndex", "<i4"),
+                              ("ix", "<f4"), ("iy", "<f4"), ("iz", "<f4")])
+    truth = [(32.0, 32.0, 20.3), (33.0, 31.0, 20.3), (32.0, 32.0, 41.7), (30.0, 34.0, 41.7)]
+    rois = np.stack([(500 + 1000 * oracle.generate_image_from_lut(64, 64, lut, 0.0, 30.0, *t)).astype(np.float32)
+                     for t in truth])
+    r = oracle.localize(cfg, LT_QI | LT_LOCALIZEZ | 256, flut, rois, jobs)
+    print("Fourier-LUT z", r["z"])
+    assert abs(r["z"][0] - 20.3) < 0.05 and abs(r["z"][2] - 41.7) < 0.05
+    assert abs(r["z"][1] - r["z"][0]) < 2e-3 and abs(r["z"][3] - r["z"][2]) < 2e-3
