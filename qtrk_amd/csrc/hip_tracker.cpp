@@ -207,6 +207,10 @@ void QueuedHIPTracker::BuildTables()
 	// radial steps, built when the sizes fit; SetLocalizationMode rejects the mode otherwise
 	if (cfg.xc1_profileLength >= 2 && cfg.xc1_profileLength <= kMaxFFTLen) xc_fft.build(cfg.xc1_profileLength);
 	if (cfg.zlut_radialsteps >= 1 && 2 * cfg.zlut_radialsteps <= kMaxFFTLen) qa_fft.build(2 * cfg.zlut_radialsteps);
+	if (cfg.width <= kMaxFFTLen && cfg.height <= kMaxFFTLen) {
+		fx_fft.build(cfg.width);
+		fy_fft.build(cfg.height);
+	}
 	// QuadrantAlign's quadrant profiles take all qi_angstepspq directions
 	// (cputrack/cpu_tracker.cpp:476, 642-657): at least 4 samples, stride trunc(A / samples)
 	qa_steps = A < MIN_RADPROFILE_SMP_COUNT ? MIN_RADPROFILE_SMP_COUNT : A;
@@ -303,6 +307,9 @@ void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void*
 	P.qa_steps = qa_steps;
 	P.qa_stride = qa_stride;
 	P.qa_fft = qa_fft.plan();
+	P.fx_fft = fx_fft.plan();
+	P.fy_fft = fy_fft.plan();
+	P.fourier = 0;
 	P.z_nb = cfg.zlut_radialsteps;
 	P.z_A = cfg.zlut_angularsteps;
 	P.z_minr = cfg.zlut_minradius;
@@ -686,7 +693,7 @@ void QueuedHIPTracker::ClearResults()
 void QueuedHIPTracker::SetLocalizationMode(LocMode_t mode)
 {
 	const int supported = LT_QI | LT_Gaussian2D | LT_LocalizeZ | LT_NormalizeProfile | LT_ClearFirstFourPixels |
-	                      LT_XCor1D | LT_ZLUTAlign | LT_LocalizeZWeighted;
+	                      LT_XCor1D | LT_ZLUTAlign | LT_LocalizeZWeighted | LT_FourierLUT;
 	if (mode & ~supported) {
 		std::ostringstream os;
 		os << "localization mode bits 0x" << std::hex << (mode & ~supported) << " are not implemented";
@@ -698,6 +705,8 @@ void QueuedHIPTracker::SetLocalizationMode(LocMode_t mode)
 		fail("LT_XCor1D needs xc1_iterations >= 0 and xc1_profileWidth >= 0");
 	if ((mode & LT_ZLUTAlign) && (mode & LT_LocalizeZ) && qa_fft.n == 0)
 		fail("LT_ZLUTAlign needs zlut_radialsteps in 1.." + std::to_string(kMaxFFTLen / 2));
+	if ((mode & LT_FourierLUT) && (mode & LT_LocalizeZ) && fx_fft.n == 0)
+		fail("LT_FourierLUT needs a ROI of at most " + std::to_string(kMaxFFTLen) + " pixels a side");
 	{
 		LocalizeParams P{};
 		FillParams(P, 2, 1, nullptr, nullptr, 0, nullptr, nullptr);
@@ -824,7 +833,9 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
 	WaitAllLocked(lk);
-	if (buildFlags & (BUILDLUT_IMAGELUT | BUILDLUT_FOURIER)) fail("BUILDLUT_IMAGELUT / BUILDLUT_FOURIER are not implemented");
+	if (buildFlags & BUILDLUT_IMAGELUT) fail("BUILDLUT_IMAGELUT is not implemented");
+	if ((buildFlags & BUILDLUT_FOURIER) && fx_fft.n == 0)
+		fail("BUILDLUT_FOURIER needs a ROI of at most " + std::to_string(kMaxFFTLen) + " pixels a side");
 	if (zcount <= 0) fail("BuildLUT needs a LUT allocated with SetRadialZLUT");
 	if (plane < 0 || plane >= zplanes) fail("BuildLUT plane out of range");
 	const int bpp = PDT_BytesPerPixel(pdt);
@@ -844,6 +855,9 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 	P.build_lut = d_zlut.as<float>();
 	P.zcmp = nullptr;
 	P.cmp_out = nullptr;
+	// BUILDLUT_FOURIER: the profile of each image's power spectrum (cputrack/QueuedCPUTracker.cpp:630-637)
+	P.fourier = (buildFlags & BUILDLUT_FOURIER) ? 1 : 0;
+	if (localize_lds_bytes(P) > lds_limit) fail("BUILDLUT_FOURIER: the ROI's spectrum does not fit the LDS");
 	if (known_pos) {
 		upload(d_build_pos, known_pos, sizeof(vector2f) * zcount);
 		P.known_pos = d_build_pos.as<float>();

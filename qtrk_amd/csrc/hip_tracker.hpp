@@ -119,6 +119,7 @@ private:
 	HostFFT qi_fft;   // length 2*qi_radialsteps
 	HostFFT xc_fft;   // length xc1_profileLength (LT_XCor1D), when 2..kMaxFFTLen
 	HostFFT qa_fft;   // length 2*zlut_radialsteps (LT_ZLUTAlign), when <= kMaxFFTLen
+	HostFFT fx_fft, fy_fft;  // ROI width and height (LT_FourierLUT, BUILDLUT_FOURIER), when <= kMaxFFTLen
 	int qa_steps = 0, qa_stride = 0;
 
 	// radial Z-LUT [count][planes][zlut_radialsteps], weights, compare profiles

@@ -24,6 +24,7 @@
 #include "peakfit.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace qtrk {
 
@@ -37,6 +38,7 @@ struct LdsLayout {
 	int xprof, xfft, xtw, xperm, xtmp;            // XCor1D phase
 	int zdir, zpart, zcnt;                        // radial-profile phase
 	int adir, aprof, alut, afft, atw, aperm, atmp;  // QuadrantAlign phase
+	int fbuf, fcol, ftmp, ftwx, ftwy, fpx, fpy;     // 2D power-spectrum phase
 	int scores, zw;                               // LUT phase (zw: adjusted weights)
 	int total;
 };
@@ -51,9 +53,13 @@ __host__ __device__ inline bool mode_zweighted(const LocalizeParams& P)
 {
 	return !P.build && (P.mode & kModeLocalizeZWeighted) && (P.mode & kModeLocalizeZ);
 }
+__host__ __device__ inline bool mode_fourier(const LocalizeParams& P)
+{
+	return P.build ? P.fourier != 0 : ((P.mode & kModeFourierLUT) && (P.mode & kModeLocalizeZ));
+}
 __host__ __device__ inline bool mode_ext(const LocalizeParams& P)
 {
-	return mode_xcor(P) || mode_align(P) || mode_zweighted(P);
+	return mode_xcor(P) || mode_align(P) || mode_zweighted(P) || mode_fourier(P);
 }
 
 // Bytes per ROI pixel as staged in LDS: the source format when it converts to
@@ -115,6 +121,19 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 		L.aperm = a;  a += align16(Na * 4);
 		L.atmp = a;   a += align16(3 * Na * 8);
 	}
+	// 2D power-spectrum phase: the complex ROI, per-wavefront column buffers and generic-radix
+	// staging, the row and column twiddles and leaf orders
+	int f = u;
+	if (ext && mode_fourier(P)) {
+		const int W = P.width, H = P.height, M = W > H ? W : H;
+		L.fbuf = f;   f += align16(W * H * 8);
+		L.fcol = f;   f += align16(4 * H * 8);
+		L.ftmp = f;   f += align16(4 * M * 8);
+		L.ftwx = f;   f += align16(W * 8);
+		L.ftwy = f;   f += align16(H * 8);
+		L.fpx = f;    f += align16(W * 4);
+		L.fpy = f;    f += align16(H * 4);
+	}
 	// LUT phase
 	L.scores = u;
 	int s = u + align16(P.zplanes * 8);
@@ -124,6 +143,7 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 	o = o > s ? o : s;
 	o = o > x ? o : x;
 	o = o > a ? o : a;
+	o = o > f ? o : f;
 	L.total = o;
 	return L;
 }
@@ -632,8 +652,63 @@ __device__ __forceinline__ void quadrant_align_phase(const LocalizeParams& P, co
 	__syncthreads();
 }
 
-// EXT compiles in the optional modes (LT_XCor1D, LT_ZLUTAlign,
-// LT_LocalizeZWeighted); the variant without them is the one the common modes run.
+// FFT2D::Apply (cputrack/cpu_tracker.cpp:920-952) on the ROI in LDS: kissfft of every row
+// (four rows at a time, one per wavefront, written in leaf order straight into the complex
+// buffer), then of every column (gathered into per-wavefront buffers in leaf order and
+// scattered back), then the squared magnitude written over the ROI with the zero frequency
+// moved to (W/2, H/2).
+__device__ __forceinline__ void power_spectrum_phase(const LocalizeParams& P, const LdsLayout& L, unsigned char* lds,
+                                                     float* img, int tid, int lane, int wid)
+{
+	const int W = P.width, H = P.height;
+	float2* cb = (float2*)(lds + L.fbuf);
+	float2* col = (float2*)(lds + L.fcol);
+	float2* tmp = (float2*)(lds + L.ftmp);
+	float2* twx = (float2*)(lds + L.ftwx);
+	float2* twy = (float2*)(lds + L.ftwy);
+	int* px = (int*)(lds + L.fpx);
+	int* py = (int*)(lds + L.fpy);
+	for (int i = tid; i < W; i += kBlock) {
+		twx[i] = P.fx_fft.tw_fwd[i];
+		px[i] = P.fx_fft.perm[i];
+	}
+	for (int i = tid; i < H; i += kBlock) {
+		twy[i] = P.fy_fft.tw_fwd[i];
+		py[i] = P.fy_fft.perm[i];
+	}
+	__syncthreads();
+	for (int y0 = 0; y0 < H; y0 += 4) {
+		const int nr = H - y0 < 4 ? H - y0 : 4;
+		if (wid < nr) {
+			float2* F = cb + (y0 + wid) * W;
+			const float* src = img + (y0 + wid) * W;
+			for (int j = lane; j < W; j += 64) F[j] = make_float2(src[px[j]], 0.0f);
+		}
+		__syncthreads();
+		fft_stages<true>(cb + y0 * W, tmp, nr, P.fx_fft, twx, 1.0f, wid, lane);
+	}
+	for (int x0 = 0; x0 < W; x0 += 4) {
+		const int nc = W - x0 < 4 ? W - x0 : 4;
+		if (wid < nc) {
+			float2* F = col + wid * H;
+			for (int j = lane; j < H; j += 64) F[j] = cb[py[j] * W + x0 + wid];
+		}
+		__syncthreads();
+		fft_stages<true>(col, tmp, nc, P.fy_fft, twy, 1.0f, wid, lane);
+		if (wid < nc)
+			for (int y = lane; y < H; y += 64) cb[y * W + x0 + wid] = col[wid * H + y];
+		__syncthreads();
+	}
+	for (int i = tid; i < W * H; i += kBlock) {
+		const int y = i / W, x = i - y * W;
+		const float2 v = cb[i];
+		img[(x + W / 2) % W + ((y + H / 2) % H) * W] = v.x * v.x + v.y * v.y;
+	}
+	__syncthreads();
+}
+
+// EXT compiles in the optional modes (LT_XCor1D, LT_ZLUTAlign, LT_LocalizeZWeighted,
+// LT_FourierLUT); the variant without them is the one the common modes run.
 template <typename SPix, bool EXT>
 __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 {
@@ -916,12 +991,22 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	const bool lz = (P.mode & kModeLocalizeZ) || build;
 	if (lz && P.z_nb > 0) {
 		const int nb = P.z_nb, nch = z_chunks(P);
-		const float rstep = (P.z_maxr - P.z_minr) / (float)nb;
+		// LT_FourierLUT / BUILDLUT_FOURIER: the profile of the ROI's power spectrum about (W/2, H/2)
+		// over radii 1 .. 0.3 W, fallback 0, in logarithm (cputrack/cpu_tracker.cpp:954-963)
+		const bool fourier = EXT && mode_fourier(P);
+		const float pminr = fourier ? 1.0f : P.z_minr;
+		const float pmaxr = fourier ? (float)W * 0.3f : P.z_maxr;
+		const float rstep = (pmaxr - pminr) / (float)nb;
 		const bool has_lut = !build && P.zluts && bead >= 0 && bead < P.zcount && P.zplanes > 0;
 		const float* L0 = has_lut ? P.zluts + (size_t)bead * P.zplanes * nb : nullptr;
 		const int passes = (EXT && mode_align(P) && has_lut) ? 5 : 1;
 		for (int pass = 0; pass < passes; pass++) {
-			const bool ok = !isnan(cx) && !isnan(cy);
+			if constexpr (EXT && std::is_same<SPix, float>::value) {
+				if (fourier) power_spectrum_phase(P, L, lds, img, tid, lane, wid);
+			}
+			const float pcx = fourier ? (float)(W / 2) : cx, pcy = fourier ? (float)(H / 2) : cy;
+			const float pfall = fourier ? 0.0f : mean;
+			const bool ok = !isnan(pcx) && !isnan(pcy);
 			for (int j = tid; j < P.z_A; j += kBlock) zdir[j] = P.z_dirs[j];
 			__syncthreads();
 			for (int base = 0; base < nch * nb; base += kBlock) {
@@ -929,7 +1014,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				if (p < nch * nb) {
 					const int ch = p / nb, bin = p - ch * nb;
 					const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
-					const float r = P.z_minr + rstep * (float)bin;
+					const float r = pminr + rstep * (float)bin;
 					double s = 0.0;
 					int c = 0;
 					int a = a0;
@@ -938,14 +1023,14 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 #pragma unroll
 						for (int u = 0; u < 4; u++) {
 							const float2 d = zdir[a + u];
-							x[u] = cx + d.x * r;
-							y[u] = cy + d.y * r;
+							x[u] = pcx + d.x * r;
+							y[u] = pcy + d.y * r;
 						}
 						sample_n_acc<SPix, double, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
 					}
 					for (; a < a1; a++) {
 						const float2 d = zdir[a];
-						const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+						const float x[1] = { pcx + d.x * r }, y[1] = { pcy + d.y * r };
 						sample_n_acc<SPix, double, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
 					}
 					zpart[p] = s;
@@ -960,13 +1045,16 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					s += zpart[ch * nb + bin];
 					c += zcnt[ch * nb + bin];
 				}
-				zprof[bin] = c > 4 ? (float)(s / (double)c) : mean;
+				const float v = c > 4 ? (float)(s / (double)c) : pfall;
+				zprof[bin] = fourier ? logf(v) : v;
 			}
 			__syncthreads();
-			if ((P.mode & kModeNormalizeProfile) && tid == 0) normalize_profile(zprof, nb);
-			__syncthreads();
-			boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
-			           (cy - P.z_maxr < 0.0f);
+			if (!fourier) {
+				if ((P.mode & kModeNormalizeProfile) && tid == 0) normalize_profile(zprof, nb);
+				__syncthreads();
+				boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
+				           (cy - P.z_maxr < 0.0f);
+			}
 			if (P.prof_out && !build)
 				for (int bin = tid; bin < nb; bin += kBlock) P.prof_out[(size_t)roi * nb + bin] = zprof[bin];
 			if constexpr (EXT) {
@@ -1076,6 +1164,8 @@ hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
 	const bool ext = mode_ext(p);
 	if (ext && mode_xcor(p) && (p.xc_fft.n < 2 || p.xc_fft.n > kMaxFFTLen)) return hipErrorInvalidValue;
 	if (ext && mode_align(p) && (p.qa_fft.n != 2 * p.z_nb || p.qa_fft.n > kMaxFFTLen)) return hipErrorInvalidValue;
+	if (ext && mode_fourier(p) && (p.lds_pix != 2 || p.fx_fft.n != p.width || p.fy_fft.n != p.height))
+		return hipErrorInvalidValue;
 	if (p.lds_pix == 0) ext ? launch_variant<uint8_t, true>(p, lds, stream) : launch_variant<uint8_t, false>(p, lds, stream);
 	else if (p.lds_pix == 1) ext ? launch_variant<uint16_t, true>(p, lds, stream) : launch_variant<uint16_t, false>(p, lds, stream);
 	else ext ? launch_variant<float, true>(p, lds, stream) : launch_variant<float, false>(p, lds, stream);

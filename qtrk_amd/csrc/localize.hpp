@@ -92,6 +92,11 @@ struct LocalizeParams {
 	int qa_steps, qa_stride;
 	FFTPlan qa_fft;
 
+	// LT_FourierLUT (cputrack/cpu_tracker.cpp:920-963): transforms of the ROI's rows (length
+	// width) and columns (length height); `fourier` selects BUILDLUT_FOURIER in the build variant
+	FFTPlan fx_fft, fy_fft;
+	int fourier;
+
 	// radial profile + Z-LUT
 	int z_nb, z_A;
 	float z_minr, z_maxr;

@@ -66,9 +66,9 @@ int Pipeline::Add(QueuedHIPTracker* trk, int n, int pdt, int precision, const fl
 	if (pdt < 0 || pdt > 2) fail("unknown pixel type");
 	if (precision < kMatchFP32 || precision > kMatchFP16) fail("unknown match precision");
 	if (!trk->DeviceZLUT() || trk->ZPlanes() <= 0) fail("the tracker has no radial Z-LUT");
-	if (trk->LocalizeMode() & (LT_ZLUTAlign | LT_LocalizeZWeighted))
-		fail("LT_ZLUTAlign and LT_LocalizeZWeighted need the per-ROI compare: the pipeline's grouped match "
-		     "scores one profile per ROI");
+	if (trk->LocalizeMode() & (LT_ZLUTAlign | LT_LocalizeZWeighted | LT_FourierLUT))
+		fail("LT_ZLUTAlign, LT_LocalizeZWeighted and LT_FourierLUT need the per-ROI compare: the pipeline's "
+		     "grouped match scores one normalised radial profile per ROI");
 	if (iters < 0) fail("mleIterations must be >= 0");
 	release_graph();
 	std::unique_ptr<PipelineTracker> t(new PipelineTracker());
@@ -172,7 +172,7 @@ void Pipeline::enqueue_localize(PipelineTracker& t, hipStream_t s)
 	t.trk->FillParams(P, t.pdt, t.n, t.rois.p, nullptr, t.width * pdt_bytes(t.pdt), t.jobs.as<DevJob>(),
 	                  t.results.as<DevResult>());
 	P.mode |= kModeLocalizeZ | kModeNormalizeProfile;
-	P.mode &= ~(kModeZLUTAlign | kModeLocalizeZWeighted);
+	P.mode &= ~(kModeZLUTAlign | kModeLocalizeZWeighted | kModeFourierLUT);
 	P.zluts = nullptr;  // profiles only: the grouped match scores them
 	P.prof_out = t.prof.as<float>();
 	P.best_out = nullptr;

@@ -111,7 +111,7 @@ def test_optional_mode_validation(gpu):
         with pytest.raises(api.QTrkError, match="xc1_profileLength"):
             trk.set_localization_mode(api.LT_XCor1D)
         with pytest.raises(api.QTrkError, match="not implemented"):
-            trk.set_localization_mode(api.LT_QI | api.LT_FourierLUT)
+            trk.set_localization_mode(api.LT_QI | 32)  # a bit LocalizeModeEnum does not define
         trk.set_localization_mode(api.LT_QI | api.LT_ZLUTAlign | Z)
     finally:
         trk.free()
@@ -203,4 +203,68 @@ def test_zbias_correction_applied_to_results(gpu, oracle):
         trk.set_radial_zlut(w["luts"])
         assert trk.get_zlut_bias_table() is None
     finally:
+        trk.free()
+
+
+# ------------------------------------------------------------ LT_FourierLUT
+
+@pytest.mark.parametrize("size", [64, 100, 96], ids=["64px", "100px", "96px"])
+def test_fourier_lut_build_and_localize(gpu, oracle, size):
+    """BUILDLUT_FOURIER and LT_FourierLUT (cputrack/QueuedCPUTracker.cpp:324-326, 630-637;
+    cputrack/cpu_tracker.cpp:920-963): the 2D power spectrum is kissfft's in rows and columns
+    (lengths 64 = 4^3, 100 = 4*5*5, 96 = 4*4*2*3), its profile is taken in log.  The device logf and
+    glibc's may differ in the last place, so profiles and z are compared within tolerance."""
+    w = synth.make_workload("plain64", n=16, width=size, height=size)
+    cfg = w["cfg"]
+    o_cfg = oracle.config_from_bytes(bytes(cfg))
+    planes, count = 40, 2
+    lut_src = w["render_luts"][0]
+    trk = api.QTrkInstance(w["settings"])
+    try:
+        trk.set_radial_zlut((count, planes))
+        trk.begin_lut(api.BUILDLUT_FOURIER | api.BUILDLUT_NORMALIZE)  # the Fourier profile ignores NORMALIZE
+        ref = np.zeros((count, planes, cfg.zlut_radialsteps), np.float32)
+        c = size / 2
+        for p in range(planes):
+            posn = np.stack([np.full(count, c), np.full(count, c), np.full(count, 2.0 + 2.4 * p)], 1).astype(np.float32)
+            imgs = api.generate_images_from_lut(lut_src[None], posn, np.zeros(count, np.int32), size, size,
+                                                cfg.zlut_minradius, cfg.zlut_maxradius)
+            imgs = (1000 + 800 * imgs).astype(np.float32)
+            trk.build_lut(imgs, p)
+            oracle.build_lut(o_cfg, imgs, p, ref, fourier=True)
+        got = trk.get_radial_zlut()
+        d = np.abs(got - ref)
+        print(f"[fourier {size}] LUT identical {(d == 0).mean():.3f}, max|d| {d.max():.2e}", flush=True)
+        assert d.max() <= 1e-5 * max(1.0, np.abs(ref).max())
+        # localize with the oracle's LUT in both, so only the localization is compared
+        trk.set_radial_zlut(ref)
+        mode = api.LT_QI | api.LT_LocalizeZ | api.LT_FourierLUT
+        trk.set_localization_mode(mode)
+        jobs = w["jobs"].copy()
+        jobs["zlutIndex"] = np.arange(len(jobs)) % count
+        rois = w["rois"]
+        got_r, best, prof = _batch(trk, rois, jobs)
+        ref_r, rbest, rprof = _oracle(oracle, dict(w, luts=ref), mode=mode, jobs=jobs)
+        _report(f"fourier {size}", got_r, ref_r)
+        _assert_parity(got_r, ref_r)
+        np.testing.assert_allclose(prof, rprof, rtol=1e-5, atol=1e-5)
+        assert np.mean(best == rbest) >= 0.9  # a last-place logf difference can tip a near tie
+    finally:
+        trk.free()
+
+
+def test_fourier_lut_validation(gpu):
+    w = synth.make_workload("plain64", n=2)
+    trk = api.QTrkInstance(w["settings"])
+    p = api.Pipeline()
+    try:
+        trk.set_radial_zlut(w["luts"])
+        trk.set_localization_mode(api.LT_QI | api.LT_LocalizeZ | api.LT_FourierLUT)
+        with pytest.raises(api.QTrkError, match="LT_FourierLUT"):
+            p.add(trk, 2, api.QTrkU16)
+        trk.begin_lut(api.BUILDLUT_IMAGELUT)
+        with pytest.raises(api.QTrkError, match="BUILDLUT_IMAGELUT"):
+            trk.build_lut(w["rois"][:1], 0)
+    finally:
+        p.free()
         trk.free()

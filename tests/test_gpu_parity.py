@@ -396,8 +396,9 @@ def test_large_roi_default_settings(gpu, oracle):
 def test_unsupported_mode_raises(gpu):
     w = synth.make_workload("plain64", n=2)
     trk = api.QTrkInstance(w["settings"])
-    with pytest.raises(api.QTrkError):
-        trk.set_localization_mode(api.LT_FourierLUT | api.LT_QI)
+    for bits in (32, 1024, 1 << 20):  # bits LocalizeModeEnum does not define (cputrack/qtrk_c_api.h:10-24)
+        with pytest.raises(api.QTrkError, match="not implemented"):
+            trk.set_localization_mode(bits | api.LT_QI)
     trk.free()
 
 
