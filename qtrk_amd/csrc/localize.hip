@@ -206,6 +206,45 @@ __device__ __forceinline__ void sample_n_acc(const T* img, int W, float Wf, floa
 	}
 }
 
+// The same samples when every one is known to be inside (the caller checked the circle
+// against the ROI): no range test, no masking; the sum runs in the same order.
+template <typename T, typename Acc, int U>
+__device__ __forceinline__ void sample_n_inside(const T* img, int W, const float (&x)[U], const float (&y)[U], Acc& sum)
+{
+	int rx[U], ry[U];
+	float v00[U], v10[U], v01[U], v11[U];
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		rx[u] = trunc_sat(x[u]);
+		ry[u] = trunc_sat(y[u]);
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const T* p = img + __mul24(ry[u], W) + rx[u];
+		v00[u] = (float)p[0];
+		v10[u] = (float)p[1];
+		v01[u] = (float)p[W];
+		v11[u] = (float)p[W + 1];
+	}
+#pragma unroll
+	for (int u = 0; u < U; u++) {
+		const float fx = x[u] - (float)rx[u];
+		const float a0 = v00[u] + (v10[u] - v00[u]) * fx;
+		const float a1 = v01[u] + (v11[u] - v01[u]) * fx;
+		sum += (Acc)(a0 + (a1 - a0) * (y[u] - (float)ry[u]));
+	}
+}
+
+// True when every sample cx + d.x r, cy + d.y r (|d.x|, |d.y| <= 1, float arithmetic) of a
+// circle of radius |r| truncates into [0, W-2] x [0, H-2]: |fl(d r)| <= |r| and rounding is
+// monotonic, so the extremes are fl(c -+ |r|), kept at least 1 px from the low edges and
+// within W-2, H-2 at the high ones.  False for a NaN centre or radius.
+__device__ __forceinline__ bool circle_inside(float cx, float cy, float r, float Wf, float Hf)
+{
+	const float a = fabsf(r);
+	return cx - a >= 1.0f && cx + a <= Wf - 2.0f && cy - a >= 1.0f && cy + a <= Hf - 2.0f;
+}
+
 __device__ __forceinline__ void wave_argmax(float& v, int& i)
 {
 	for (int off = 32; off > 0; off >>= 1) {
@@ -340,6 +379,58 @@ __device__ __forceinline__ void fft_stages(float2* base, float2* tmp, int nbufs,
 				for (int t = lane; t < N; t += 64) base[wid * N + t] = tmp[wid * N + t];
 			__syncthreads();
 		}
+	}
+}
+
+// The same stages for QI's power-of-two lengths (radices 4 and 2 only): the group and
+// twiddle indices are shifts and masks, and a wavefront takes as many transforms as give a
+// radix-4 stage 64 butterflies (2 of length 128, 4 of length 64), so the lanes are busy.
+// The butterflies, and so the rounding, are those of fft_stages.
+__device__ __forceinline__ void fft_pow2_stages(float2* base, int nbufs, const FFTPlan& F, const float2* tw,
+                                                float nii, int wid, int lane)
+{
+	const int ln = __builtin_ctz(F.n);
+	const int lt = ln < 8 ? 8 - ln : 0;  // log2 transforms per wavefront
+	const int t0 = wid << lt;
+#pragma unroll
+	for (int si = 0; si < kMaxFFTStages; si++) {
+		const int s = kMaxFFTStages - 1 - si;
+		if (s >= F.nst) continue;  // uniform: every thread skips together
+		const int r = F.radix[s];
+		const int lr = r == 4 ? 2 : 1, lm = __builtin_ctz(F.m[s]), lf = __builtin_ctz(F.fs[s]);
+		const int lb = ln - lr;  // log2 butterflies per transform
+		if (t0 < nbufs) {
+			const int nt = nbufs - t0 < (1 << lt) ? nbufs - t0 : (1 << lt);
+			const int total = nt << lb;
+			for (int e = lane; e < total; e += 64) {
+				const int t = e >> lb, b = e & ((1 << lb) - 1);
+				const int g = b >> lm, k = b & ((1 << lm) - 1);
+				const int m = 1 << lm;
+				float2* G = base + ((t0 + t) << ln) + (g << (lm + lr));
+				const int k1 = k << lf;
+				if (r == 4) {
+					const float2 s0 = cmul(G[k + m], tw[k1]);
+					const float2 s1 = cmul(G[k + 2 * m], tw[k1 << 1]);
+					const float2 s2 = cmul(G[k + 3 * m], tw[k1 + (k1 << 1)]);
+					float2 f0 = G[k];
+					const float2 s5 = csub(f0, s1);
+					f0 = cadd(f0, s1);
+					const float2 s3 = cadd(s0, s2);
+					float2 s4 = csub(s0, s2);
+					s4 = make_float2(s4.y * nii, -s4.x * nii);
+					G[k + 2 * m] = csub(f0, s3);
+					G[k] = cadd(f0, s3);
+					G[k + m] = cadd(s5, s4);
+					G[k + 3 * m] = csub(s5, s4);
+				} else {
+					const float2 tt = cmul(G[m + k], tw[k1]);
+					const float2 f0 = G[k];
+					G[m + k] = csub(f0, tt);
+					G[k] = cadd(f0, tt);
+				}
+			}
+		}
+		__syncthreads();
 	}
 }
 
@@ -824,11 +915,13 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			tw[N + i] = P.qi_fft.tw_inv[i];
 			perm[i] = P.qi_fft.perm[i];
 		}
-		// direction table per quadrant with the quadrant's signs applied: (mx*dx, my*dy),
-		// exact sign flips of the reference's mx*quadrantDirs[i].x (cputrack/cpu_tracker.cpp:658-659)
+		// direction table angle-major, the 4 quadrants' entries of an angle adjacent (a wavefront's
+		// 4 quadrants read 4 neighbouring words, in different banks), each with the quadrant's
+		// signs applied: (mx*dx, my*dy), exact sign flips of the reference's mx*quadrantDirs[i].x
+		// (cputrack/cpu_tracker.cpp:658-659)
 		for (int j = tid; j < 4 * P.qi_A; j += kBlock) {
-			const int q = j / P.qi_A;
-			const float2 d = P.qi_dirs[j - q * P.qi_A];
+			const int a = j >> 2, q = j & 3;
+			const float2 d = P.qi_dirs[a];
 			const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
 			const float sy = (q < 2) ? 1.0f : -1.0f;
 			qdir[j] = make_float2(sx * d.x, sy * d.y);
@@ -840,33 +933,69 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			const bool ok = !isnan(cx) && !isnan(cy);
 			// uniform loop over (quadrant, radius) slots: the angle index is the same in
 			// every lane, so the direction reads from LDS are broadcasts
+			// (quadrant, radius) slots in segments of 32 consecutive radii of one quadrant: each
+			// half-wavefront samples along one ray (the LDS serves a half-wavefront at a time, and
+			// neighbouring radii read neighbouring pixels), and a wavefront's two segments share
+			// their radius band, so wavefronts of inner radii, whose circles lie inside the ROI,
+			// take the unmasked loop
 			for (int base = 0; base < 4 * nr; base += kBlock) {
 				const int p = base + tid;
-				if (p < 4 * nr) {
-					const int q = p / nr, ri = p - q * nr;
-					const float2* qd = qdir + q * P.qi_A;
-					const float r = minr + rstep * (float)ri;
+				int q, ri;
+				if ((nr & 31) == 0) {
+					const int sg = p >> 5;
+					q = sg & 3;
+					ri = ((sg >> 2) << 5) + (p & 31);
+				} else {
+					q = p / nr;
+					ri = p - q * nr;
+				}
+				const float r = minr + rstep * (float)ri;
+				const bool valid = p < 4 * nr;
+				const bool clear = !valid || (ok && circle_inside(cx, cy, r, Wf, Hf));
+				const bool fast = __ballot(!clear) == 0;  // uniform over the wavefront
+				if (valid) {
+					const float2* qd = qdir + q;
+					const int st4 = 4 * stride;
 					float sum = 0.0f;
 					int cnt = 0;
 					int a = 0;
-					for (; a + 4 <= steps; a += 4) {
-						float x[4], y[4];
+					if (fast) {
+						for (; a + 4 <= steps; a += 4) {
+							float x[4], y[4];
 #pragma unroll
-						for (int u = 0; u < 4; u++) {
-							const float2 d = qd[stride * (a + u)];
-							x[u] = cx + d.x * r;
-							y[u] = cy + d.y * r;
+							for (int u = 0; u < 4; u++) {
+								const float2 d = qd[st4 * (a + u)];
+								x[u] = cx + d.x * r;
+								y[u] = cy + d.y * r;
+							}
+							sample_n_inside<SPix, float, 4>(img, W, x, y, sum);
 						}
-						sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
-					}
-					for (; a < steps; a++) {
-						const float2 d = qd[stride * a];
-						const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
-						sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+						for (; a < steps; a++) {
+							const float2 d = qd[st4 * a];
+							const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+							sample_n_inside<SPix, float, 1>(img, W, x, y, sum);
+						}
+						cnt = steps;
+					} else {
+						for (; a + 4 <= steps; a += 4) {
+							float x[4], y[4];
+#pragma unroll
+							for (int u = 0; u < 4; u++) {
+								const float2 d = qd[st4 * (a + u)];
+								x[u] = cx + d.x * r;
+								y[u] = cy + d.y * r;
+							}
+							sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+						}
+						for (; a < steps; a++) {
+							const float2 d = qd[st4 * a];
+							const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+							sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+						}
 					}
 					float val = cnt >= 4 ? sum / (float)cnt : mean;
 					if (window) val *= window[ri];
-					qprof[p] = val;
+					qprof[q * nr + ri] = val;
 				}
 			}
 			__syncthreads();
@@ -881,7 +1010,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				}
 			}
 			__syncthreads();
-			fft_stages<false>(fft, nullptr, 4, P.qi_fft, tw, 1.0f, wid, lane);
+			fft_pow2_stages(fft, 4, P.qi_fft, tw, 1.0f, wid, lane);
 			// F(p) * conj(F(rev p)) for X and Y, held in registers until every thread has
 			// read the forward spectra, then written in leaf order into buffers 0 and 1
 			// for the inverse transforms (2N / kBlock <= 4 values per thread)
@@ -906,7 +1035,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				}
 			}
 			__syncthreads();
-			fft_stages<false>(fft, nullptr, 2, P.qi_fft, tw + N, -1.0f, wid, lane);
+			fft_pow2_stages(fft, 2, P.qi_fft, tw + N, -1.0f, wid, lane);
 			if (wid < 2) {
 				const float2* G = fft + wid * N;
 				float bv = 0.0f;
@@ -1009,32 +1138,56 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			const bool ok = !isnan(pcx) && !isnan(pcy);
 			for (int j = tid; j < P.z_A; j += kBlock) zdir[j] = P.z_dirs[j];
 			__syncthreads();
+			// slots bin-major (lane = nch chunks x bins), so that wavefronts of inner bins take
+			// the unmasked loop
 			for (int base = 0; base < nch * nb; base += kBlock) {
 				const int p = base + tid;
-				if (p < nch * nb) {
-					const int ch = p / nb, bin = p - ch * nb;
+				const int bin = p / nch, ch = p - bin * nch;
+				const float r = pminr + rstep * (float)bin;
+				const bool valid = p < nch * nb;
+				const bool clear = !valid || (ok && circle_inside(pcx, pcy, r, Wf, Hf));
+				const bool fast = __ballot(!clear) == 0;  // uniform over the wavefront
+				if (valid) {
 					const int a0 = (ch * P.z_A) / nch, a1 = ((ch + 1) * P.z_A) / nch;
-					const float r = pminr + rstep * (float)bin;
 					double s = 0.0;
 					int c = 0;
 					int a = a0;
-					for (; a + 4 <= a1; a += 4) {
-						float x[4], y[4];
+					if (fast) {
+						for (; a + 4 <= a1; a += 4) {
+							float x[4], y[4];
 #pragma unroll
-						for (int u = 0; u < 4; u++) {
-							const float2 d = zdir[a + u];
-							x[u] = pcx + d.x * r;
-							y[u] = pcy + d.y * r;
+							for (int u = 0; u < 4; u++) {
+								const float2 d = zdir[a + u];
+								x[u] = pcx + d.x * r;
+								y[u] = pcy + d.y * r;
+							}
+							sample_n_inside<SPix, double, 4>(img, W, x, y, s);
 						}
-						sample_n_acc<SPix, double, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+						for (; a < a1; a++) {
+							const float2 d = zdir[a];
+							const float x[1] = { pcx + d.x * r }, y[1] = { pcy + d.y * r };
+							sample_n_inside<SPix, double, 1>(img, W, x, y, s);
+						}
+						c = a1 - a0;
+					} else {
+						for (; a + 4 <= a1; a += 4) {
+							float x[4], y[4];
+#pragma unroll
+							for (int u = 0; u < 4; u++) {
+								const float2 d = zdir[a + u];
+								x[u] = pcx + d.x * r;
+								y[u] = pcy + d.y * r;
+							}
+							sample_n_acc<SPix, double, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+						}
+						for (; a < a1; a++) {
+							const float2 d = zdir[a];
+							const float x[1] = { pcx + d.x * r }, y[1] = { pcy + d.y * r };
+							sample_n_acc<SPix, double, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
+						}
 					}
-					for (; a < a1; a++) {
-						const float2 d = zdir[a];
-						const float x[1] = { pcx + d.x * r }, y[1] = { pcy + d.y * r };
-						sample_n_acc<SPix, double, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, s, c);
-					}
-					zpart[p] = s;
-					zcnt[p] = c;
+					zpart[ch * nb + bin] = s;
+					zcnt[ch * nb + bin] = c;
 				}
 			}
 			__syncthreads();
