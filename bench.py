@@ -209,10 +209,12 @@ def precision_bench(a):
 MLE_SETTINGS = dict(width=48, height=48, qi_minradius=1.0, qi_roi_coverage=1.0, qi_radial_coverage=1.4,
                     qi_angular_coverage=1.5, qi_angstep_factor=1.0, qi_iterations=5, zlut_minradius=0.0,
                     zlut_roi_coverage=1.0, zlut_radial_coverage=2.0, zlut_angular_coverage=0.7)
-# FLOPs of one model pass per pixel, counted from lutmodel.hip: per sub-sample 30 (position 6,
-# radius 6, clamp/bin 6, value 4, z-slope 4, x/y slope 4 + the r-derivative factor), ovs^2 = 16
-# sub-samples, then mu and the Jacobian 5, the gradient 12 and the Fisher matrix 45.
-MLE_FLOP_PER_PX_PASS = 16 * 30 + 5 + 12 + 45
+# FLOPs of one model pass per pixel, counted from the fit's pixel model in lutmodel.hip (a fused
+# multiply-add counts 2): per sub-sample 20 (r^2 and its floor 2, reciprocal square root 1, scaled
+# radius 3, clamp 1, fraction 1, value 3, z-slope 3, slope 2, x/y slope 4), ovs^2 = 16 sub-samples,
+# the per-pixel sub-sample offsets 24, then mu and the Jacobian 7, the weights 3, the gradient 10,
+# the Fisher matrix 35 and the final scaling 6.
+MLE_FLOP_PER_PX_PASS = 16 * 20 + 24 + 7 + 3 + 10 + 35 + 6
 
 
 def mle_bench(a):

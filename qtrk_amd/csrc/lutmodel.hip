@@ -18,7 +18,7 @@ namespace qtrk {
 constexpr int kModelBlock = 256;  // lanes per ROI of the forward/backward passes; the fit takes 64-256
 
 struct ModelLds {
-	int zi, dzi, data, red, th, total;
+	int zi, dzi, row, data, red, th, total;
 };
 __host__ __device__ inline int m16(int x) { return (x + 15) & ~15; }
 __host__ __device__ inline ModelLds model_lds(const LutModelParams& P, bool fit)
@@ -27,6 +27,7 @@ __host__ __device__ inline ModelLds model_lds(const LutModelParams& P, bool fit)
 	int o = 0;
 	L.zi = o;    o += m16(P.steps * 4);
 	L.dzi = o;   o += m16(P.steps * 4);
+	L.row = o;   o += fit ? m16(P.steps * 16) : 0;
 	L.data = o;  o += fit ? m16(P.width * P.height * 4) : 0;
 	L.red = o;   o += m16(5 * 24 * 8);  // 4 wavefront partials + the totals
 	L.th = o;    o += m16(8 * 4);
@@ -133,47 +134,64 @@ __device__ __forceinline__ PixelG pixel_g(const LutModelParams& P, int x, int y,
 }
 
 // The same model for the fit, where nothing has to reproduce the reference's
-// rounding: the hardware reciprocal square root (|p - c| = r2 * rsq(r2), and
-// 1/|p - c| for the slope) replaces the IEEE square root and division, the
-// sub-sample offsets are hoisted, and products are fused.  Agrees with pixel_g
-// to float rounding.
+// rounding: the hardware reciprocal square root replaces the IEEE square root
+// and division, the row segment (a, b - a, da, db - da) of each radial bin is
+// one 16-byte LDS read from a table built per iteration, the bin clamp is one
+// med3 (a sample is inside the LUT's radii exactly when the clamp leaves it
+// unchanged), the sub-sample offsets are hoisted per pixel, and products are
+// fused.  Agrees with pixel_g to float rounding.
 __device__ __forceinline__ PixelG pixel_g_fast(const LutModelParams& P, int x, int y, float px, float py,
-                                               float radialcov, float step, const float* zi, const float* dzi)
+                                               float radialcov, float step, const float4* row)
 {
-	float s = 0, sx = 0, sy = 0, sz = 0;
+	constexpr int kMaxOvs = kMaxFitOvs;
 	const float rmax = (float)(P.steps - 2);
 	const float mr = P.minr * radialcov;
 	const float X0 = (float)x + 0.5f * step - px - 0.5f;
 	const float Y0 = (float)y + 0.5f * step - py - 0.5f;
-	for (int ox = 0; ox < P.ovs; ox++) {
+	const int ovs = P.ovs;
+	float Ys[kMaxOvs], Y2[kMaxOvs];
+#pragma unroll
+	for (int o = 0; o < kMaxOvs; o++) {
+		Ys[o] = __builtin_fmaf((float)o, step, Y0);
+		Y2[o] = Ys[o] * Ys[o];
+	}
+	float sa = 0, sb = 0, za = 0, zb = 0, sx = 0, sy = 0;
+	for (int ox = 0; ox < ovs; ox++) {
 		const float X = __builtin_fmaf((float)ox, step, X0);
 		const float X2 = X * X;
-		for (int oy = 0; oy < P.ovs; oy++) {
-			const float Y = __builtin_fmaf((float)oy, step, Y0);
-			const float r2 = __builtin_fmaf(Y, Y, X2);
-			const float ip = r2 > 0 ? __builtin_amdgcn_rsqf(r2) : 0.0f;
-			const float ru = __builtin_fmaf(r2 * ip, radialcov, -mr);
-			const bool inside = ru >= 0 && ru <= rmax;
-			const float r = fminf(fmaxf(ru, 0.0f), rmax);
+#pragma unroll
+		for (int oy = 0; oy < kMaxOvs; oy++) {
+			if (oy >= ovs) break;
+			const float r2 = fmaxf(X2 + Y2[oy], 1e-30f);  // the centre itself: slope term 0 * finite
+			const float rc = radialcov * __builtin_amdgcn_rsqf(r2);
+			const float ru = __builtin_fmaf(r2, rc, -mr);
+			const float r = __builtin_amdgcn_fmed3f(ru, 0.0f, rmax);
 			const int i = (int)r;
-			const float fr = r - i;
-			const float a = zi[i], b = zi[i + 1];
-			const float ba = b - a;
-			s += __builtin_fmaf(ba, fr, a);
-			const float da = dzi[i], db = dzi[i + 1];
-			sz += __builtin_fmaf(db - da, fr, da);
-			const float k = inside ? ba * radialcov * ip : 0.0f;
+			const float fr = __builtin_amdgcn_fractf(r);
+			const float4 q = row[i];
+			sa += q.x;
+			sb = __builtin_fmaf(q.y, fr, sb);
+			za += q.z;
+			zb = __builtin_fmaf(q.w, fr, zb);
+			const float k = ru == r ? q.y * rc : 0.0f;  // the slope in r is zero where r is clamped
 			sx = __builtin_fmaf(-k, X, sx);
-			sy = __builtin_fmaf(-k, Y, sy);
+			sy = __builtin_fmaf(-k, Ys[oy], sy);
 		}
 	}
-	const float inv = 1.0f / (float)(P.ovs * P.ovs);
+	const float inv = 1.0f / (float)(ovs * ovs);
 	PixelG o;
-	o.g = s * inv;
+	o.g = (sa + sb) * inv;
 	o.gx = sx * inv;
 	o.gy = sy * inv;
-	o.gz = sz * inv;
+	o.gz = (za + zb) * inv;
 	return o;
+}
+
+// The fit's row table from zi and dzi (bins 0 .. steps-2; the clamp keeps i <= steps-2).
+__device__ __forceinline__ void build_row_table(const LutModelParams& P, const float* zi, const float* dzi, float4* row)
+{
+	for (int r = threadIdx.x; r < P.steps - 1; r += blockDim.x)
+		row[r] = make_float4(zi[r], zi[r + 1] - zi[r], dzi[r], dzi[r + 1] - dzi[r]);
 }
 
 // ------------------------------------------------------------ forward / backward
@@ -326,6 +344,7 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 	const ModelLds L = model_lds(P, true);
 	float* zi = (float*)(smem + L.zi);
 	float* dzi = (float*)(smem + L.dzi);
+	float4* row = (float4*)(smem + L.row);
 	float* dat = (float*)(smem + L.data);
 	double* red = (double*)(smem + L.red);
 	float* th = (float*)(smem + L.th);
@@ -347,10 +366,12 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		// start: I and bg by linear least squares of d = bg + I g at the start position
 		build_z_row(P, lut, th[2], zi, dzi);
 		__syncthreads();
+		build_row_table(P, zi, dzi, row);
+		__syncthreads();
 		float v[4] = { 0, 0, 0, 0 };
 		for (int p = threadIdx.x; p < npx; p += nt) {
 			const int y = p / P.width, x = p - y * P.width;
-			const PixelG G = pixel_g_fast(P, x, y, th[0], th[1], radialcov, step, zi, dzi);
+			const PixelG G = pixel_g_fast(P, x, y, th[0], th[1], radialcov, step, row);
 			const float d = dat[p];
 			v[0] += G.g;
 			v[1] += G.g * G.g;
@@ -375,23 +396,27 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		const float x0 = th[0], y0 = th[1], z0 = th[2], I = th[3], bg = th[4];
 		build_z_row(P, lut, z0, zi, dzi);
 		__syncthreads();
+		build_row_table(P, zi, dzi, row);
+		__syncthreads();
 		const bool last = it == P.iters;
 		float v[21];
 #pragma unroll
 		for (int k = 0; k < 21; k++) v[k] = 0;
 		for (int p = threadIdx.x; p < npx; p += nt) {
 			const int y = p / P.width, x = p - y * P.width;
-			const PixelG G = pixel_g_fast(P, x, y, x0, y0, radialcov, step, zi, dzi);
-			float mu = bg + I * G.g;
+			const PixelG G = pixel_g_fast(P, x, y, x0, y0, radialcov, step, row);
+			float mu = __builtin_fmaf(I, G.g, bg);
 			mu = mu > 1e-6f ? mu : 1e-6f;
 			const float d = dat[p];
 			const float J[5] = { I * G.gx, I * G.gy, I * G.gz, G.g, 1.0f };
-			const float gw = 1.0f - d / mu, fw = 1.0f / mu;
+			const float fw = __builtin_amdgcn_rcpf(mu);
+			const float gw = __builtin_fmaf(-d, fw, 1.0f);
 #pragma unroll
 			for (int i = 0; i < 5; i++) {
-				v[i] += gw * J[i];
+				v[i] = __builtin_fmaf(gw, J[i], v[i]);
+				const float wJ = fw * J[i];
 #pragma unroll
-				for (int j = i; j < 5; j++) v[5 + tri5(i, j)] += fw * J[i] * J[j];
+				for (int j = i; j < 5; j++) v[5 + tri5(i, j)] = __builtin_fmaf(wJ, J[j], v[5 + tri5(i, j)]);
 			}
 			if (last) v[20] += mu - d * logf(mu);
 		}
@@ -489,6 +514,7 @@ hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int 
 {
 	if (p.n <= 0) return hipSuccess;
 	if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
+	if (p.ovs < 1 || p.ovs > kMaxFitOvs || p.steps < 2) return hipErrorInvalidValue;
 	const size_t lds = lutmodel_lds_bytes(p, true);
 	hipError_t e = set_lds((const void*)lutmodel_fit_kernel, lds);
 	if (e != hipSuccess) return e;

@@ -22,6 +22,7 @@
 namespace qtrk {
 
 constexpr int kModelParams = 5;  // x, y, z, I, bg
+constexpr int kMaxFitOvs = 8;    // sub-samples per pixel side the fit supports (forward/backward: any)
 
 struct LutModelParams {
 	const float* luts;       // [count][planes][steps]

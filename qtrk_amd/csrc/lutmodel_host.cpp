@@ -109,6 +109,7 @@ QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float*
 	return GUARD({
 		if (!m || !data || !thetaInit || !theta) fail("null argument");
 		if (iterations < 0) fail("iterations must be >= 0");
+		if (m->base.ovs > kMaxFitOvs) fail("the fit supports at most " + std::to_string(kMaxFitOvs) + " sub-samples per pixel side");
 		m->check(n);
 		HIP_CHECK(hipSetDevice(m->dev));
 		LutModelParams p = m->params(thetaInit, lutIndex, n);
