@@ -330,9 +330,15 @@ QTRK_API ResultManager* RMCreate(const char* file, const char* frameinfo, Result
 	return rm;
 }
 
+static ResultManager* checked(ResultManager* rm)
+{
+	if (!rm) fail("null result manager");
+	return rm;
+}
+
 QTRK_API void RMSetTracker(ResultManager* rm, QueuedTracker* qtrk)
 {
-	GUARD({ rm->SetTracker(qtrk); });
+	GUARD({ checked(rm)->SetTracker(qtrk); });
 }
 
 QTRK_API void RMDestroy(ResultManager* rm)
@@ -342,39 +348,51 @@ QTRK_API void RMDestroy(ResultManager* rm)
 
 QTRK_API void RMStoreFrameInfo(ResultManager* rm, int frame, double timestamp, float* cols)
 {
-	GUARD({ rm->StoreFrameInfo(frame, timestamp, cols); });
+	GUARD({ checked(rm)->StoreFrameInfo(frame, timestamp, cols); });
 }
 
 QTRK_API int RMGetBeadResults(ResultManager* rm, int start, int numFrames, int bead, LocalizationResult* results)
 {
 	int n = 0;
-	GUARD({ n = rm->GetBeadPositions(start, start + numFrames, bead, results); });
+	GUARD({
+		if (numFrames > 0 && !results) fail("null results");
+		n = checked(rm)->GetBeadPositions(start, start + numFrames, bead, results);
+	});
 	return n;
 }
 
 QTRK_API void RMGetFrameCounters(ResultManager* rm, RMFrameCounters* fc)
 {
-	GUARD({ *fc = rm->GetFrameCounters(); });
+	GUARD({
+		if (!fc) fail("null counters");
+		*fc = checked(rm)->GetFrameCounters();
+	});
 }
 
 QTRK_API void RMFlush(ResultManager* rm)
 {
-	GUARD({ rm->Flush(); });
+	GUARD({ checked(rm)->Flush(); });
 }
 
 QTRK_API int RMGetResults(ResultManager* rm, int startFrame, int numFrames, LocalizationResult* results)
 {
 	int n = 0;
-	GUARD({ n = rm->GetResults(results, startFrame, numFrames); });
+	GUARD({
+		if (numFrames > 0 && !results) fail("null results");
+		n = checked(rm)->GetResults(results, startFrame, numFrames);
+	});
 	return n;
 }
 
 QTRK_API void RMRemoveBead(ResultManager* rm, int bead)
 {
-	GUARD({ rm->RemoveBeadResults(bead); });
+	GUARD({ checked(rm)->RemoveBeadResults(bead); });
 }
 
 QTRK_API void RMGetConfig(ResultManager* rm, ResultManagerConfig* cfg)
 {
-	GUARD({ *cfg = rm->Config(); });
+	GUARD({
+		if (!cfg) fail("null config");
+		*cfg = checked(rm)->Config();
+	});
 }

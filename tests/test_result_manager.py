@@ -198,3 +198,15 @@ def test_result_manager_trims_memory_and_removes_beads(gpu, tmp_path):
     finally:
         rm.free()
         trk.free()
+
+
+def test_null_handles_raise():
+    L = api.lib()
+    L.QTrkClearLastError()
+    L.RMFlush(None)
+    with pytest.raises(api.QTrkError, match="null result manager"):
+        api.check()
+    fc = api.RMFrameCounters()
+    L.RMGetFrameCounters(None, api.C.byref(fc))
+    with pytest.raises(api.QTrkError, match="null result manager"):
+        api.check()
