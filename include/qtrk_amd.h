@@ -175,7 +175,9 @@ QTRK_API int QTrkLUTModelBackward(QTrkLUTModel* m, const float* theta, const int
  * Fisher-scoring Newton steps (FP64 5x5 solves); a start with I <= 0 first gets I and bg by
  * linear least squares at its (x, y, z).  Writes theta [n][5] (may alias thetaInit)
  * and, at the fitted theta, the Fisher matrix [n][25] (FP64), the CRLB sqrt(diag(F^-1)) [n][5]
- * and the negative log-likelihood sum(mu - data*ln(mu)) [n]; those three may be NULL. */
+ * and the negative log-likelihood sum(mu - data*ln(mu)) [n]; those three may be NULL.
+ * The fit takes models of ovs <= 8 (at most 8x8 sub-samples per pixel); larger ones
+ * fail with an error, as Forward and Backward still take them. */
 QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float* thetaInit, const int* lutIndex,
                                 int n, int iterations, float* theta, double* fisher, float* crlb, float* nll,
                                 void* stream);

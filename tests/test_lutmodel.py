@@ -72,6 +72,25 @@ def test_forward_bit_exact(oracle, lut, gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("ovs,spline", [(1, True), (3, True), (8, True), (4, False)])
+def test_forward_bit_exact_sub_sampling_and_linear(oracle, lut, gpu, ovs, spline):
+    """Other sub-sample counts, and linear interpolation between planes."""
+    import torch
+    rng = np.random.default_rng(12 + ovs)
+    th = synth.model_params(8, rng, W)
+    th[0, 2] = 198.7
+    m = api.LUTModel(lut, 0.0, MAXR, W, H, ovs=ovs, spline=spline)
+    try:
+        mu = m.forward(torch.from_numpy(th)).cpu().numpy()
+        for i in range(len(th)):
+            g = oracle.generate_image_from_lut(W, H, lut, 0.0, MAXR, *map(float, th[i, :3]), spline=spline, ovs=ovs)
+            ref = np.float32(th[i, 4]) + np.float32(th[i, 3]) * g
+            assert np.array_equal(mu[i], ref), f"ROI {i}: max diff {np.max(np.abs(mu[i] - ref))}"
+    finally:
+        m.free()
+
+
+@pytest.mark.gpu
 def test_jacobian_and_adjoint_vs_finite_differences(oracle64, lut, gpu):
     import torch
     rng = np.random.default_rng(12)
@@ -171,6 +190,40 @@ def test_fit_noise_free_recovers_truth(lut, gpu):
         assert np.max(np.abs(fit[:, 2] - th[:, 2])) < 1e-2
         assert np.max(np.abs(fit[:, 3] / th[:, 3] - 1)) < 1e-3
         assert np.max(np.abs(fit[:, 4] - th[:, 4])) < 1e-2
+    finally:
+        m.free()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ovs", [1, 3, 8])
+def test_fit_noise_free_other_sub_sampling(lut, gpu, ovs):
+    """The fit's packed row table at the smallest, an odd and the largest sub-sample count it takes."""
+    import torch
+    rng = np.random.default_rng(40 + ovs)
+    th = synth.model_params(32, rng, W)
+    m = api.LUTModel(lut, 0.0, MAXR, W, H, ovs=ovs)
+    try:
+        data = m.forward(torch.from_numpy(th))
+        t0 = th.copy()
+        t0[:, :2] += rng.uniform(-0.3, 0.3, (32, 2)).astype(np.float32)
+        t0[:, 2] += rng.uniform(-1.0, 1.0, 32).astype(np.float32)
+        fit, _, _, _ = m.fit(data, torch.from_numpy(t0), iterations=10)
+        fit = fit.cpu().numpy()
+        assert np.max(np.abs(fit[:, :2] - th[:, :2])) < 1e-3
+        assert np.max(np.abs(fit[:, 2] - th[:, 2])) < 1e-2
+    finally:
+        m.free()
+
+
+@pytest.mark.gpu
+def test_fit_rejects_more_than_8_sub_samples(lut, gpu):
+    import torch
+    th = synth.model_params(4, np.random.default_rng(3), W)
+    m = api.LUTModel(lut, 0.0, MAXR, W, H, ovs=9)
+    try:
+        data = m.forward(torch.from_numpy(th))          # rendering takes any count
+        with pytest.raises(api.QTrkError, match="at most 8 sub-samples"):
+            m.fit(data, torch.from_numpy(th))
     finally:
         m.free()
 
