@@ -167,6 +167,16 @@ __device__ __forceinline__ int trunc_sat(float x)
 	return r;
 }
 
+// a * b + c for 0 <= a, b < 2^24, as one instruction; the compiler otherwise
+// scales b * a and c separately when the sum indexes a float array.  b is
+// wavefront-uniform.
+__device__ __forceinline__ int mad_u24(int a, int b, int c)
+{
+	int r;
+	asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
+	return r;
+}
+
 // Bilinear samples with the reference's "outside" rule (cputrack/utils.h:39-58):
 // a sample is inside iff trunc(x) is in [0, W-2] and trunc(y) in [0, H-2] (the
 // saturating conversion keeps huge coordinates out of range); the caller passes
@@ -188,8 +198,10 @@ __device__ __forceinline__ void sample_n_acc(const T* img, int W, float Wf, floa
 	}
 #pragma unroll
 	for (int u = 0; u < U; u++) {
-		// ROIs are far below 2^24 pixels: a 24-bit multiply-add forms the index
-		const T* p = img + (in[u] ? __mul24(ry[u], W) + rx[u] : 0);
+		// ROIs are far below 2^24 pixels: a 24-bit multiply-add forms the index, which
+		// is then masked to 0 outside (no branch; the product of out-of-range
+		// coordinates is discarded)
+		const T* p = img + (mad_u24(ry[u], W, rx[u]) & -in[u]);
 		v00[u] = (float)p[0];
 		v10[u] = (float)p[1];
 		v01[u] = (float)p[W];
@@ -207,20 +219,20 @@ __device__ __forceinline__ void sample_n_acc(const T* img, int W, float Wf, floa
 }
 
 // The same samples when every one is known to be inside (the caller checked the circle
-// against the ROI): no range test, no masking; the sum runs in the same order.
+// against the ROI): no range test, no masking; the sum runs in the same order.  Inside,
+// x, y >= 1, where x - (float)(int)x is x - floor(x), exactly the hardware fract (the
+// difference is a multiple of ulp(x) below 1, so it is representable), and the pixel
+// index is one 24-bit multiply-add.
 template <typename T, typename Acc, int U>
 __device__ __forceinline__ void sample_n_inside(const T* img, int W, const float (&x)[U], const float (&y)[U], Acc& sum)
 {
-	int rx[U], ry[U];
+	int idx[U];
 	float v00[U], v10[U], v01[U], v11[U];
 #pragma unroll
-	for (int u = 0; u < U; u++) {
-		rx[u] = trunc_sat(x[u]);
-		ry[u] = trunc_sat(y[u]);
-	}
+	for (int u = 0; u < U; u++) idx[u] = mad_u24(trunc_sat(y[u]), W, trunc_sat(x[u]));
 #pragma unroll
 	for (int u = 0; u < U; u++) {
-		const T* p = img + __mul24(ry[u], W) + rx[u];
+		const T* p = img + idx[u];
 		v00[u] = (float)p[0];
 		v10[u] = (float)p[1];
 		v01[u] = (float)p[W];
@@ -228,10 +240,95 @@ __device__ __forceinline__ void sample_n_inside(const T* img, int W, const float
 	}
 #pragma unroll
 	for (int u = 0; u < U; u++) {
-		const float fx = x[u] - (float)rx[u];
+		const float fx = __builtin_amdgcn_fractf(x[u]);
 		const float a0 = v00[u] + (v10[u] - v00[u]) * fx;
 		const float a1 = v01[u] + (v11[u] - v01[u]) * fx;
-		sum += (Acc)(a0 + (a1 - a0) * (y[u] - (float)ry[u]));
+		sum += (Acc)(a0 + (a1 - a0) * __builtin_amdgcn_fractf(y[u]));
+	}
+}
+
+// ROI staging.  A contiguous ROI (rows `pitch` = W pixels apart) at a 16-byte
+// aligned address without calibration is read in 16-byte chunks, up to 8 per
+// thread in flight, and written to LDS in 16-byte stores; otherwise pixels are
+// read one by one, 8 per thread in flight.  Either way each pixel's value is
+// the reference's: the source value as float, 0 for the first four with
+// ClearFirstFourPixels, then offset/gain with their unit factors.
+template <typename PT, typename SPix>
+__device__ __forceinline__ void stage_roi(const uint8_t* base, int pitch, int W, int npx, const float* off,
+                                          const float* gain, bool clear4, SPix* img, int tid)
+{
+	constexpr int K = 8;
+	if constexpr (sizeof(SPix) >= sizeof(PT)) {  // (the LDS format is never narrower than the source)
+	if (!off && !gain && pitch == W * (int)sizeof(PT) && ((uintptr_t)base & 15) == 0) {
+		constexpr int PPC = 16 / sizeof(PT);  // pixels per chunk
+		const int nchunk = npx / PPC;
+		const uint4* src = (const uint4*)base;
+		for (int c0 = tid; c0 < nchunk; c0 += K * kBlock) {
+			uint4 raw[K];
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				const int c = c0 + k * kBlock;
+				raw[k] = c < nchunk ? src[c] : make_uint4(0, 0, 0, 0);
+			}
+#pragma unroll
+			for (int k = 0; k < K; k++) {
+				const int c = c0 + k * kBlock;
+				if (c < nchunk) {
+					PT px[PPC];
+					__builtin_memcpy(px, &raw[k], 16);
+					SPix out[PPC];
+#pragma unroll
+					for (int j = 0; j < PPC; j++) {
+						float v = (float)px[j];
+						if (clear4 && c * PPC + j < 4) v = 0.0f;
+						out[j] = (SPix)v;  // exact: SPix is float or the source type
+					}
+					uint4 o[PPC * sizeof(SPix) / 16];
+					__builtin_memcpy(o, out, sizeof(o));
+					uint4* dst = (uint4*)(img + c * PPC);
+#pragma unroll
+					for (int q = 0; q < (int)(PPC * sizeof(SPix) / 16); q++) dst[q] = o[q];
+				}
+			}
+		}
+		for (int i = nchunk * PPC + tid; i < npx; i += kBlock) {
+			float v = (float)((const PT*)base)[i];
+			if (clear4 && i < 4) v = 0.0f;
+			img[i] = (SPix)v;
+		}
+		return;
+	}
+	}
+	const float invW = 1.0f / (float)W;
+	for (int i0 = tid; i0 < npx; i0 += K * kBlock) {
+		PT v[K];
+		float o[K], g[K];
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			// unconditional loads (past the end: the last pixel again), all in flight together
+			const int i = min(i0 + k * kBlock, npx - 1);
+			// row = i / W: the float estimate is within one row, then corrected
+			int y = (int)((float)i * invW);
+			int x = i - y * W;
+			const int lo = x < 0, hi = x >= W;
+			y += hi - lo;
+			x += (lo - hi) * W;
+			v[k] = ((const PT*)(base + (int64_t)y * pitch))[x];
+			o[k] = off ? off[i] : 0.0f;
+			g[k] = gain ? gain[i] : 0.0f;
+		}
+#pragma unroll
+		for (int k = 0; k < K; k++) {
+			const int i = i0 + k * kBlock;
+			if (i < npx) {
+				float t = (float)v[k];
+				if (clear4 && i < 4) t = 0.0f;
+				if (off && !gain) t = t + o[k] * 1.0f;
+				else if (gain && !off) t = t * g[k] * 1.0f;
+				else if (gain && off) t = (t + o[k] * 1.0f) * g[k] * 1.0f;
+				img[i] = (SPix)t;  // exact: SPix is float whenever t is not an integer of the source range
+			}
+		}
 	}
 }
 
@@ -436,16 +533,32 @@ __device__ __forceinline__ void fft_pow2_stages(float2* base, int nbufs, const F
 
 // NormalizeRadialProfile (cputrack/utils.cpp:230-261) by one thread: double sums,
 // float mean and deviations, 1/sqrt of the double mean square.
+// The sums run in index order; the values are read 8 at a time ahead of them,
+// so that the serial additions do not wait on each LDS read.
 __device__ __forceinline__ void normalize_profile(float* p, int n)
 {
+	constexpr int K = 8;
 	double s = 0.0;
-	for (int i = 0; i < n; i++) s += (double)p[i];
+	for (int i0 = 0; i0 < n; i0 += K) {
+		float v[K];
+#pragma unroll
+		for (int j = 0; j < K; j++) v[j] = p[min(i0 + j, n - 1)];
+#pragma unroll
+		for (int j = 0; j < K; j++)
+			if (i0 + j < n) s += (double)v[j];
+	}
 	const float pm = (float)(s / (double)n);
 	double s2 = 0.0;
-	for (int i = 0; i < n; i++) {
-		const float v = p[i] - pm;
-		p[i] = v;
-		s2 += (double)(float)(v * v);
+	for (int i0 = 0; i0 < n; i0 += K) {
+		float v[K];
+#pragma unroll
+		for (int j = 0; j < K; j++) v[j] = p[min(i0 + j, n - 1)] - pm;
+#pragma unroll
+		for (int j = 0; j < K; j++)
+			if (i0 + j < n) {
+				p[i0 + j] = v[j];
+				s2 += (double)(float)(v[j] * v[j]);
+			}
 	}
 	const double inv = (double)1.0f / sqrt(s2 / (double)n);
 	for (int i = 0; i < n; i++) p[i] = (float)((double)p[i] * inv);
@@ -835,21 +948,9 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		const float* off = calib && P.calib_offset ? P.calib_offset + (size_t)bead * npx : nullptr;
 		const float* gain = calib && P.calib_gain ? P.calib_gain + (size_t)bead * npx : nullptr;
 		const bool clear4 = (P.mode & kModeClearFirstFour) && !build;
-		for (int y = wid; y < H; y += kBlock / 64) {
-			const uint8_t* row = base + (int64_t)y * P.pitch;
-			for (int x = lane; x < W; x += 64) {
-				float v;
-				if (P.pdt == 1) v = (float)((const uint16_t*)row)[x];
-				else if (P.pdt == 0) v = (float)row[x];
-				else v = ((const float*)row)[x];
-				const int i = y * W + x;
-				if (clear4 && i < 4) v = 0.0f;
-				if (off && !gain) v = v + off[i] * 1.0f;
-				else if (gain && !off) v = v * gain[i] * 1.0f;
-				else if (gain && off) v = (v + off[i] * 1.0f) * gain[i] * 1.0f;
-				img[i] = (SPix)v;  // exact: SPix is float whenever v is not an integer of the source range
-			}
-		}
+		if (P.pdt == 1) stage_roi<uint16_t>(base, P.pitch, W, npx, off, gain, clear4, img, tid);
+		else if (P.pdt == 0) stage_roi<uint8_t>(base, P.pitch, W, npx, off, gain, clear4, img, tid);
+		else stage_roi<float>(base, P.pitch, W, npx, off, gain, clear4, img, tid);
 	}
 	__syncthreads();
 	const float Wf = (float)W, Hf = (float)H;
@@ -1224,19 +1325,83 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				for (int bin = tid; bin < nb; bin += kBlock) dst[bin] += zprof[bin];
 			} else if (has_lut) {
 				// ---- LUT scores, best plane, FP64 peak fit (cputrack/cpu_tracker.cpp:791-868)
-				for (int k = tid; k < P.zplanes; k += kBlock) {
-					const float* Lk = L0 + (size_t)k * nb;
-					double s = 0.0;
-					for (int r = 0; r < nb; r++) {
-						double d = (double)(zprof[r] - Lk[r]);
-						if (P.zcmp) d *= (double)P.zcmp[r];
-						d = -d * d;
-						s += d;
+				// One thread per plane sums its bins in order.  Unless QuadrantAlign samples the
+				// ROI again, the LUT goes through LDS tiles in the image's place: coalesced
+				// 16-byte loads, rows padded to nb + 1 floats so that the threads' reads of a
+				// bin fall in different banks.
+				const int pitchL = nb + 1;
+				const int tmax = min(kBlock, (W * H * (int)sizeof(SPix)) / (pitchL * 4));
+				const bool tiled = !(EXT && mode_align(P)) && (nb & 3) == 0 && ((uintptr_t)L0 & 15) == 0 &&
+				                   tmax >= 32;
+				if (tiled) {
+					float* tile = (float*)(lds + L.img);
+					const int ntile = (P.zplanes + tmax - 1) / tmax;
+					const int tp = (P.zplanes + ntile - 1) / ntile;  // planes per tile, balanced
+					const int qpr = nb >> 2;                         // 16-byte chunks per row
+					for (int k0 = 0; k0 < P.zplanes; k0 += tp) {
+						const int np = min(tp, P.zplanes - k0);
+						const float4* src = (const float4*)(L0 + (size_t)k0 * nb);
+						const int nq = np * qpr;
+						for (int c0 = tid; c0 < nq; c0 += 4 * kBlock) {
+							float4 v[4];
+#pragma unroll
+							for (int u = 0; u < 4; u++) {
+								const int c = c0 + u * kBlock;
+								v[u] = src[c < nq ? c : nq - 1];
+							}
+#pragma unroll
+							for (int u = 0; u < 4; u++) {
+								const int c = c0 + u * kBlock;
+								if (c < nq) {
+									const int kk = c / qpr;
+									float* d = tile + kk * pitchL + ((c - kk * qpr) << 2);
+									d[0] = v[u].x;
+									d[1] = v[u].y;
+									d[2] = v[u].z;
+									d[3] = v[u].w;
+								}
+							}
+						}
+						__syncthreads();
+						if (tid < np) {
+							// bins read 4 at a time ahead of the serial sum (nb is a multiple of 4)
+							const float* row = tile + tid * pitchL;
+							double s = 0.0;
+							for (int r0 = 0; r0 < nb; r0 += 4) {
+								float lv[4], pv[4];
+#pragma unroll
+								for (int j = 0; j < 4; j++) {
+									lv[j] = row[r0 + j];
+									pv[j] = zprof[r0 + j];
+								}
+#pragma unroll
+								for (int j = 0; j < 4; j++) {
+									double d = (double)(pv[j] - lv[j]);
+									if (P.zcmp) d *= (double)P.zcmp[r0 + j];
+									d = -d * d;
+									s += d;
+								}
+							}
+							scores[k0 + tid] = s;
+							if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k0 + tid] = (float)s;
+						}
+						__syncthreads();
 					}
-					scores[k] = s;
-					if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k] = (float)s;
+				} else {
+					for (int k = tid; k < P.zplanes; k += kBlock) {
+						const float* Lk = L0 + (size_t)k * nb;
+						double s = 0.0;
+						for (int r = 0; r < nb; r++) {
+							double d = (double)(zprof[r] - Lk[r]);
+							if (P.zcmp) d *= (double)P.zcmp[r];
+							d = -d * d;
+							s += d;
+						}
+						scores[k] = s;
+						if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k] = (float)s;
+					}
+					__syncthreads();
 				}
-				__syncthreads();
 				int best = 0;
 				z = scores_peak(scores, P.zplanes, red, st, tid, lane, wid, best);
 				if (tid == 0 && P.best_out) P.best_out[roi] = best;

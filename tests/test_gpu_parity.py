@@ -151,6 +151,24 @@ def test_pixel_formats_and_pitch(gpu, oracle):
         trk.free()
 
 
+@pytest.mark.parametrize("size", [(45, 45), (50, 38)], ids=["45x45", "50x38"])
+def test_odd_sizes_staging(gpu, oracle, size):
+    """ROI byte counts that are not multiples of 16: in one contiguous batch some ROIs start
+    16-byte aligned (read in 16-byte chunks plus a tail) and others do not (read pixel by
+    pixel); every format, with ClearFirstFourPixels, against the oracle."""
+    w = synth.make_workload("plain64", n=48, width=size[0], height=size[1])
+    scale = w["rois"] / w["rois"].max()
+    for dt, vmax in ((np.uint8, 255), (np.uint16, 60000), (np.float32, 1.0)):
+        rois = np.ascontiguousarray((scale * vmax).astype(dt))
+        for mode in (w["mode"], w["mode"] | api.LT_ClearFirstFourPixels):
+            ref, rbest, _ = _oracle(oracle, w, mode=mode, rois=rois)
+            trk = _tracker(w, mode=mode)
+            got, best, _ = _batch(trk, rois, w["jobs"])
+            _report(f"{size} {np.dtype(dt).name} mode {mode}", got, ref)
+            _assert_parity(got, ref, best, rbest)
+            trk.free()
+
+
 def test_mode_variants(gpu, oracle):
     w = synth.make_workload("plain64", n=128)
     for mode in (api.LT_OnlyCOM, api.LT_QI, api.LT_QI | api.LT_LocalizeZ, api.LT_LocalizeZ | api.LT_NormalizeProfile,
