@@ -479,55 +479,60 @@ __device__ __forceinline__ void fft_stages(float2* base, float2* tmp, int nbufs,
 	}
 }
 
-// The same stages for QI's power-of-two lengths (radices 4 and 2 only): the group and
-// twiddle indices are shifts and masks, and a wavefront takes as many transforms as give a
-// radix-4 stage 64 butterflies (2 of length 128, 4 of length 64), so the lanes are busy.
-// The butterflies, and so the rounding, are those of fft_stages.
-__device__ __forceinline__ void fft_pow2_stages(float2* base, int nbufs, const FFTPlan& F, const float2* tw,
-                                                float nii, int wid, int lane)
+// Orders the LDS accesses of one wavefront: what its lanes wrote before is seen by its
+// lanes after (a wavefront's LDS instructions execute in order; this keeps the compiler
+// from moving them across).
+__device__ __forceinline__ void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One radix-4 or radix-2 butterfly of kissfft's decimation in time (the same operations as
+// fft_stages): G is the group, k the butterfly, m the group's sub-length, k1 = k * fstride.
+__device__ __forceinline__ void bfly_pow2(float2* G, int r, int k, int m, int k1, const float2* tw, float nii)
+{
+	if (r == 4) {
+		const float2 s0 = cmul(G[k + m], tw[k1]);
+		const float2 s1 = cmul(G[k + 2 * m], tw[k1 << 1]);
+		const float2 s2 = cmul(G[k + 3 * m], tw[k1 + (k1 << 1)]);
+		float2 f0 = G[k];
+		const float2 s5 = csub(f0, s1);
+		f0 = cadd(f0, s1);
+		const float2 s3 = cadd(s0, s2);
+		float2 s4 = csub(s0, s2);
+		s4 = make_float2(s4.y * nii, -s4.x * nii);
+		G[k + 2 * m] = csub(f0, s3);
+		G[k] = cadd(f0, s3);
+		G[k + m] = cadd(s5, s4);
+		G[k + 3 * m] = csub(s5, s4);
+	} else {
+		const float2 tt = cmul(G[m + k], tw[k1]);
+		const float2 f0 = G[k];
+		G[m + k] = csub(f0, tt);
+		G[k] = cadd(f0, tt);
+	}
+}
+
+// One power-of-two transform (QI's lengths: radices 4 and 2, group and twiddle indices by
+// shifts and masks) by one wavefront in place, its stages ordered by wave_sync: no
+// workgroup barrier.  The butterflies, and so the rounding, are those of fft_stages.
+__device__ __forceinline__ void fft_pow2_wave(float2* X, const FFTPlan& F, const float2* tw, float nii, int lane)
 {
 	const int ln = __builtin_ctz(F.n);
-	const int lt = ln < 8 ? 8 - ln : 0;  // log2 transforms per wavefront
-	const int t0 = wid << lt;
 #pragma unroll
 	for (int si = 0; si < kMaxFFTStages; si++) {
 		const int s = kMaxFFTStages - 1 - si;
-		if (s >= F.nst) continue;  // uniform: every thread skips together
+		if (s >= F.nst) continue;  // uniform
 		const int r = F.radix[s];
 		const int lr = r == 4 ? 2 : 1, lm = __builtin_ctz(F.m[s]), lf = __builtin_ctz(F.fs[s]);
-		const int lb = ln - lr;  // log2 butterflies per transform
-		if (t0 < nbufs) {
-			const int nt = nbufs - t0 < (1 << lt) ? nbufs - t0 : (1 << lt);
-			const int total = nt << lb;
-			for (int e = lane; e < total; e += 64) {
-				const int t = e >> lb, b = e & ((1 << lb) - 1);
-				const int g = b >> lm, k = b & ((1 << lm) - 1);
-				const int m = 1 << lm;
-				float2* G = base + ((t0 + t) << ln) + (g << (lm + lr));
-				const int k1 = k << lf;
-				if (r == 4) {
-					const float2 s0 = cmul(G[k + m], tw[k1]);
-					const float2 s1 = cmul(G[k + 2 * m], tw[k1 << 1]);
-					const float2 s2 = cmul(G[k + 3 * m], tw[k1 + (k1 << 1)]);
-					float2 f0 = G[k];
-					const float2 s5 = csub(f0, s1);
-					f0 = cadd(f0, s1);
-					const float2 s3 = cadd(s0, s2);
-					float2 s4 = csub(s0, s2);
-					s4 = make_float2(s4.y * nii, -s4.x * nii);
-					G[k + 2 * m] = csub(f0, s3);
-					G[k] = cadd(f0, s3);
-					G[k + m] = cadd(s5, s4);
-					G[k + 3 * m] = csub(s5, s4);
-				} else {
-					const float2 tt = cmul(G[m + k], tw[k1]);
-					const float2 f0 = G[k];
-					G[m + k] = csub(f0, tt);
-					G[k] = cadd(f0, tt);
-				}
-			}
+		const int nb = 1 << (ln - lr);  // butterflies
+		for (int e = lane; e < nb; e += 64) {
+			const int g = e >> lm, k = e & ((1 << lm) - 1);
+			bfly_pow2(X + (g << (lm + lr)), r, k, 1 << lm, k << lf, tw, nii);
 		}
-		__syncthreads();
+		wave_sync();
 	}
 }
 
@@ -1100,7 +1105,8 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				}
 			}
 			__syncthreads();
-			// leaves of the 4 forward transforms: X, reversed X, Y, reversed Y
+			// wavefront w: the leaves of transform w (X, reversed X, Y, reversed Y) into buffer
+			// w, then its forward transform
 			{
 				float2* F = fft + wid * N;
 				const int axis = wid >> 1;
@@ -1109,36 +1115,36 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					if (wid & 1) s = N - 1 - s;
 					F[j] = make_float2(qi_input(qprof, nr, axis, s), 0.0f);
 				}
+				wave_sync();
+				fft_pow2_wave(F, P.qi_fft, tw, 1.0f, lane);
 			}
 			__syncthreads();
-			fft_pow2_stages(fft, 4, P.qi_fft, tw, 1.0f, wid, lane);
-			// F(p) * conj(F(rev p)) for X and Y, held in registers until every thread has
-			// read the forward spectra, then written in leaf order into buffers 0 and 1
-			// for the inverse transforms (2N / kBlock <= 4 values per thread)
-			{
-				float2 prod[4];
+			// wavefront 0 takes X (buffers 0, 1), wavefront 1 Y (buffers 2, 3): F(p) * conj(F(rev p))
+			// read into registers, written in leaf order into the first buffer, inverse
+			// transform there, then the peak
+			if (wid < 2) {
+				float2* A = fft + (2 * wid) * N;
+				const float2* B = A + N;
+				constexpr int KP = kMaxFFTLen / 64;
+				float2 prod[KP];
 #pragma unroll
-				for (int i = 0; i < 4; i++) {
-					const int t = tid + i * kBlock;
-					if (t < 2 * N) {
-						const int ax = t >= N ? 1 : 0;
-						const int s = perm[t - ax * N];
-						const float2 a = fft[(2 * ax) * N + s];
-						const float2 b = fft[(2 * ax + 1) * N + s];
-						prod[i] = cmul(a, make_float2(b.x, -b.y));
+				for (int i = 0; i < KP; i++) {
+					const int t = lane + 64 * i;
+					if (t < N) {
+						const int s = perm[t];
+						const float2 b = B[s];
+						prod[i] = cmul(A[s], make_float2(b.x, -b.y));
 					}
 				}
-				__syncthreads();
+				wave_sync();
 #pragma unroll
-				for (int i = 0; i < 4; i++) {
-					const int t = tid + i * kBlock;
-					if (t < 2 * N) fft[t] = prod[i];  // t < N: X into buffer 0, else Y into buffer 1
+				for (int i = 0; i < KP; i++) {
+					const int t = lane + 64 * i;
+					if (t < N) A[t] = prod[i];
 				}
-			}
-			__syncthreads();
-			fft_pow2_stages(fft, 2, P.qi_fft, tw + N, -1.0f, wid, lane);
-			if (wid < 2) {
-				const float2* G = fft + wid * N;
+				wave_sync();
+				fft_pow2_wave(A, P.qi_fft, tw + N, -1.0f, lane);
+				const float2* G = A;
 				float bv = 0.0f;
 				int bi = 0x7fffffff;
 				for (int x = lane; x < N; x += 64) {
