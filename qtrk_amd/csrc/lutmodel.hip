@@ -137,61 +137,83 @@ __device__ __forceinline__ PixelG pixel_g(const LutModelParams& P, int x, int y,
 // rounding: the hardware reciprocal square root replaces the IEEE square root
 // and division, the row segment (a, b - a, da, db - da) of each radial bin is
 // one 16-byte LDS read from a table built per iteration, the bin clamp is one
-// med3 (a sample is inside the LUT's radii exactly when the clamp leaves it
-// unchanged), the sub-sample offsets are hoisted per pixel, and products are
-// fused.  Agrees with pixel_g to float rounding.
+// med3 into that table, whose end entries have zero slope (so the r-derivative
+// of clamped samples is zero without a test), the sub-sample offsets are
+// hoisted per pixel, and products are fused.  Agrees with pixel_g to float
+// rounding (at r exactly on the last bin's edge the derivative is taken as the
+// clamped side's).
+// OVS, the sub-samples per pixel side, is a compile-time constant so that the loops
+// unroll without exits and the table reads of several sub-samples are in flight at once.
+template <int OVS>
 __device__ __forceinline__ PixelG pixel_g_fast(const LutModelParams& P, int x, int y, float px, float py,
                                                float radialcov, float step, const float4* row)
 {
-	constexpr int kMaxOvs = kMaxFitOvs;
-	const float rmax = (float)(P.steps - 2);
-	const float mr = P.minr * radialcov;
-	const float X0 = (float)x + 0.5f * step - px - 0.5f;
-	const float Y0 = (float)y + 0.5f * step - py - 0.5f;
-	const int ovs = P.ovs;
+	constexpr int kMaxOvs = OVS;
+	const float tmax = (float)(P.steps - 1);
+	const float mr1 = P.minr * radialcov - 1.0f;  // t = (r - minr) * radialcov + 1
+	// offsets in bin units (X' = X radialcov): r' = |(X', Y')| = r radialcov, and the slope
+	// term q.y radialcov / r times X is q.y / r' times X', scaled back once at the end
+	const float sstep = step * radialcov;
+	const float X0 = ((float)x + 0.5f * step - px - 0.5f) * radialcov;
+	const float Y0 = ((float)y + 0.5f * step - py - 0.5f) * radialcov;
+	constexpr int ovs = OVS;
 	float Ys[kMaxOvs], Y2[kMaxOvs];
 #pragma unroll
 	for (int o = 0; o < kMaxOvs; o++) {
-		Ys[o] = __builtin_fmaf((float)o, step, Y0);
-		Y2[o] = Ys[o] * Ys[o];
+		Ys[o] = __builtin_fmaf((float)o, sstep, Y0);
+		// + 1e-30 keeps r'^2 > 0 at the centre itself (its slope term is then 0 * finite)
+		Y2[o] = __builtin_fmaf(Ys[o], Ys[o], 1e-30f);
 	}
 	float sa = 0, sb = 0, za = 0, zb = 0, sx = 0, sy = 0;
-	for (int ox = 0; ox < ovs; ox++) {
-		const float X = __builtin_fmaf((float)ox, step, X0);
-		const float X2 = X * X;
 #pragma unroll
-		for (int oy = 0; oy < kMaxOvs; oy++) {
-			if (oy >= ovs) break;
-			const float r2 = fmaxf(X2 + Y2[oy], 1e-30f);  // the centre itself: slope term 0 * finite
-			const float rc = radialcov * __builtin_amdgcn_rsqf(r2);
-			const float ru = __builtin_fmaf(r2, rc, -mr);
-			const float r = __builtin_amdgcn_fmed3f(ru, 0.0f, rmax);
-			const int i = (int)r;
-			const float fr = __builtin_amdgcn_fractf(r);
-			const float4 q = row[i];
-			sa += q.x;
-			sb = __builtin_fmaf(q.y, fr, sb);
-			za += q.z;
-			zb = __builtin_fmaf(q.w, fr, zb);
-			const float k = ru == r ? q.y * rc : 0.0f;  // the slope in r is zero where r is clamped
+	for (int ox = 0; ox < ovs; ox++) {
+		const float X = __builtin_fmaf((float)ox, sstep, X0);
+		const float X2 = X * X;
+		// a column's table reads are all issued before the first is used
+		float rs[ovs], fr[ovs];
+		float4 q[ovs];
+#pragma unroll
+		for (int oy = 0; oy < ovs; oy++) {
+			const float r2 = X2 + Y2[oy];
+			rs[oy] = __builtin_amdgcn_rsqf(r2);
+			const float t = __builtin_amdgcn_fmed3f(__builtin_fmaf(r2, rs[oy], -mr1), 0.0f, tmax);
+			fr[oy] = __builtin_amdgcn_fractf(t);
+			q[oy] = row[(int)t];
+		}
+#pragma unroll
+		for (int oy = 0; oy < ovs; oy++) {
+			sa += q[oy].x;
+			sb = __builtin_fmaf(q[oy].y, fr[oy], sb);
+			za += q[oy].z;
+			zb = __builtin_fmaf(q[oy].w, fr[oy], zb);
+			const float k = q[oy].y * rs[oy];  // zero in the clamped entries
 			sx = __builtin_fmaf(-k, X, sx);
 			sy = __builtin_fmaf(-k, Ys[oy], sy);
 		}
 	}
 	const float inv = 1.0f / (float)(ovs * ovs);
+	const float dinv = inv * radialcov;
 	PixelG o;
 	o.g = (sa + sb) * inv;
-	o.gx = sx * inv;
-	o.gy = sy * inv;
+	o.gx = sx * dinv;
+	o.gy = sy * dinv;
 	o.gz = (za + zb) * inv;
 	return o;
 }
 
-// The fit's row table from zi and dzi (bins 0 .. steps-2; the clamp keeps i <= steps-2).
+// The fit's row table from zi and dzi, indexed by t = r + 1 clamped to [0, steps-1]:
+// entry 0 holds the radii below the LUT (zi[0], zero slope), entries 1 .. steps-2 the
+// bins 0 .. steps-3, entry steps-1 the radii at and beyond the last bin the clamp keeps
+// (zi[steps-2], zero slope).  Zero slopes make the clamped samples' r-derivative zero
+// without a test.
 __device__ __forceinline__ void build_row_table(const LutModelParams& P, const float* zi, const float* dzi, float4* row)
 {
-	for (int r = threadIdx.x; r < P.steps - 1; r += blockDim.x)
-		row[r] = make_float4(zi[r], zi[r + 1] - zi[r], dzi[r], dzi[r + 1] - dzi[r]);
+	const int S = P.steps;
+	for (int r = threadIdx.x; r < S; r += blockDim.x) {
+		if (r == 0) row[0] = make_float4(zi[0], 0.0f, dzi[0], 0.0f);
+		else if (r == S - 1) row[r] = make_float4(zi[S - 2], 0.0f, dzi[S - 2], 0.0f);
+		else row[r] = make_float4(zi[r - 1], zi[r] - zi[r - 1], dzi[r - 1], dzi[r] - dzi[r - 1]);
+	}
 }
 
 // ------------------------------------------------------------ forward / backward
@@ -336,6 +358,7 @@ __host__ __device__ constexpr int tri5(int i, int j) { return i * 5 - i * (i - 1
 // turn non-positive are halved instead; z stays within the LUT.  After the
 // last update one more pass evaluates F, its FP64 inverse (the CRLB) and the
 // negative log-likelihood at the fitted parameters.
+template <int OVS>
 __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -371,7 +394,7 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		float v[4] = { 0, 0, 0, 0 };
 		for (int p = threadIdx.x; p < npx; p += nt) {
 			const int y = p / P.width, x = p - y * P.width;
-			const PixelG G = pixel_g_fast(P, x, y, th[0], th[1], radialcov, step, row);
+			const PixelG G = pixel_g_fast<OVS>(P, x, y, th[0], th[1], radialcov, step, row);
 			const float d = dat[p];
 			v[0] += G.g;
 			v[1] += G.g * G.g;
@@ -404,7 +427,7 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		for (int k = 0; k < 21; k++) v[k] = 0;
 		for (int p = threadIdx.x; p < npx; p += nt) {
 			const int y = p / P.width, x = p - y * P.width;
-			const PixelG G = pixel_g_fast(P, x, y, x0, y0, radialcov, step, row);
+			const PixelG G = pixel_g_fast<OVS>(P, x, y, x0, y0, radialcov, step, row);
 			float mu = __builtin_fmaf(I, G.g, bg);
 			mu = mu > 1e-6f ? mu : 1e-6f;
 			const float d = dat[p];
@@ -510,16 +533,32 @@ hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream)
 	return hipGetLastError();
 }
 
+template <int OVS>
+static hipError_t launch_fit(const LutModelParams& p, size_t lds, hipStream_t stream, int block)
+{
+	static_assert(OVS >= 1 && OVS <= kMaxFitOvs, "sub-sample count");
+	hipError_t e = set_lds((const void*)lutmodel_fit_kernel<OVS>, lds);
+	if (e != hipSuccess) return e;
+	lutmodel_fit_kernel<OVS><<<p.n, block, lds, stream>>>(p);
+	return hipGetLastError();
+}
+
 hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int block)
 {
 	if (p.n <= 0) return hipSuccess;
 	if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
 	if (p.ovs < 1 || p.ovs > kMaxFitOvs || p.steps < 2) return hipErrorInvalidValue;
 	const size_t lds = lutmodel_lds_bytes(p, true);
-	hipError_t e = set_lds((const void*)lutmodel_fit_kernel, lds);
-	if (e != hipSuccess) return e;
-	lutmodel_fit_kernel<<<p.n, block, lds, stream>>>(p);
-	return hipGetLastError();
+	switch (p.ovs) {
+	case 1: return launch_fit<1>(p, lds, stream, block);
+	case 2: return launch_fit<2>(p, lds, stream, block);
+	case 3: return launch_fit<3>(p, lds, stream, block);
+	case 4: return launch_fit<4>(p, lds, stream, block);
+	case 5: return launch_fit<5>(p, lds, stream, block);
+	case 6: return launch_fit<6>(p, lds, stream, block);
+	case 7: return launch_fit<7>(p, lds, stream, block);
+	default: return launch_fit<8>(p, lds, stream, block);
+	}
 }
 
 }  // namespace qtrk
