@@ -164,7 +164,11 @@ __device__ __forceinline__ PixelG pixel_g_fast(const LutModelParams& P, int x, i
 		// + 1e-30 keeps r'^2 > 0 at the centre itself (its slope term is then 0 * finite)
 		Y2[o] = __builtin_fmaf(Ys[o], Ys[o], 1e-30f);
 	}
-	float sa = 0, sb = 0, za = 0, zb = 0, sx = 0, sy = 0;
+	// the slope sums sum_{ox,oy} k X_ox and sum k Y_oy are taken as sum_ox X_ox (sum_oy k)
+	// and sum_oy Y_oy (sum_ox k): per sub-sample two fused adds into the marginals
+	float sa = 0, sb = 0, za = 0, zb = 0, sx = 0, kr[ovs];
+#pragma unroll
+	for (int oy = 0; oy < ovs; oy++) kr[oy] = 0.0f;
 #pragma unroll
 	for (int ox = 0; ox < ovs; ox++) {
 		const float X = __builtin_fmaf((float)ox, sstep, X0);
@@ -180,17 +184,22 @@ __device__ __forceinline__ PixelG pixel_g_fast(const LutModelParams& P, int x, i
 			fr[oy] = __builtin_amdgcn_fractf(t);
 			q[oy] = row[(int)t];
 		}
+		float kc = 0.0f;
 #pragma unroll
 		for (int oy = 0; oy < ovs; oy++) {
 			sa += q[oy].x;
 			sb = __builtin_fmaf(q[oy].y, fr[oy], sb);
 			za += q[oy].z;
 			zb = __builtin_fmaf(q[oy].w, fr[oy], zb);
-			const float k = q[oy].y * rs[oy];  // zero in the clamped entries
-			sx = __builtin_fmaf(-k, X, sx);
-			sy = __builtin_fmaf(-k, Ys[oy], sy);
+			// k = q.y / r', zero in the clamped entries
+			kc = __builtin_fmaf(q[oy].y, rs[oy], kc);
+			kr[oy] = __builtin_fmaf(q[oy].y, rs[oy], kr[oy]);
 		}
+		sx = __builtin_fmaf(-kc, X, sx);
 	}
+	float sy = 0.0f;
+#pragma unroll
+	for (int oy = 0; oy < ovs; oy++) sy = __builtin_fmaf(-kr[oy], Ys[oy], sy);
 	const float inv = 1.0f / (float)(ovs * ovs);
 	const float dinv = inv * radialcov;
 	PixelG o;
@@ -383,6 +392,10 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 	const float* lut = P.luts + (size_t)lut_of(P, roi) * P.planes * P.steps;
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;
+	// pixel p = y * width + x stepped by the block size without a division per pixel
+	// (nt <= 256, so one wrap per step needs dxp < width: dxp = nt mod width)
+	const int y1 = (int)threadIdx.x / P.width, x1 = (int)threadIdx.x - y1 * P.width;
+	const int dyp = nt / P.width, dxp = nt - dyp * P.width;
 	__syncthreads();
 
 	if (!(th[3] > 0)) {
@@ -392,8 +405,9 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		build_row_table(P, zi, dzi, row);
 		__syncthreads();
 		float v[4] = { 0, 0, 0, 0 };
-		for (int p = threadIdx.x; p < npx; p += nt) {
-			const int y = p / P.width, x = p - y * P.width;
+		int x = x1, y = y1;
+		for (int p = threadIdx.x; p < npx; p += nt, x += dxp, y += dyp) {
+			if (x >= P.width) { x -= P.width; y++; }
 			const PixelG G = pixel_g_fast<OVS>(P, x, y, th[0], th[1], radialcov, step, row);
 			const float d = dat[p];
 			v[0] += G.g;
@@ -425,8 +439,9 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		float v[21];
 #pragma unroll
 		for (int k = 0; k < 21; k++) v[k] = 0;
-		for (int p = threadIdx.x; p < npx; p += nt) {
-			const int y = p / P.width, x = p - y * P.width;
+		int x = x1, y = y1;
+		for (int p = threadIdx.x; p < npx; p += nt, x += dxp, y += dyp) {
+			if (x >= P.width) { x -= P.width; y++; }
 			const PixelG G = pixel_g_fast<OVS>(P, x, y, x0, y0, radialcov, step, row);
 			float mu = __builtin_fmaf(I, G.g, bg);
 			mu = mu > 1e-6f ? mu : 1e-6f;
