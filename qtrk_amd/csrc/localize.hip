@@ -40,11 +40,16 @@ struct LdsLayout {
 	int adir, aprof, alut, afft, atw, aperm, atmp;  // QuadrantAlign phase
 	int fbuf, fcol, ftmp, ftwx, ftwy, fpx, fpy;     // 2D power-spectrum phase
 	int scores, zw;                               // LUT phase (zw: adjusted weights)
+	int gcol;                                     // 2D Gaussian MLE: per-column and per-row terms
 	int total;
 };
 
 // Optional modes the parameters ask for (never in the LUT-build variant).
 __host__ __device__ inline bool mode_xcor(const LocalizeParams& P) { return !P.build && (P.mode & kModeXCor1D); }
+__host__ __device__ inline bool mode_gauss(const LocalizeParams& P)
+{
+	return !P.build && !(P.mode & kModeXCor1D) && !(P.mode & kModeQI) && (P.mode & kModeGauss2D);
+}
 __host__ __device__ inline bool mode_align(const LocalizeParams& P)
 {
 	return !P.build && (P.mode & kModeZLUTAlign) && (P.mode & kModeLocalizeZ);
@@ -134,6 +139,11 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 		L.fpx = f;    f += align16(W * 4);
 		L.fpy = f;    f += align16(H * 4);
 	}
+	// 2D Gaussian MLE phase: 4 terms per column, 3 per row
+	int g = u;
+	if (mode_gauss(P)) {
+		L.gcol = g;   g += align16((4 * P.width + 3 * P.height) * 4);
+	}
 	// LUT phase
 	L.scores = u;
 	int s = u + align16(P.zplanes * 8);
@@ -144,6 +154,7 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 	o = o > x ? o : x;
 	o = o > a ? o : a;
 	o = o > f ? o : f;
+	o = o > g ? o : g;
 	L.total = o;
 	return L;
 }
@@ -359,12 +370,15 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i)
 	}
 }
 
-// Sum of K doubles over the workgroup; result in red[32..32+K) for all threads.
+// Sum of K doubles over the workgroup; result in red[32..32+K) for all threads.  The K
+// butterfly reductions advance level by level, so their shuffles are in flight together.
 template <int K>
 __device__ __forceinline__ void block_sum(double (&v)[K], double* red, int lane, int wid)
 {
-	for (int k = 0; k < K; k++)
-		for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+		for (int k = 0; k < K; k++) v[k] += __shfl_xor(v[k], off);
 	if (lane == 0)
 		for (int k = 0; k < K; k++) red[wid * K + k] = v[k];
 	__syncthreads();
@@ -1170,7 +1184,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	// ---- 2D Gaussian MLE when QI is off (cputrack/cpu_tracker.cpp:514-589, selected at
 	// cputrack/QueuedCPUTracker.cpp:309-313): pixel-integrated Gaussian, independent Newton
 	// steps for x, y, I0 and bg, float model terms summed in double over the ROI.
-	if (!build && !xcor && !(P.mode & kModeQI) && (P.mode & kModeGauss2D)) {
+	if (mode_gauss(P)) {
 		const float sigma = P.gauss_sigma;
 		float px = comx, py = comy;
 		float I0 = mean * 0.5f * (float)W * (float)H;
@@ -1178,35 +1192,60 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		const float s2 = 1.0f / (sqrtf(2.0f) * sigma);
 		const float s2pi = 1.0f / (sqrtf(2 * 3.14159265359f) * sigma);
 		const float s2pi3 = 1.0f / (sqrtf(2 * 3.14159265359f) * sigma * sigma * sigma);
+		// The terms that depend on x alone (or y alone) are computed once per column (row)
+		// and iteration: D = 0.5 erf(e0) - 0.5 erf(e1), C1 = (I0 s2pi)(exp(-e1^2) - exp(-e0^2)),
+		// C2 = (I0 s2pi3)((c - p - 0.5) exp(-e1^2) - (c - p + 0.5) exp(-e0^2)), and I0 D for
+		// columns; each pixel then forms the reference's left-to-right products from them
+		// (I0 DX DY = (I0 DX) DY, I0 s2pi (eX1 - eX0) DY = C1x DY, ...), so its terms round as
+		// the per-pixel formulas do.  Pixels are spread over all threads.
+		float* gx = (float*)(lds + L.gcol);  // [4][W]: DX, C1x, C2x, I0 DX
+		float* gy = gx + 4 * W;              // [3][H]: DY, C1y, C2y
+		const int y1 = tid / W, x1 = tid - y1 * W, dyp = kBlock / W, dxp = kBlock - dyp * W;
 		for (int it = 0; it < P.gauss_iters; it++) {
-			double acc[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
-			for (int y = wid; y < H; y += kBlock / 64) {
-				const float Ye0 = ((float)y - py + 0.5f) * s2;
-				const float Ye1 = ((float)y - py - 0.5f) * s2;
-				const float DY = 0.5f * erff(Ye0) - 0.5f * erff(Ye1);
-				const float eY1 = expf(-Ye1 * Ye1), eY0 = expf(-Ye0 * Ye0);
-				for (int x = lane; x < W; x += 64) {
-					const float Xe0 = ((float)x - px + 0.5f) * s2;
-					const float Xe1 = ((float)x - px - 0.5f) * s2;
-					const float DX = 0.5f * erff(Xe0) - 0.5f * erff(Xe1);
-					const float mu = bg + I0 * DX * DY;
-					const float eX1 = expf(-Xe1 * Xe1), eX0 = expf(-Xe0 * Xe0);
-					const float dmx = I0 * s2pi * (eX1 - eX0) * DY;
-					const float dmy = I0 * s2pi * (eY1 - eY0) * DX;
-					const float dmI = DX * DY;
-					const float smp = (float)img[y * W + x];
-					const float f = smp / mu - 1;
-					acc[0] += (double)(dmx * f);
-					acc[1] += (double)(dmy * f);
-					acc[2] += (double)(dmI * f);
-					acc[3] += (double)f;
-					const float d2x = I0 * s2pi3 * (((float)x - px - 0.5f) * eX1 - ((float)x - px + 0.5f) * eX0) * DY;
-					const float d2y = I0 * s2pi3 * (((float)y - py - 0.5f) * eY1 - ((float)y - py + 0.5f) * eY0) * DX;
-					acc[4] += (double)(d2x * f - dmx * dmx * smp / (mu * mu));
-					acc[5] += (double)(d2y * f - dmy * dmy * smp / (mu * mu));
-					acc[6] += (double)(-dmI * dmI * smp / (mu * mu));
-					acc[7] += (double)(-smp / (mu * mu));
+			const float Is = I0 * s2pi, Is3 = I0 * s2pi3;
+			for (int c = tid; c < W + H; c += kBlock) {
+				const bool row = c >= W;
+				const int i = row ? c - W : c;
+				const float q = row ? py : px;
+				const float e0 = ((float)i - q + 0.5f) * s2;
+				const float e1 = ((float)i - q - 0.5f) * s2;
+				const float D = 0.5f * erff(e0) - 0.5f * erff(e1);
+				const float x1e = expf(-e1 * e1), x0e = expf(-e0 * e0);
+				const float C1 = Is * (x1e - x0e);
+				const float C2 = Is3 * (((float)i - q - 0.5f) * x1e - ((float)i - q + 0.5f) * x0e);
+				if (row) {
+					gy[i] = D;
+					gy[H + i] = C1;
+					gy[2 * H + i] = C2;
+				} else {
+					gx[i] = D;
+					gx[W + i] = C1;
+					gx[2 * W + i] = C2;
+					gx[3 * W + i] = I0 * D;
 				}
+			}
+			__syncthreads();
+			double acc[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+			int x = x1, y = y1;
+			for (int p = tid; p < npx; p += kBlock, x += dxp, y += dyp) {
+				if (x >= W) { x -= W; y++; }
+				const float DX = gx[x], DY = gy[y];
+				const float mu = bg + gx[3 * W + x] * DY;
+				const float dmx = gx[W + x] * DY;
+				const float dmy = gy[H + y] * DX;
+				const float dmI = DX * DY;
+				const float smp = (float)img[p];
+				const float f = smp / mu - 1;
+				acc[0] += (double)(dmx * f);
+				acc[1] += (double)(dmy * f);
+				acc[2] += (double)(dmI * f);
+				acc[3] += (double)f;
+				const float d2x = gx[2 * W + x] * DY;
+				const float d2y = gy[2 * H + y] * DX;
+				acc[4] += (double)(d2x * f - dmx * dmx * smp / (mu * mu));
+				acc[5] += (double)(d2y * f - dmy * dmy * smp / (mu * mu));
+				acc[6] += (double)(-dmI * dmI * smp / (mu * mu));
+				acc[7] += (double)(-smp / (mu * mu));
 			}
 			block_sum<8>(acc, red, lane, wid);
 			px = (float)((double)px - red[32] / red[36]);
