@@ -1508,6 +1508,184 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	}
 }
 
+// ---- 2D Gaussian MLE alone on small ROIs: one wavefront per ROI
+//
+// The same computation as localize_kernel's staging, mean/COM and Gaussian MLE phases
+// (cputrack/QueuedCPUTracker.cpp:258-313, cputrack/cpu_tracker.cpp:514-589, 681-729)
+// with every sum kept inside the wavefront: each lane sums its pixels (p = lane + 64 k),
+// then a butterfly over the wavefront.  No workgroup barrier, and a fourth of the
+// shuffles per ROI of the four-wavefront form.  The double sums group the pixels
+// differently from the workgroup form (exact for integer-valued pixels).
+constexpr int kGaussWaveMaxPx = 1024;
+
+__host__ __device__ inline int gauss_wave_lds_floats(const LocalizeParams& P)
+{
+	return (P.width * P.height + 4 * P.width + 3 * P.height + 3) & ~3;
+}
+inline bool gauss_wave_eligible(const LocalizeParams& P)
+{
+	return mode_gauss(P) && !(P.mode & kModeLocalizeZ) && P.width * P.height <= kGaussWaveMaxPx && P.width >= 2 &&
+	       P.height >= 2 && P.pdt >= 0 && P.pdt <= 2;
+}
+
+template <int K>
+__device__ __forceinline__ void wave_sum(double (&v)[K])
+{
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+		for (int k = 0; k < K; k++) v[k] += __shfl_xor(v[k], off);
+}
+
+__global__ void __launch_bounds__(kBlock) gauss_wave_kernel(LocalizeParams P)
+{
+	extern __shared__ __align__(16) unsigned char lds[];
+	const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+	const int roi = blockIdx.x * (kBlock / 64) + wid;
+	if (roi >= P.n) return;  // whole wavefronts; nothing below waits on other wavefronts
+	const int W = P.width, H = P.height, npx = W * H;
+	float* img = (float*)lds + (size_t)wid * gauss_wave_lds_floats(P);
+	float* gx = img + npx;     // [4][W]: DX, C1x, C2x, I0 DX
+	float* gy = gx + 4 * W;    // [3][H]: DY, C1y, C2y
+	const DevJob job = P.jobs ? P.jobs[roi] : DevJob{};
+	const int bead = job.zlutIndex;
+
+	// ROI -> LDS as float, ClearFirstFourPixels, offset/gain with unit factors
+	{
+		const uint8_t* base = P.src + (P.roi_offset ? P.roi_offset[roi] : (int64_t)roi * H * P.pitch);
+		const bool calib = (P.calib_offset || P.calib_gain) && bead >= 0 && bead < P.zcount;
+		const float* off = calib && P.calib_offset ? P.calib_offset + (size_t)bead * npx : nullptr;
+		const float* gain = calib && P.calib_gain ? P.calib_gain + (size_t)bead * npx : nullptr;
+		const bool clear4 = (P.mode & kModeClearFirstFour) != 0;
+		const int y1 = lane / W, x1 = lane - y1 * W, dyp = 64 / W, dxp = 64 - dyp * W;
+		int x = x1, y = y1;
+		for (int i = lane; i < npx; i += 64, x += dxp, y += dyp) {
+			if (x >= W) { x -= W; y++; }
+			const uint8_t* row = base + (int64_t)y * P.pitch;
+			float v;
+			if (P.pdt == 1) v = (float)((const uint16_t*)row)[x];
+			else if (P.pdt == 0) v = (float)row[x];
+			else v = ((const float*)row)[x];
+			if (clear4 && i < 4) v = 0.0f;
+			if (off && !gain) v = v + off[i] * 1.0f;
+			else if (gain && !off) v = v * gain[i] * 1.0f;
+			else if (gain && off) v = (v + off[i] * 1.0f) * gain[i] * 1.0f;
+			img[i] = v;
+		}
+	}
+	wave_sync();
+
+	// mean, stdev and background-corrected COM
+	double m1[2] = { 0.0, 0.0 };
+	for (int i = lane; i < npx; i += 64) {
+		const float v = img[i];
+		m1[0] += (double)v;
+		m1[1] += (double)(float)(v * v);
+	}
+	wave_sum<2>(m1);
+	const float invN = 1.0f / (float)npx;
+	const float mean = (float)(m1[0] * (double)invN);
+	const double var = m1[1] * (double)invN - (double)(mean * mean);
+	const float stdev = sqrtf((float)var);
+	const float bgs = P.com_bgcorrection * stdev;
+	const int y1 = lane / W, x1 = lane - y1 * W, dyp = 64 / W, dxp = 64 - dyp * W;
+	double m2[3] = { 0.0, 0.0, 0.0 };
+	{
+		int x = x1, y = y1;
+		for (int i = lane; i < npx; i += 64, x += dxp, y += dyp) {
+			if (x >= W) { x -= W; y++; }
+			const float d = fabsf(img[i] - mean) - bgs;
+			const float v = (0.0f < d) ? d : 0.0f;
+			m2[0] += (double)v;
+			m2[1] += (double)x * (double)v;
+			m2[2] += (double)y * (double)v;
+		}
+	}
+	wave_sum<3>(m2);
+	float comx = (float)(m2[1] / m2[0]);
+	float comy = (float)(m2[2] / m2[0]);
+	if (isnan(comx) || isnan(comy)) {
+		comx = (float)(W / 2);
+		comy = (float)(H / 2);
+	}
+
+	// Gaussian MLE, as in localize_kernel: per-column and per-row terms, then per pixel
+	const float sigma = P.gauss_sigma;
+	float px = comx, py = comy;
+	float I0 = mean * 0.5f * (float)W * (float)H;
+	float bg = mean * 0.5f;
+	const float s2 = 1.0f / (sqrtf(2.0f) * sigma);
+	const float s2pi = 1.0f / (sqrtf(2 * 3.14159265359f) * sigma);
+	const float s2pi3 = 1.0f / (sqrtf(2 * 3.14159265359f) * sigma * sigma * sigma);
+	for (int it = 0; it < P.gauss_iters; it++) {
+		const float Is = I0 * s2pi, Is3 = I0 * s2pi3;
+		for (int c = lane; c < W + H; c += 64) {
+			const bool row = c >= W;
+			const int i = row ? c - W : c;
+			const float q = row ? py : px;
+			const float e0 = ((float)i - q + 0.5f) * s2;
+			const float e1 = ((float)i - q - 0.5f) * s2;
+			const float D = 0.5f * erff(e0) - 0.5f * erff(e1);
+			const float x1e = expf(-e1 * e1), x0e = expf(-e0 * e0);
+			const float C1 = Is * (x1e - x0e);
+			const float C2 = Is3 * (((float)i - q - 0.5f) * x1e - ((float)i - q + 0.5f) * x0e);
+			if (row) {
+				gy[i] = D;
+				gy[H + i] = C1;
+				gy[2 * H + i] = C2;
+			} else {
+				gx[i] = D;
+				gx[W + i] = C1;
+				gx[2 * W + i] = C2;
+				gx[3 * W + i] = I0 * D;
+			}
+		}
+		wave_sync();
+		double acc[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+		int x = x1, y = y1;
+		for (int p = lane; p < npx; p += 64, x += dxp, y += dyp) {
+			if (x >= W) { x -= W; y++; }
+			const float DX = gx[x], DY = gy[y];
+			const float mu = bg + gx[3 * W + x] * DY;
+			const float dmx = gx[W + x] * DY;
+			const float dmy = gy[H + y] * DX;
+			const float dmI = DX * DY;
+			const float smp = img[p];
+			const float f = smp / mu - 1;
+			acc[0] += (double)(dmx * f);
+			acc[1] += (double)(dmy * f);
+			acc[2] += (double)(dmI * f);
+			acc[3] += (double)f;
+			const float d2x = gx[2 * W + x] * DY;
+			const float d2y = gy[2 * H + y] * DX;
+			acc[4] += (double)(d2x * f - dmx * dmx * smp / (mu * mu));
+			acc[5] += (double)(d2y * f - dmy * dmy * smp / (mu * mu));
+			acc[6] += (double)(-dmI * dmI * smp / (mu * mu));
+			acc[7] += (double)(-smp / (mu * mu));
+		}
+		wave_sum<8>(acc);
+		px = (float)((double)px - acc[0] / acc[4]);
+		py = (float)((double)py - acc[1] / acc[5]);
+		I0 = (float)((double)I0 - acc[2] / acc[6]);
+		bg = (float)((double)bg - acc[3] / acc[7]);
+		wave_sync();  // every lane has read this iteration's terms before they are rewritten
+	}
+
+	if (lane == 0) {
+		if (P.best_out) P.best_out[roi] = -1;
+		DevResult R;
+		R.job = job;
+		R.x = px;
+		R.y = py;
+		R.z = 0.0f;
+		R.comx = comx;
+		R.comy = comy;
+		R.error = 0u;
+		R.imageMean = mean;
+		P.results[roi] = R;
+	}
+}
+
 template <typename SPix, bool EXT>
 static void launch_variant(const LocalizeParams& p, size_t lds, hipStream_t stream)
 {
@@ -1529,6 +1707,11 @@ hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
 	if (ext && mode_align(p) && (p.qa_fft.n != 2 * p.z_nb || p.qa_fft.n > kMaxFFTLen)) return hipErrorInvalidValue;
 	if (ext && mode_fourier(p) && (p.lds_pix != 2 || p.fx_fft.n != p.width || p.fy_fft.n != p.height))
 		return hipErrorInvalidValue;
+	if (!p.build && gauss_wave_eligible(p)) {
+		const size_t wl = (size_t)(kBlock / 64) * gauss_wave_lds_floats(p) * sizeof(float);
+		hipLaunchKernelGGL(gauss_wave_kernel, dim3((p.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), wl, stream, p);
+		return hipGetLastError();
+	}
 	if (p.lds_pix == 0) ext ? launch_variant<uint8_t, true>(p, lds, stream) : launch_variant<uint8_t, false>(p, lds, stream);
 	else if (p.lds_pix == 1) ext ? launch_variant<uint16_t, true>(p, lds, stream) : launch_variant<uint16_t, false>(p, lds, stream);
 	else ext ? launch_variant<float, true>(p, lds, stream) : launch_variant<float, false>(p, lds, stream);
