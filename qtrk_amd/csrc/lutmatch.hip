@@ -1,7 +1,7 @@
 // lutmatch.hip -- grouped radial Z-LUT matching for gfx950 (exact path and
 // BF16/FP16 matrix-core path; the scoring rule is stated in lutmatch.hpp).
 //
-// Matrix-core path, one workgroup (4 wavefronts) per bead and 64 profiles:
+// Matrix-core path, one workgroup (4 wavefronts) per bead and 32 (or 64) profiles:
 //   * the bead's profiles are staged in LDS (float) and turned into MFMA
 //     B operands once: B[k=r][n=profile] = w_r^2 * prof[r], rounded to BF16/FP16;
 //   * the LUT was split once (launch_lutmatch_prepare) into FP32 tile means and
@@ -637,7 +637,7 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 	}
 	if (p.res > kMatchMaxRes) return hipErrorInvalidValue;
 	if (frame_tile != 0 && frame_tile != 32 && frame_tile != 64) return hipErrorInvalidValue;
-	const int FT = frame_tile ? frame_tile / 32 : (max_per_bead > 32 ? 2 : 1);
+	const int FT = frame_tile ? frame_tile / 32 : 1;
 	q.chunks = (max_per_bead + FT * 32 - 1) / (FT * 32);
 	const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
 	if (prec == kMatchBF16) return launch_mfma_prec<kMatchBF16>(q, FT, grid, stream);
@@ -648,7 +648,7 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile)
 {
 	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes).total;
-	const int FT = frame_tile ? frame_tile / 32 : (max_per_bead > 32 ? 2 : 1);
+	const int FT = frame_tile ? frame_tile / 32 : 1;
 	return (size_t)mfma_lds(FT, res, ksteps).total;
 }
 

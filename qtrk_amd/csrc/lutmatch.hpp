@@ -63,7 +63,9 @@ hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* fra
                                    float* raw_mean, hipStream_t stream);
 // One grouped match of all beads' profiles (prec: kMatchFP32 / BF16 / FP16);
 // max_per_bead bounds start[b+1] - start[b].  frame_tile: profiles per
-// matrix-core workgroup, 32 or 64 (0: 64 when a bead has more than 32).
+// matrix-core workgroup, 32 or 64 (0: 32, which with half the registers and
+// LDS of 64 holds more workgroups per CU and was faster on every shape measured,
+// although a bead's LUT tiles are then read once per 32 profiles).
 hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream,
                            int frame_tile = 0);
 // Dynamic LDS bytes of that launch.
