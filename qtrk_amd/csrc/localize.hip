@@ -1686,16 +1686,18 @@ __global__ void __launch_bounds__(kBlock) gauss_wave_kernel(LocalizeParams P)
 	}
 }
 
+// Launches above the default 64 KB of dynamic LDS raise the kernel's limit first (on the
+// current device, every time: the setting is per device).
 template <typename SPix, bool EXT>
-static void launch_variant(const LocalizeParams& p, size_t lds, hipStream_t stream)
+static hipError_t launch_variant(const LocalizeParams& p, size_t lds, hipStream_t stream)
 {
-	static bool attr_done = false;
-	if (!attr_done) {
-		hipFuncSetAttribute((const void*)localize_kernel<SPix, EXT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-		                    160 * 1024);
-		attr_done = true;
+	if (lds > 64 * 1024) {
+		const hipError_t e = hipFuncSetAttribute((const void*)localize_kernel<SPix, EXT>,
+		                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+		if (e != hipSuccess) return e;
 	}
 	hipLaunchKernelGGL((localize_kernel<SPix, EXT>), dim3(p.n), dim3(kBlock), lds, stream, p);
+	return hipGetLastError();
 }
 
 hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
@@ -1712,10 +1714,10 @@ hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
 		hipLaunchKernelGGL(gauss_wave_kernel, dim3((p.n + kBlock / 64 - 1) / (kBlock / 64)), dim3(kBlock), wl, stream, p);
 		return hipGetLastError();
 	}
-	if (p.lds_pix == 0) ext ? launch_variant<uint8_t, true>(p, lds, stream) : launch_variant<uint8_t, false>(p, lds, stream);
-	else if (p.lds_pix == 1) ext ? launch_variant<uint16_t, true>(p, lds, stream) : launch_variant<uint16_t, false>(p, lds, stream);
-	else ext ? launch_variant<float, true>(p, lds, stream) : launch_variant<float, false>(p, lds, stream);
-	return hipGetLastError();
+	if (lds > 160 * 1024) return hipErrorInvalidValue;
+	if (p.lds_pix == 0) return ext ? launch_variant<uint8_t, true>(p, lds, stream) : launch_variant<uint8_t, false>(p, lds, stream);
+	if (p.lds_pix == 1) return ext ? launch_variant<uint16_t, true>(p, lds, stream) : launch_variant<uint16_t, false>(p, lds, stream);
+	return ext ? launch_variant<float, true>(p, lds, stream) : launch_variant<float, false>(p, lds, stream);
 }
 
 // NormalizeRadialProfile on each row (FinalizeLUT, cputrack/QueuedCPUTracker.cpp:648-659).
