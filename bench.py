@@ -277,6 +277,25 @@ def mle_bench(a):
     # those (|error| > 6 CRLB in any parameter) and give the precision of the others
     inl = np.all(np.abs(e) <= 6, axis=1)
     flop = n * size * size * MLE_FLOP_PER_PX_PASS * (iters + 1)
+
+    # the differentiable renderer on the same ROIs: forward (mu) and adjoint (d loss / d theta from
+    # d loss / d mu), the QTrkLUTModelForward / QTrkLUTModelBackward pair
+    gmu = torch.ones_like(d)
+    model.forward(t0)
+    model.backward(t0, gmu)
+    torch.cuda.synchronize()
+    ev0.record()
+    for _ in range(a.steps):
+        model.forward(t0)
+    ev1.record()
+    torch.cuda.synchronize()
+    render_ms = ev0.elapsed_time(ev1) / a.steps
+    ev0.record()
+    for _ in range(a.steps):
+        model.backward(t0, gmu)
+    ev1.record()
+    torch.cuda.synchronize()
+    adjoint_ms = ev0.elapsed_time(ev1) / a.steps
     model.free()
 
     # 2D Gaussian MLE through the tracker (LT_Gaussian2D), 65,536 ROIs of 16x16
@@ -313,6 +332,8 @@ def mle_bench(a):
                    "mean_error_over_crlb": dict(zip("x y z I bg".split(), np.round(e[inl].mean(0), 4).tolist())),
                    "rois_with_crlb": int(ok.sum()), "outliers_beyond_6_crlb": int((~inl).sum()),
                    "std_error_over_crlb_all": dict(zip("x y z I bg".split(), np.round(e.std(0), 3).tolist())),
+                   "renderer": {"forward_ms": round(render_ms, 4), "adjoint_ms": round(adjoint_ms, 4),
+                                "note": f"{n} ROIs of 48x48, 4x4 sub-samples per pixel, reference rounding"},
                    "gaussian2d_16x16": {"rois": len(g["jobs"]), "rois_per_s": round(len(g["jobs"]) * a.steps / g_wall, 1),
                                         "kernel_ms": round(g_ms, 4), "rms_xy_px_finite": round(g_rms, 4),
                                         "nan_results": int((~gfin).sum()),

@@ -94,14 +94,19 @@ __device__ __forceinline__ int lut_of(const LutModelParams& P, int roi)
 struct PixelG {
 	float g, gx, gy, gz;
 };
-template <bool DERIV>
+// OVS > 0: the sub-sample count as a compile-time constant (a column's sub-samples
+// unroll, their LUT reads in flight together); 0: P.ovs at run time.
+template <bool DERIV, int OVS>
 __device__ __forceinline__ PixelG pixel_g(const LutModelParams& P, int x, int y, float px, float py, float radialcov,
                                           float step, const float* zi, const float* dzi)
 {
 	float s = 0, sx = 0, sy = 0, sz = 0;
 	const float rmax = (float)(P.steps - 2);
-	for (int ox = 0; ox < P.ovs; ox++) {
-		for (int oy = 0; oy < P.ovs; oy++) {
+	const int ovs = OVS > 0 ? OVS : P.ovs;
+#pragma unroll 1
+	for (int ox = 0; ox < ovs; ox++) {  // one column's sub-samples unrolled
+#pragma unroll
+		for (int oy = 0; oy < ovs; oy++) {
 			const float X = x + (ox + 0.5f) * step - px - 0.5f;
 			const float Y = y + (oy + 0.5f) * step - py - 0.5f;
 			const float pr = sqrtf(X * X + Y * Y);
@@ -124,9 +129,9 @@ __device__ __forceinline__ PixelG pixel_g(const LutModelParams& P, int x, int y,
 			}
 		}
 	}
-	const float inv = 1.0f / (float)(P.ovs * P.ovs);
+	const float inv = 1.0f / (float)(ovs * ovs);
 	PixelG o;
-	o.g = s / (float)(P.ovs * P.ovs);
+	o.g = s / (float)(ovs * ovs);
 	o.gx = sx * inv;
 	o.gy = sy * inv;
 	o.gz = sz * inv;
@@ -227,7 +232,7 @@ __device__ __forceinline__ void build_row_table(const LutModelParams& P, const f
 
 // ------------------------------------------------------------ forward / backward
 
-template <bool JAC>
+template <bool JAC, int OVS>
 __global__ __launch_bounds__(kModelBlock) void lutmodel_forward_kernel(LutModelParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -242,9 +247,12 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_forward_kernel(LutModelP
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;
 	const int npx = P.width * P.height;
-	for (int p = threadIdx.x; p < npx; p += kModelBlock) {
-		const int y = p / P.width, x = p - y * P.width;
-		const PixelG G = pixel_g<JAC>(P, x, y, x0, y0, radialcov, step, zi, dzi);
+	// pixel p = y * width + x, stepped by the block size without a division per pixel
+	const int dyp = kModelBlock / P.width, dxp = kModelBlock - dyp * P.width;
+	int y = (int)threadIdx.x / P.width, x = (int)threadIdx.x - y * P.width;
+	for (int p = threadIdx.x; p < npx; p += kModelBlock, x += dxp, y += dyp) {
+		if (x >= P.width) { x -= P.width; y++; }
+		const PixelG G = pixel_g<JAC, OVS>(P, x, y, x0, y0, radialcov, step, zi, dzi);
 		if (P.mu) P.mu[(size_t)roi * npx + p] = bg + I * G.g;
 		if (JAC) {
 			float* J = P.jac + (size_t)roi * kModelParams * npx + p;
@@ -282,6 +290,7 @@ __device__ __forceinline__ void block_sum_d(const float (&v)[K], double* red)
 	__syncthreads();
 }
 
+template <int OVS>
 __global__ __launch_bounds__(kModelBlock) void lutmodel_backward_kernel(LutModelParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -298,9 +307,11 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_backward_kernel(LutModel
 	const float step = 1.0f / P.ovs;
 	const int npx = P.width * P.height;
 	float acc[kModelParams] = { 0, 0, 0, 0, 0 };
-	for (int p = threadIdx.x; p < npx; p += kModelBlock) {
-		const int y = p / P.width, x = p - y * P.width;
-		const PixelG G = pixel_g<true>(P, x, y, x0, y0, radialcov, step, zi, dzi);
+	const int dyp = kModelBlock / P.width, dxp = kModelBlock - dyp * P.width;
+	int y = (int)threadIdx.x / P.width, x = (int)threadIdx.x - y * P.width;
+	for (int p = threadIdx.x; p < npx; p += kModelBlock, x += dxp, y += dyp) {
+		if (x >= P.width) { x -= P.width; y++; }
+		const PixelG G = pixel_g<true, OVS>(P, x, y, x0, y0, radialcov, step, zi, dzi);
 		const float gm = P.gmu[(size_t)roi * npx + p];
 		acc[0] += gm * I * G.gx;
 		acc[1] += gm * I * G.gy;
@@ -522,19 +533,44 @@ static hipError_t set_lds(const void* k, size_t lds)
 	return hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 }
 
+template <bool JAC, int OVS>
+static hipError_t launch_forward(const LutModelParams& p, size_t lds, hipStream_t stream)
+{
+	hipError_t e = set_lds((const void*)lutmodel_forward_kernel<JAC, OVS>, lds);
+	if (e != hipSuccess) return e;
+	lutmodel_forward_kernel<JAC, OVS><<<p.n, kModelBlock, lds, stream>>>(p);
+	return hipGetLastError();
+}
+
+template <bool JAC>
+static hipError_t launch_forward_ovs(const LutModelParams& p, size_t lds, hipStream_t stream)
+{
+	switch (p.ovs) {
+	case 1: return launch_forward<JAC, 1>(p, lds, stream);
+	case 2: return launch_forward<JAC, 2>(p, lds, stream);
+	case 3: return launch_forward<JAC, 3>(p, lds, stream);
+	case 4: return launch_forward<JAC, 4>(p, lds, stream);
+	case 5: return launch_forward<JAC, 5>(p, lds, stream);
+	case 6: return launch_forward<JAC, 6>(p, lds, stream);
+	case 7: return launch_forward<JAC, 7>(p, lds, stream);
+	case 8: return launch_forward<JAC, 8>(p, lds, stream);
+	default: return launch_forward<JAC, 0>(p, lds, stream);
+	}
+}
+
 hipError_t launch_lutmodel_forward(const LutModelParams& p, hipStream_t stream)
 {
 	if (p.n <= 0) return hipSuccess;
 	const size_t lds = lutmodel_lds_bytes(p, false);
-	if (p.jac) {
-		hipError_t e = set_lds((const void*)lutmodel_forward_kernel<true>, lds);
-		if (e != hipSuccess) return e;
-		lutmodel_forward_kernel<true><<<p.n, kModelBlock, lds, stream>>>(p);
-	} else {
-		hipError_t e = set_lds((const void*)lutmodel_forward_kernel<false>, lds);
-		if (e != hipSuccess) return e;
-		lutmodel_forward_kernel<false><<<p.n, kModelBlock, lds, stream>>>(p);
-	}
+	return p.jac ? launch_forward_ovs<true>(p, lds, stream) : launch_forward_ovs<false>(p, lds, stream);
+}
+
+template <int OVS>
+static hipError_t launch_backward(const LutModelParams& p, size_t lds, hipStream_t stream)
+{
+	hipError_t e = set_lds((const void*)lutmodel_backward_kernel<OVS>, lds);
+	if (e != hipSuccess) return e;
+	lutmodel_backward_kernel<OVS><<<p.n, kModelBlock, lds, stream>>>(p);
 	return hipGetLastError();
 }
 
@@ -542,10 +578,17 @@ hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream)
 {
 	if (p.n <= 0) return hipSuccess;
 	const size_t lds = lutmodel_lds_bytes(p, false);
-	hipError_t e = set_lds((const void*)lutmodel_backward_kernel, lds);
-	if (e != hipSuccess) return e;
-	lutmodel_backward_kernel<<<p.n, kModelBlock, lds, stream>>>(p);
-	return hipGetLastError();
+	switch (p.ovs) {
+	case 1: return launch_backward<1>(p, lds, stream);
+	case 2: return launch_backward<2>(p, lds, stream);
+	case 3: return launch_backward<3>(p, lds, stream);
+	case 4: return launch_backward<4>(p, lds, stream);
+	case 5: return launch_backward<5>(p, lds, stream);
+	case 6: return launch_backward<6>(p, lds, stream);
+	case 7: return launch_backward<7>(p, lds, stream);
+	case 8: return launch_backward<8>(p, lds, stream);
+	default: return launch_backward<0>(p, lds, stream);
+	}
 }
 
 template <int OVS>
