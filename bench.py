@@ -35,6 +35,7 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "localize_realtime100_hbm_traff
 
 
 SQ_PROFILE = os.path.join(ROOT, "profiles", "localize_realtime100_sq_counters.txt")
+MATCH_TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "lutmatch_precision_hbm_traffic.txt")
 # VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
@@ -198,7 +199,8 @@ def precision_bench(a):
                                      "bf16_equal_to_oracle": bool(np.array_equal(zb[:nb], zo.astype(np.float32))),
                                      "max_abs_dz_bf16_vs_fp64_oracle": float(np.max(np.abs(zb[:nb] - z64)))}},
         "roofline": {"bound": "hbm", "achieved": out["bf16"]["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(out["bf16"]["hbm_gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(out["bf16"]["hbm_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": _profile_value(MATCH_TRAFFIC_PROFILE, "traffic_bytes_per_launch"),
                      "kernel_ms": out["bf16"]["kernel_ms"]},
         "cpu_baseline": {"value": round(nb * frames / t_cpu, 1), "unit": "profiles/s", "cores": 1, "kind": "port",
                          "sample": f"{nb * frames} profiles through the oracle's LUTProfileCompare + FP64 fit"},
