@@ -20,6 +20,10 @@
 // Double sums over many pixels/angles (COM moments, radial-profile bins) are
 // reduced in a fixed tree instead of left to right; for integer pixel data
 // the mean/variance sums are exact either way.
+//
+// The 2D Gaussian MLE alone on ROIs of up to 1,024 pixels runs instead in
+// gauss_wave_kernel, one wavefront per ROI (the same computation, every sum
+// inside the wavefront).
 #include "localize.hpp"
 #include "peakfit.hpp"
 
