@@ -86,7 +86,7 @@ def cpu_baseline(w, seconds_target=12.0):
     jobs["frame"] = np.arange(len(jobs))
     t, _ = o.queued_run(cfg, w["mode"], w["luts"], rois, jobs, threads)
     return {"value": round(len(jobs) / t, 1), "unit": "localizations/s", "cores": threads, "kind": "port",
-            "sample": f"{len(jobs)} ROIs ({reps}x the 4096-ROI batch), {threads} worker threads, oracle "
+            "sample": f"{len(jobs)} ROIs ({reps}x the {n}-ROI batch), {threads} worker threads, oracle "
                       f"QueuedCPUTracker restatement; 1 thread: {k1 / t1:.0f} loc/s"}
 
 
@@ -429,10 +429,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--rois", type=int, default=4096)
+    ap.add_argument("--rois", type=int, default=None, help="ROIs per step (realtime100: 4096, plain64: 1000)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
-    ap.add_argument("--workload", default="realtime100", choices=["realtime100", "precision", "mle", "batching"])
+    ap.add_argument("--workload", default="realtime100",
+                    choices=["realtime100", "plain64", "precision", "mle", "batching"])
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -451,14 +452,23 @@ def main():
         batching_bench(a)
         return
 
+    localize_bench(a, a.workload)
+
+
+def localize_bench(a, name):
+    """The localization chain through the tracker's device batch: realtime100 (the headline:
+    4,096 ROIs of 100x100 u16, 256 beads with 200 x 64 LUTs) or plain64 (the CPU-runnable
+    case: 1,000 ROIs of 64x64 f32, one bead with a 100 x 32 LUT)."""
     from qtrk_amd import api, synth
 
-    w = synth.make_workload("realtime100", n=a.rois)
+    realtime = name == "realtime100"
+    n_req = a.rois if a.rois else (4096 if realtime else 1000)
+    w = synth.make_workload(name, n=n_req)
     trk = api.QTrkInstance(w["settings"])
     trk.set_radial_zlut(w["luts"])
     trk.set_localization_mode(w["mode"])
     n = len(w["jobs"])
-    b = api.DeviceBatch(trk, n, api.QTrkU16)
+    b = api.DeviceBatch(trk, n, api.PDT_OF_DTYPE[w["rois"].dtype])
     b.upload(w["rois"], w["jobs"])
     b.run(n, a.warmup)
     b.sync()
@@ -472,11 +482,19 @@ def main():
     ms_per_step = wall / a.steps * 1e3
     value = n * a.steps / wall
     kern_s = ev_ms / a.steps / 1e3
-    achieved = n * BYTES_PER_LOC / kern_s / 1e9
+    H, W = w["rois"].shape[1:]
+    # ROI + the bead's LUT shared by the bead's ROIs + job + result
+    lut_bytes = w["luts"][0].size * 4
+    bytes_per_loc = BYTES_PER_LOC if realtime else W * H * w["rois"].itemsize + lut_bytes / n + 24 + 52
+    achieved = n * bytes_per_loc / kern_s / 1e9
     err = {k: float(np.sqrt(np.mean((res[k] - w["truth"][:, i]) ** 2))) for i, k in enumerate("xyz")}
-
+    cfg = w["cfg"]
+    workload = (f"{name}: {n} ROIs of {W}x{H} {np.dtype(w['rois'].dtype).name}; COM + QI ({cfg.qi_iterations} "
+                f"iterations, {cfg.qi_radialsteps} radial x {cfg.qi_angstepspq} angular/quadrant) + "
+                f"{'per-bead ' if realtime else ''}Z-LUT ({w['luts'].shape[1]} planes x {w['luts'].shape[2]} steps) "
+                "with 7-point FP64 fit; ROIs resident in HBM")
     out = {
-        "metric": METRIC,
+        "metric": METRIC if realtime else "localizations/sec (COM+QI+Z-LUT, 64×64 f32 ROI); px/plane error",
         "value": round(value, 1),
         "unit": "localizations/s",
         "n_gpus": 1,
@@ -485,21 +503,19 @@ def main():
         "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": round(value / PUBLISHED_CPU_LOCS, 1),
+        "vs_baseline": round(value / PUBLISHED_CPU_LOCS, 1) if realtime else None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "realtime100: 4096 ROIs (256 beads x 16 frames) of 100x100 u16; COM + QI "
-                               "(5 iterations, 64 radial x 128 angular/quadrant) + per-bead Z-LUT "
-                               "(200 planes x 64 steps) with 7-point FP64 fit; ROIs resident in HBM",
-                   "rois_per_step": n, "roi": [100, 100], "pixel": "u16", "parallelism": "1 GPU",
+        "config": {"workload": workload, "rois_per_step": n, "roi": [W, H],
+                   "pixel": np.dtype(w["rois"].dtype).name, "parallelism": "1 GPU",
                    "rms_error_vs_truth": {"x_px": round(err["x"], 4), "y_px": round(err["y"], 4),
                                           "z_planes": round(err["z"], 4)}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic() if realtime else None,
                      "kernel_ms": round(kern_s * 1e3, 4)},
     }
     valu = _profile_value(SQ_PROFILE, "SQ_INSTS_VALU")
-    if valu and a.rois == 4096:
+    if realtime and valu and n == 4096:
         # the sampling stages are VALU-issue bound: instructions per launch from the counter summary
         out["roofline"]["valu_issue"] = {"wave_instr_per_launch": valu,
                                          "achieved": round(valu / kern_s / 1e9, 1),
