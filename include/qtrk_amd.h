@@ -44,8 +44,11 @@ QTRK_API int QTrkBatchUpload(QTrkBatch* b, const void* rois, const LocalizationJ
 QTRK_API void* QTrkBatchDeviceROIs(QTrkBatch* b);
 QTRK_API void* QTrkBatchDeviceJobs(QTrkBatch* b);
 QTRK_API void* QTrkBatchDeviceResults(QTrkBatch* b);
-/* Launches `repeats` back-to-back localizations of the first n ROIs
- * (asynchronous).  HIP events on the batch stream bracket all of them. */
+/* Launches `repeats` localizations of the first n ROIs (asynchronous), as a
+ * stream of consecutive batches: the repeats alternate between the batch
+ * stream and a second one, so a launch's first workgroups fill the CUs the
+ * previous launch's last ones leave idle; all write the same (identical)
+ * results.  HIP events on the batch stream bracket all of them. */
 QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats);
 /* Waits for the batch stream; returns the event-timed milliseconds of the
  * last QTrkBatchRun (all repeats), or a negative value on error. */

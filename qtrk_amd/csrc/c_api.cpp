@@ -249,8 +249,8 @@ struct QTrkBatch {
 	int maxROIs = 0, pdt = 0;
 	DeviceBuffer rois, jobs, results, best, prof;
 	bool diag = false;
-	hipStream_t stream = nullptr;
-	hipEvent_t e0 = nullptr, e1 = nullptr;
+	hipStream_t stream = nullptr, stream2 = nullptr;  // repeats alternate between the two
+	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
 };
 
 QTRK_API QTrkBatch* QTrkBatchCreate(QueuedTracker* q, int maxROIs, QTRK_PixelDataType pdt)
@@ -270,8 +270,10 @@ QTRK_API QTrkBatch* QTrkBatchCreate(QueuedTracker* q, int maxROIs, QTRK_PixelDat
 		b->jobs.ensure(sizeof(DevJob) * maxROIs);
 		b->results.ensure(sizeof(DevResult) * maxROIs);
 		HIP_CHECK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+		HIP_CHECK(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
 		HIP_CHECK(hipEventCreate(&b->e0));
 		HIP_CHECK(hipEventCreate(&b->e1));
+		HIP_CHECK(hipEventCreateWithFlags(&b->e2, hipEventDisableTiming));
 	});
 	return b;
 }
@@ -280,9 +282,12 @@ QTRK_API void QTrkBatchFree(QTrkBatch* b)
 {
 	if (!b) return;
 	hipStreamSynchronize(b->stream);
+	hipStreamSynchronize(b->stream2);
 	hipEventDestroy(b->e0);
 	hipEventDestroy(b->e1);
+	hipEventDestroy(b->e2);
 	hipStreamDestroy(b->stream);
+	hipStreamDestroy(b->stream2);
 	delete b;
 }
 
@@ -314,8 +319,16 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			P.best_out = b->best.as<int>();
 			P.prof_out = b->prof.as<float>();
 		}
+		// Repeats of the batch alternate between two streams, so that a launch's first
+		// workgroups fill the CUs the previous launch's last ones leave idle, as consecutive
+		// camera batches do; e0 .. e1 spans both.  They write the same (identical) results.
 		HIP_CHECK(hipEventRecord(b->e0, b->stream));
-		for (int r = 0; r < repeats; r++) HIP_CHECK(launch_localize(P, b->stream));
+		if (repeats > 1) HIP_CHECK(hipStreamWaitEvent(b->stream2, b->e0, 0));
+		for (int r = 0; r < repeats; r++) HIP_CHECK(launch_localize(P, (r & 1) ? b->stream2 : b->stream));
+		if (repeats > 1) {
+			HIP_CHECK(hipEventRecord(b->e2, b->stream2));
+			HIP_CHECK(hipStreamWaitEvent(b->stream, b->e2, 0));
+		}
 		HIP_CHECK(hipEventRecord(b->e1, b->stream));
 	});
 	return ok ? 0 : -1;
