@@ -433,7 +433,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--workload", default="realtime100",
-                    choices=["realtime100", "plain64", "precision", "mle", "batching"])
+                    choices=["realtime100", "plain64", "queue", "precision", "mle", "batching"])
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -451,8 +451,87 @@ def main():
     if a.workload == "batching":
         batching_bench(a)
         return
+    if a.workload == "queue":
+        queue_bench(a)
+        return
 
     localize_bench(a, a.workload)
+
+
+def queue_bench(a):
+    """The realtime100 ROIs through the drop-in queue API from host memory: 16 camera frames of
+    1,600 x 1,600 u16 per step, each holding a 16 x 16 grid of the batch's 100 x 100 ROIs (one per
+    bead), handed to QTrkScheduleFrame with the 256 ROI positions.  Timed as a camera stream: the
+    frames of all steps are scheduled one after another and the results fetched as they arrive,
+    then Flush and the last results.  The rate includes the frames' copies into pinned memory and
+    over PCIe.  A drained pass (16 frames, Flush, wait for all 4,096 results) is timed as well."""
+    from qtrk_amd import api, synth
+
+    w = synth.make_workload("realtime100", n=4096)
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_radial_zlut(w["luts"])
+    trk.set_localization_mode(w["mode"])
+    per, side = 256, 16
+    frames = np.zeros((16, side * 100, side * 100), np.uint16)
+    for k in range(4096):
+        f, i = divmod(k, per)
+        r, c = divmod(i, side)
+        frames[f, r * 100:(r + 1) * 100, c * 100:(c + 1) * 100] = w["rois"][k]
+    pos = np.array([[(i % side) * 100, (i // side) * 100] for i in range(per)], np.int32)
+    jobs = [api.make_jobs(1, frame=np.array([f], np.uint32), zlut_index=np.array([0], np.int32))[0]
+            for f in range(16)]
+
+    def one_pass():
+        for f in range(16):
+            trk.schedule_frame(frames[f], pos, jobs[f])
+        trk.wait_for(4096)
+        return trk.fetch_results(4096)
+
+    def stream(steps):
+        got = 0
+        for _ in range(steps):
+            for f in range(16):
+                trk.schedule_frame(frames[f], pos, jobs[f])
+            got += len(trk.fetch_results())
+        trk.wait_for(4096 * steps - got)
+        got += len(trk.fetch_results())
+        return got
+
+    for _ in range(a.warmup):
+        one_pass()
+    stream(a.warmup)
+    t0 = time.perf_counter()
+    got = stream(a.steps)
+    wall = time.perf_counter() - t0
+    assert got == 4096 * a.steps, got
+    passes = []
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        res = one_pass()
+        passes.append((time.perf_counter() - t) * 1e3)
+    # the same localizations as the device-resident batch (results are LIFO: order by frame, bead)
+    b = api.DeviceBatch(trk, 4096, api.QTrkU16)
+    b.upload(w["rois"], w["jobs"])
+    b.run(4096, 1)
+    b.sync()
+    ref = b.download(4096)
+    b.free()
+    trk.free()
+    res = res[np.lexsort((res["zlutIndex"], res["frame"]))]
+    same = bool(np.array_equal(res["x"], ref["x"]) and np.array_equal(res["z"], ref["z"]))
+    value = 4096 * a.steps / wall
+    out = {"metric": "localizations/sec through QTrkScheduleFrame from host frames (PCIe included)",
+           "value": round(value, 1), "unit": "localizations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(wall / a.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "realtime100 ROIs as 16 frames of 1600x1600 u16 (256 ROIs each) per step, "
+                                  "QTrkScheduleFrame streamed, results fetched as they arrive",
+                      "host_bytes_per_step": int(frames.nbytes),
+                      "pcie_gbs": round(frames.nbytes * a.steps / wall / 1e9, 2),
+                      "drained_pass_ms_min_median_max": [round(float(v), 3) for v in
+                                                         (min(passes), np.median(passes), max(passes))],
+                      "results_equal_device_batch": same}}
+    print(json.dumps(out), flush=True)
 
 
 def localize_bench(a, name):

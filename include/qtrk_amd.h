@@ -16,7 +16,9 @@ QTRK_API void QTrkClearLastError(void);
 
 /* QueuedTracker::SetConfigValue / GetConfigValues (cputrack/QueuedTracker.h:76-78),
  * which the reference's C ABI leaves commented out (cputrack/qtrk_c_api.h:193-196).
- * Keys: "batch_size" (ROIs per GPU launch), "num_streams" (batches in flight).
+ * Keys: "batch_size" (ROIs per GPU launch), "num_streams" (batches in flight),
+ * "frame_copy_threads" (threads, the caller's included, that copy a
+ * QTrkScheduleFrame frame into pinned memory; default 5).
  * GetConfigValues writes "key=value" lines. */
 QTRK_API void QTrkSetConfigValue(QueuedTracker* qtrk, const char* name, const char* value);
 QTRK_API void QTrkGetConfigValues(QueuedTracker* qtrk, char* dst, int maxStrLen);

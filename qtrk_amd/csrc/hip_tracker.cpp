@@ -10,6 +10,12 @@
 
 namespace qtrk {
 
+// a batch slot holds the row blocks of several frames, each starting at a
+// multiple of kFrameBlockAlign bytes, up to kFrameSlotMaxBytes unless one block
+// alone is larger
+static constexpr size_t kFrameBlockAlign = 256;
+static constexpr size_t kFrameSlotMaxBytes = (size_t)64 << 20;
+
 void DeviceBuffer::ensure(size_t n)
 {
 	if (n <= bytes && p) return;
@@ -144,7 +150,6 @@ QueuedHIPTracker::QueuedHIPTracker(const QTrkComputedConfig& cc)
 	lds_limit = prop.sharedMemPerBlock;
 	if (cfg.width < 2 || cfg.height < 2) fail("ROI width and height must be >= 2");
 	if (cfg.downsample != 0) fail("downsample != 0 is not supported");
-	HIP_CHECK(hipStreamCreateWithFlags(&main_stream, hipStreamNonBlocking));
 	BuildTables();
 	AllocSlots();
 }
@@ -158,7 +163,6 @@ QueuedHIPTracker::~QueuedHIPTracker()
 			if (s->state == Slot::Launched) hipEventSynchronize(s->ev_done);
 	}
 	FreeSlots();
-	if (main_stream) hipStreamDestroy(main_stream);
 }
 
 void QueuedHIPTracker::BuildTables()
@@ -359,7 +363,7 @@ void QueuedHIPTracker::ScheduleLocalization(void* data, int pitch, QTRK_PixelDat
 	if ((int)pdt < 0 || (int)pdt > 2) fail("invalid pixel data type");
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
-	if (filling && filling->n > 0 && filling->pdt != (int)pdt) {
+	if (filling && filling->n > 0 && (filling->pdt != (int)pdt || filling->frame)) {
 		Launch(filling);
 		filling = nullptr;
 	}
@@ -387,15 +391,15 @@ void QueuedHIPTracker::Launch(Slot* s)
 {
 	const int bpp = PDT_BytesPerPixel((QTRK_PixelDataType)s->pdt);
 	const size_t rb = (size_t)cfg.width * cfg.height * bpp;
-	HIP_CHECK(hipEventRecord(s->ev_start, s->stream));
 	LocalizeParams P;
 	if (s->frame) {
-		HIP_CHECK(hipMemcpyAsync(s->d_rois.p, s->h_rois.p, s->frame_bytes, hipMemcpyHostToDevice, s->stream));
+		// the row blocks are on their way already (ScheduleFrame), ev_start recorded before the first
 		HIP_CHECK(hipMemcpyAsync(s->d_off.p, s->h_off.p, sizeof(int64_t) * s->n, hipMemcpyHostToDevice, s->stream));
 		HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
 		FillParams(P, s->pdt, s->n, s->d_rois.p, s->d_off.as<int64_t>(), s->frame_pitch, s->d_jobs.as<DevJob>(),
 		           s->d_results.as<DevResult>());
 	} else {
+		HIP_CHECK(hipEventRecord(s->ev_start, s->stream));
 		HIP_CHECK(hipMemcpyAsync(s->d_rois.p, s->h_rois.p, rb * s->n, hipMemcpyHostToDevice, s->stream));
 		HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
 		FillParams(P, s->pdt, s->n, s->d_rois.p, nullptr, cfg.width * bpp, s->d_jobs.as<DevJob>(),
@@ -412,10 +416,12 @@ void QueuedHIPTracker::Launch(Slot* s)
 
 // QueuedTracker::ScheduleFrame (cputrack/QueuedTracker.cpp:91-111) with the
 // extraction on the GPU: ROIs outside the frame are skipped by the reference's
-// rule and job.zlutIndex = i + jobInfo->zlutIndex as there; then, per
-// batch_size ROIs, the frame rows those ROIs cover are copied once into a batch
-// slot and the kernel reads each ROI at its byte offset in them.  Returns the
-// number of ROIs queued.
+// rule and job.zlutIndex = i + jobInfo->zlutIndex as there; then the frame
+// rows the ROIs cover are copied once into the filling batch slot, after the
+// row blocks of earlier frames of the same pixel type and pitch, and on to the
+// GPU at once; the kernel reads each ROI at its byte offset in the slot.  A slot launches when it holds
+// batch_size ROIs, when its row blocks would outgrow it, or on Flush, as
+// scheduled ROIs do.  Returns the number of ROIs queued.
 int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int height, ROIPosition* positions,
                                     int numROI, QTRK_PixelDataType pdt, const LocalizationJob* jobInfo)
 {
@@ -434,12 +440,23 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 	const uint8_t* img = (const uint8_t*)imgptr;
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
-	if (filling && filling->n > 0) {  // ROIs queued before the frame go first
-		Launch(filling);
+	if (filling && filling->n > 0 &&
+	    !(filling->frame && filling->pdt == (int)pdt && filling->frame_pitch == pitch)) {
+		Launch(filling);  // ROIs queued before the frame go first
 		filling = nullptr;
 	}
-	for (size_t c0 = 0; c0 < sel.size(); c0 += (size_t)batchSize) {
-		const size_t c1 = std::min(sel.size(), c0 + (size_t)batchSize);
+	if (!copier) copier.reset(new CopyPool(frameCopyThreads - 1, true));
+	size_t c0 = 0;
+	while (c0 < sel.size()) {
+		if (!filling) {
+			filling = AcquireSlot(lk);
+			filling->frame = true;
+			filling->pdt = (int)pdt;
+			filling->frame_pitch = pitch;
+			filling->frame_bytes = 0;
+		}
+		Slot* s = filling;
+		const size_t c1 = std::min(sel.size(), c0 + (size_t)(batchSize - s->n));
 		int y0 = height, y1 = 0;
 		for (size_t k = c0; k < c1; k++) {
 			y0 = std::min(y0, positions[sel[k]].y);
@@ -447,27 +464,44 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 		}
 		// rows y0 .. y1-1; the last one only as far as the frame's width
 		const size_t bytes = (size_t)pitch * (y1 - y0 - 1) + (size_t)width * bpp;
-		Slot* s = AcquireSlot(lk);
-		s->frame = true;
-		s->pdt = (int)pdt;
-		s->frame_pitch = pitch;
-		s->frame_bytes = bytes;
-		s->n = (int)(c1 - c0);
-		s->h_rois.ensure(bytes);
-		s->d_rois.ensure(bytes);
-		s->h_off.ensure(sizeof(int64_t) * (size_t)batchSize);
-		s->d_off.ensure(sizeof(int64_t) * (size_t)batchSize);
-		memcpy(s->h_rois.p, img + (size_t)pitch * y0, bytes);
-		int64_t* off = s->h_off.as<int64_t>();
-		DevJob* jobs = s->h_jobs.as<DevJob>();
+		const size_t at = (s->frame_bytes + kFrameBlockAlign - 1) & ~(kFrameBlockAlign - 1);
+		if (s->n > 0 && at + bytes > std::min(s->h_rois.bytes, s->d_rois.bytes)) {
+			Launch(s);
+			filling = nullptr;
+			continue;
+		}
+		if (s->n == 0) {
+			// room for as many blocks like this one as make a batch, within kFrameSlotMaxBytes
+			const size_t per = c1 - c0, blocks = ((size_t)batchSize + per - 1) / per;
+			const size_t block = (bytes + kFrameBlockAlign - 1) & ~(kFrameBlockAlign - 1);
+			const size_t cap = std::max(bytes, std::min(blocks * block, kFrameSlotMaxBytes));
+			s->h_rois.ensure(cap);
+			s->d_rois.ensure(cap);
+			s->h_off.ensure(sizeof(int64_t) * (size_t)batchSize);
+			s->d_off.ensure(sizeof(int64_t) * (size_t)batchSize);
+		}
+		if (s->n == 0) HIP_CHECK(hipEventRecord(s->ev_start, s->stream));
+		copier->Copy(s->h_rois.as<uint8_t>() + at, img + (size_t)pitch * y0, bytes);
+		// each block goes to the GPU as soon as it is in pinned memory; the kernel follows on
+		// the same stream when the slot launches
+		HIP_CHECK(hipMemcpyAsync(s->d_rois.as<uint8_t>() + at, s->h_rois.as<uint8_t>() + at, bytes,
+		                         hipMemcpyHostToDevice, s->stream));
+		int64_t* off = s->h_off.as<int64_t>() + s->n;
+		DevJob* jobs = s->h_jobs.as<DevJob>() + s->n;
 		for (size_t k = c0; k < c1; k++) {
 			const int i = sel[k];
-			off[k - c0] = (int64_t)pitch * (positions[i].y - y0) + (int64_t)positions[i].x * bpp;
+			off[k - c0] = (int64_t)(at + (size_t)pitch * (positions[i].y - y0) + (size_t)positions[i].x * bpp);
 			LocalizationJob job = *jobInfo;
 			job.zlutIndex = i + jobInfo->zlutIndex;
 			memcpy(&jobs[k - c0], &job, sizeof(DevJob));
 		}
-		Launch(s);
+		s->n += (int)(c1 - c0);
+		s->frame_bytes = at + bytes;
+		if (s->n >= batchSize) {
+			Launch(s);
+			filling = nullptr;
+		}
+		c0 = c1;
 	}
 	return (int)sel.size();
 }
@@ -565,10 +599,10 @@ void QueuedHIPTracker::ComputeZBiasCorrection(int biasPlanes, float* result, int
 			jobs[i] = DevJob{ 0u, 0u, bead, 0.0f, 0.0f, 0.0f };
 		}
 		HIP_CHECK(hipMemcpyAsync(d_theta.p, theta.data(), (size_t)n * kModelParams * 4, hipMemcpyHostToDevice,
-		                         main_stream));
-		HIP_CHECK(hipMemcpyAsync(d_idx.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, main_stream));
+		                         aux_stream()));
+		HIP_CHECK(hipMemcpyAsync(d_idx.p, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, aux_stream()));
 		HIP_CHECK(hipMemcpyAsync(d_jobs.p, jobs.data(), (size_t)n * sizeof(DevJob), hipMemcpyHostToDevice,
-		                         main_stream));
+		                         aux_stream()));
 		LutModelParams m{};
 		m.luts = d_zlut.as<float>();
 		m.count = zcount;
@@ -585,7 +619,7 @@ void QueuedHIPTracker::ComputeZBiasCorrection(int biasPlanes, float* result, int
 		m.lut_index = d_idx.as<int>();
 		m.lut_index_stride = 1;
 		m.mu = d_img.as<float>();
-		HIP_CHECK(launch_lutmodel_forward(m, main_stream));
+		HIP_CHECK(launch_lutmodel_forward(m, aux_stream()));
 		LocalizeParams P;
 		FillParams(P, 2, n, d_img.p, nullptr, W * 4, d_jobs.as<DevJob>(), d_res.as<DevResult>());
 		P.mode = kModeQI | kModeLocalizeZ | kModeNormalizeProfile;
@@ -597,10 +631,10 @@ void QueuedHIPTracker::ComputeZBiasCorrection(int biasPlanes, float* result, int
 		P.cmp_out = nullptr;
 		P.calib_offset = nullptr;
 		P.calib_gain = nullptr;
-		HIP_CHECK(launch_localize(P, main_stream));
+		HIP_CHECK(launch_localize(P, aux_stream()));
 		HIP_CHECK(hipMemcpyAsync(res.data(), d_res.p, (size_t)n * sizeof(DevResult), hipMemcpyDeviceToHost,
-		                         main_stream));
-		HIP_CHECK(hipStreamSynchronize(main_stream));
+		                         aux_stream()));
+		HIP_CHECK(hipStreamSynchronize(aux_stream()));
 		for (int i = 0; i < n; i++) table[c0 + i] = res[i].z - zs[i];
 	}
 	SetZLUTBiasCorrection(table.data(), biasPlanes, zcount);
@@ -846,7 +880,7 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 			memcpy(h_build_rois.as<uint8_t>() + rb * i + row * y,
 			       (const uint8_t*)data + (size_t)pitch * cfg.height * i + (size_t)pitch * y, row);
 	d_build_rois.ensure(rb * zcount);
-	HIP_CHECK(hipMemcpyAsync(d_build_rois.p, h_build_rois.p, rb * zcount, hipMemcpyHostToDevice, main_stream));
+	HIP_CHECK(hipMemcpyAsync(d_build_rois.p, h_build_rois.p, rb * zcount, hipMemcpyHostToDevice, aux_stream()));
 	LocalizeParams P;
 	FillParams(P, (int)pdt, zcount, d_build_rois.p, nullptr, (int)row, nullptr, nullptr);
 	P.mode = (buildFlags & BUILDLUT_NORMALIZE) ? LT_NormalizeProfile : 0;
@@ -862,8 +896,8 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 		upload(d_build_pos, known_pos, sizeof(vector2f) * zcount);
 		P.known_pos = d_build_pos.as<float>();
 	}
-	HIP_CHECK(launch_localize(P, main_stream));
-	HIP_CHECK(hipStreamSynchronize(main_stream));
+	HIP_CHECK(launch_localize(P, aux_stream()));
+	HIP_CHECK(hipStreamSynchronize(aux_stream()));
 	h_zlut_stale = true;
 }
 
@@ -874,8 +908,8 @@ void QueuedHIPTracker::FinalizeLUT()
 	HIP_CHECK(hipSetDevice(dev));
 	WaitAllLocked(lk);
 	if (zcount <= 0) return;
-	HIP_CHECK(launch_normalize_rows(d_zlut.as<float>(), zcount * zplanes, cfg.zlut_radialsteps, main_stream));
-	HIP_CHECK(hipStreamSynchronize(main_stream));
+	HIP_CHECK(launch_normalize_rows(d_zlut.as<float>(), zcount * zplanes, cfg.zlut_radialsteps, aux_stream()));
+	HIP_CHECK(hipStreamSynchronize(aux_stream()));
 	HIP_CHECK(hipMemcpy(h_zlut.data(), d_zlut.p, h_zlut.size() * 4, hipMemcpyDeviceToHost));
 	h_zlut_stale = false;
 }
@@ -894,6 +928,12 @@ void QueuedHIPTracker::SetConfigValue(std::string name, std::string value)
 		if (name == "batch_size") batchSize = v;
 		else numSlots = v;
 		AllocSlots();
+	} else if (name == "frame_copy_threads") {
+		int v = atoi(value.c_str());
+		if (v < 1 || v > 64) fail("frame_copy_threads must be in 1..64");
+		std::unique_lock<std::mutex> lk(mtx);
+		frameCopyThreads = v;
+		copier.reset();  // remade with the new size on the next frame
 	}
 }
 
@@ -902,6 +942,7 @@ QueuedTracker::ConfigValueMap QueuedHIPTracker::GetConfigValues()
 	ConfigValueMap m;
 	m["batch_size"] = std::to_string(batchSize);
 	m["num_streams"] = std::to_string(numSlots);
+	m["frame_copy_threads"] = std::to_string(frameCopyThreads);
 	return m;
 }
 

@@ -9,6 +9,7 @@
 #pragma once
 
 #include "QueuedTracker.h"
+#include "copy_pool.hpp"
 #include "devbuf.hpp"
 #include "localize.hpp"
 
@@ -16,6 +17,7 @@
 
 #include <cstdint>
 #include <deque>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -90,8 +92,8 @@ private:
 		uint64_t seq = 0;
 		PinnedBuffer h_rois, h_jobs, h_results;
 		DeviceBuffer d_rois, d_jobs, d_results;
-		// frame mode (ScheduleFrame): h_rois holds rows of a camera frame, `pitch` bytes
-		// apart, and ROI i starts at byte h_off[i] of it
+		// frame mode (ScheduleFrame): h_rois holds row blocks of camera frames, rows
+		// `frame_pitch` bytes apart, frame_bytes in all, and ROI i starts at byte h_off[i]
 		bool frame = false;
 		int frame_pitch = 0;
 		size_t frame_bytes = 0;
@@ -112,7 +114,10 @@ private:
 	void Warn(const std::string& w);
 
 	int dev = 0;
-	hipStream_t main_stream = nullptr;
+	// The synchronous set-up work (LUT build and normalisation, bias table) runs after the batches
+	// in flight have finished, on the first slot's stream: every stream takes one of the process's
+	// few hardware queues (GPU_MAX_HW_QUEUES), and streams that share one run one after another.
+	hipStream_t aux_stream() const { return slots[0]->stream; }
 
 	// per-config device tables
 	DeviceBuffer d_qi_steps, d_qi_stride, d_qi_dirs, d_qi_window, d_z_dirs;
@@ -143,6 +148,9 @@ private:
 	std::vector<Slot*> slots;
 	Slot* filling = nullptr;
 	uint64_t seqCounter = 0;
+	// threads, the caller's included, that copy a frame's rows into pinned memory
+	int frameCopyThreads = 5;
+	std::unique_ptr<CopyPool> copier;  // made on the first frame
 
 	mutable std::mutex mtx;
 	std::mutex res_mtx;

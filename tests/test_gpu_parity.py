@@ -325,6 +325,66 @@ def test_schedule_frame_chunks_pitch_and_formats(gpu, dtype):
         assert np.array_equal(r[k], ref[k]), k
 
 
+def test_schedule_frame_packs_frames_into_batches(gpu):
+    """Frames of one pixel type and pitch share a batch slot, their row blocks one after another,
+    until it holds batch_size ROIs (the rest goes on to the next slot) or a block would outgrow
+    it; a per-ROI call, another pitch or another pixel type launches the filling slot first.
+    Every result equals the per-ROI path."""
+    w = synth.make_workload("realtime100", n=40)
+    src = w["rois"]
+    src8 = np.clip(src // 64, 0, 255).astype(np.uint8)
+    grid = lambda n, dx, dy, x0=0, y0=0: [(x0 + dx * (i % 6), y0 + dy * (i // 6)) for i in range(n)]
+    # (dtype, frame height, width, pitch padding in pixels, ROI positions) or ("roi", count)
+    steps = [(np.uint16, 260, 700, 0, grid(12, 110, 130, 10, 5)),  # cap: 2 blocks of this size
+             (np.uint16, 260, 700, 0, grid(7, 110, 120, 30, 20)),  # 4 fill the batch, 3 go on
+             ("roi", 2),                                          # launches the 3 first
+             (np.uint16, 900, 700, 0, [(5, 0), (200, 400), (400, 790)]),
+             (np.uint16, 240, 900, 16, grid(8, 110, 120)),        # other pitch: launches
+             (np.uint16, 900, 900, 16, [(0, 0), (300, 400), (600, 790)]),  # outgrows: launches
+             (np.uint8, 300, 400, 0, [(0, 0), (150, 100), (290, 190)])]   # other type: launches
+    luts = np.concatenate([w["luts"]] * 2)
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_config_value("batch_size", "16")
+    trk.set_radial_zlut(luts)
+    trk.set_localization_mode(w["mode"])
+    k, meta = 0, []   # meta: (frame number, is u8) per ROI, in zlutIndex order
+    for f, st in enumerate(steps):
+        if st[0] == "roi":
+            for _ in range(st[1]):
+                trk.schedule_localization(src[k], api.make_jobs(1, frame=np.array([f]), zlut_index=np.array([k]))[0])
+                meta.append((f, False))
+                k += 1
+            continue
+        dtype, H, W, pad, pos = st
+        big = np.zeros((H, W + pad), dtype)
+        frame = big[:, :W]
+        for i, (x, y) in enumerate(pos):
+            assert x + 100 <= W and y + 100 <= H
+            frame[y:y + 100, x:x + 100] = (src8 if dtype == np.uint8 else src)[k + i]
+        job = api.make_jobs(1, frame=np.array([f]), zlut_index=np.array([k]))[0]
+        assert trk.schedule_frame(frame, pos, job) == len(pos)
+        meta += [(f, dtype == np.uint8)] * len(pos)
+        k += len(pos)
+    trk.wait_for(k)
+    r = trk.fetch_results(k)
+    r = r[np.argsort(r["zlutIndex"])]
+    assert list(r["zlutIndex"]) == list(range(k))
+    frames = np.array([m[0] for m in meta])
+    is8 = np.array([m[1] for m in meta])
+    trk2 = api.QTrkInstance(w["settings"])
+    trk2.set_radial_zlut(luts)
+    trk2.set_localization_mode(w["mode"])
+    ref = np.zeros(k, r.dtype)
+    for sel, rois in ((~is8, src[:k][~is8]), (is8, src8[:k][is8])):
+        idx = np.nonzero(sel)[0]
+        jobs = api.make_jobs(len(idx), frame=frames[idx], zlut_index=idx)
+        ref[idx] = _batch(trk2, np.ascontiguousarray(rois), jobs, diag=False)[0]
+    trk2.free()
+    trk.free()
+    for key in ("frame", "zlutIndex", "x", "y", "z", "comx", "comy", "error", "imageMean"):
+        assert np.array_equal(r[key], ref[key]), key
+
+
 def test_build_lut_matches_oracle(gpu, oracle):
     w = synth.make_workload("plain64", n=8)
     cfg = w["cfg"]
