@@ -218,8 +218,13 @@ QTRK_API bool QTrkIsIdle(QueuedTracker* qtrk);                              /* q
 QTRK_API void QTrkGetProfileReport(QueuedTracker* qtrk, char* dst, int maxStrLen); /* qtrk_c_api.h:198 */
 QTRK_API void QTrkGetWarnings(QueuedTracker* qtrk, char* dst, int maxStrLen);      /* qtrk_c_api.h:199 */
 
-/* Bead finding is outside this library's hot path: QTrkFindBeads returns NULL
- * with *beadcount = 0 and records an error (cputrack/qtrk_c_api.h:203-205). */
+/* Bead finding (cputrack/qtrk_c_api.cpp:33-58, cputrack/BeadFinder.cpp): the
+ * frame is cross-correlated on the GPU with the roi x roi sample at
+ * (smpCornerPosX, smpCornerPosY), which must lie inside the frame; the result
+ * is a new[] array of top-left ROI corners for QTrkFreeROIPositions, equal to
+ * the reference's positions in the same order.  Frames up to 8192 pixels a
+ * side after padding to powers of two, roi up to 200.  On an error it returns
+ * NULL with *beadcount = 0 and QTrkGetLastError() set. */
 QTRK_API void QTrkFreeROIPositions(ROIPosition* data);                      /* qtrk_c_api.h:203 */
 QTRK_API ROIPosition* QTrkFindBeads(ImageData* imgData, int smpCornerPosX, int smpCornerPosY, int roi,
                                     float imgRelDist, float acceptance, int* beadcount,

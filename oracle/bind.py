@@ -74,6 +74,8 @@ class Oracle:
             "oq_localize_trace": (None, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P, P, P]),
             "oq_build_lut": (None, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int, C.c_int]),
             "oq_power_spectrum_2d": (None, [P, C.c_int, C.c_int]),
+            "oq_bead_xcor_map": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]),
+            "oq_find_beads": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fl, fl, P, C.c_int]),
             "oq_fourier_radial_profile": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int]),
             "oq_finalize_lut": (None, [P, C.c_int, C.c_int]),
             "oq_bspline_weights": (None, [P, fl]),
@@ -281,6 +283,24 @@ class Oracle:
         out = np.array(img, self.fl, order="C")
         self.L.oq_power_spectrum_2d(_p(out), out.shape[1], out.shape[0])
         return out
+
+    def bead_xcor_map(self, img, sx, sy, roi):
+        """The correlation map of BeadFinder::Find, [H][W] at the next power-of-two size."""
+        im = np.ascontiguousarray(img, self.fl)
+        h, w = im.shape
+        W, H = 1 << max(0, (w - 1).bit_length()), 1 << max(0, (h - 1).bit_length())
+        out = np.zeros((H, W), self.fl)
+        self.L.oq_bead_xcor_map(_p(im), w, h, int(sx), int(sy), int(roi), _p(out))
+        return out
+
+    def find_beads(self, img, sx, sy, roi, img_distance, similarity):
+        """BeadFinder::Find: (n, 2) int32 top-left corners (x, y), in the reference's order."""
+        im = np.ascontiguousarray(img, self.fl)
+        h, w = im.shape
+        cap = 1 << 16
+        xy = np.zeros((cap, 2), np.int32)
+        n = self.L.oq_find_beads(_p(im), w, h, int(sx), int(sy), int(roi), img_distance, similarity, _p(xy), cap)
+        return xy[:min(n, cap)].copy()
 
     def fourier_radial_profile(self, img, nbins, nang):
         """FourierRadialProfile of a copy of img."""

@@ -1479,3 +1479,168 @@ void oq_gauss2d_mle(const fl* img, int w, int h, fl mean, fl x0, fl y0, int iter
 	}
 	out4[0] = px; out4[1] = py; out4[2] = I0; out4[3] = bg;
 }
+
+/* ------------------------------------------------------------ bead finder */
+
+/* NextPowerOf2 (cputrack/BeadFinder.cpp:11-17): the smallest power of two >= x. */
+static int oq_nearest_pow2_up(int x)
+{
+	int y = 1;
+	while (y < x) y *= 2;
+	return y;
+}
+
+/* FFT2D (cputrack/BeadFinder.cpp:45-79): kissfft of every row (length w), then of every
+ * column (length h), in place. */
+static void bf_fft2d(ocpx* d, int w, int h, int inverse)
+{
+	OFFT fx, fy;
+	oq_fft_init(&fx, w, inverse);
+	oq_fft_init(&fy, h, inverse);
+	const int m = w > h ? w : h;
+	ocpx* src = (ocpx*)malloc(sizeof(ocpx) * (size_t)m);
+	ocpx* tmp = (ocpx*)malloc(sizeof(ocpx) * (size_t)m);
+	for (int y = 0; y < h; y++) {
+		oq_fft_transform(&fx, d + (size_t)w * y, tmp);
+		memcpy(d + (size_t)w * y, tmp, sizeof(ocpx) * (size_t)w);
+	}
+	for (int x = 0; x < w; x++) {
+		for (int y = 0; y < h; y++) src[y] = d[(size_t)y * w + x];
+		oq_fft_transform(&fy, src, tmp);
+		for (int y = 0; y < h; y++) d[(size_t)w * y + x] = tmp[y];
+	}
+	free(src);
+	free(tmp);
+	oq_fft_free(&fx);
+	oq_fft_free(&fy);
+}
+
+/* ImageData::mean (cputrack/utils.h:103-108): the sum in index order, divided by the count. */
+static fl bf_mean(const fl* d, int n)
+{
+	fl s = 0;
+	for (int i = 0; i < n; i++) s += d[i];
+	return s / (fl)n;
+}
+
+/* The correlation map of BeadFinder::Find (cputrack/BeadFinder.cpp:166-198): the frame minus its
+ * mean and the sample (the roi x roi block at (sx, sy)) minus its mean, placed at
+ * ((y - roi) mod H, (x - roi) mod W), both padded to the next powers of two W x H; the real part
+ * of the inverse transform of the spectra's product, written to map[H][W]. */
+void oq_bead_xcor_map(const fl* img, int w, int h, int sx, int sy, int roi, fl* map)
+{
+	const int W = oq_nearest_pow2_up(w), H = oq_nearest_pow2_up(h);
+	const fl mean = bf_mean(img, w * h);
+	ocpx* cimg = (ocpx*)calloc((size_t)W * H, sizeof(ocpx));
+	ocpx* simg = (ocpx*)calloc((size_t)W * H, sizeof(ocpx));
+	for (int y = 0; y < h; y++)
+		for (int x = 0; x < w; x++) cimg[(size_t)y * W + x].re = img[(size_t)y * w + x] - mean;
+	bf_fft2d(cimg, W, H, 0);
+	fl* smp = (fl*)malloc(sizeof(fl) * (size_t)roi * roi);
+	for (int y = 0; y < roi; y++)
+		for (int x = 0; x < roi; x++) smp[y * roi + x] = img[(size_t)(y + sy) * w + x + sx];
+	const fl smean = bf_mean(smp, roi * roi);
+	for (int y = 0; y < roi; y++)
+		for (int x = 0; x < roi; x++)
+			simg[(size_t)((y - roi) & (H - 1)) * W + ((x - roi) & (W - 1))].re = smp[roi * y + x] - smean;
+	bf_fft2d(simg, W, H, 0);
+	for (size_t i = 0; i < (size_t)W * H; i++) {
+		const ocpx a = cimg[i], b = simg[i];
+		cimg[i].re = a.re * b.re - a.im * b.im;
+		cimg[i].im = a.re * b.im + a.im * b.re;
+	}
+	bf_fft2d(cimg, W, H, 1);
+	for (size_t i = 0; i < (size_t)W * H; i++) map[i] = cimg[i].re;
+	free(smp);
+	free(cimg);
+	free(simg);
+}
+
+/* BeadFinder::Find (cputrack/BeadFinder.cpp:163-217) on the float frame img[h][w] with the sample
+ * at (sx, sy): the raster scan over the frame's pixels that starts SearchArea (:81-103) at each
+ * value above maxVal * similarity, then RecenterAndFilter (:105-161).  Writes up to max_out (x, y)
+ * pairs to xy and returns the number of beads found. */
+int oq_find_beads(const fl* img, int w, int h, int sx, int sy, int roi, fl img_distance, fl similarity, int* xy,
+                  int max_out)
+{
+	const int W = oq_nearest_pow2_up(w), H = oq_nearest_pow2_up(h);
+	fl* map = (fl*)malloc(sizeof(fl) * (size_t)W * H);
+	oq_bead_xcor_map(img, w, h, sx, sy, roi, map);
+	fl maxVal = 0;
+	for (size_t i = 0; i < (size_t)W * H; i++) maxVal = (maxVal < map[i]) ? map[i] : maxVal;  /* std::max */
+	const fl thr = maxVal * similarity;
+	const fl mind = (fl)0.5 * img_distance * roi;  /* Config::MinPixelDistance */
+	const int dist = (int)mind;
+	int np = 0, cap = 64;
+	int* pts = (int*)malloc(sizeof(int) * 2 * cap);
+	for (int y = 0; y < h; y++)
+		for (int x = 0; x < w; x++) {
+			if (!(map[(size_t)y * W + x] > thr)) continue;
+			/* SearchArea */
+			int bx = x, by = y;
+			fl best = map[(size_t)y * W + x];
+			const int y0 = y - dist > 0 ? y - dist : 0, x0 = x - dist > 0 ? x - dist : 0;
+			const int x1 = x + dist < W - 1 ? x + dist : W - 1, y1 = y + dist < H - 1 ? y + dist : H - 1;
+			for (int yy = y0; yy <= y1; yy++)
+				for (int xx = x0; xx <= x1; xx++) {
+					const int dx = xx - x, dy = yy - y;
+					if (dx * dx + dy * dy <= dist * dist) {
+						const fl v = map[(size_t)yy * W + xx];
+						if (v > best) { bx = xx; by = yy; best = v; }
+						map[(size_t)yy * W + xx] = 0;
+					}
+				}
+			if (np == cap) { cap *= 2; pts = (int*)realloc(pts, sizeof(int) * 2 * cap); }
+			pts[2 * np] = bx;
+			pts[2 * np + 1] = by;
+			np++;
+		}
+	/* RecenterAndFilter: kept[] is the list, in order */
+	int* kept = (int*)malloc(sizeof(int) * 2 * (np ? np : 1));
+	int nk = 0;
+	for (int i = 0; i < np; i++) {
+		int px = pts[2 * i], py = pts[2 * i + 1];
+		if (px < 0 || py < 0 || px > w - 1 - roi || py > h - 1 - roi) continue;
+		fl sum = 0;
+		for (int y = 0; y < roi; y++)
+			for (int x = 0; x < roi; x++) sum += img[(size_t)(y + py) * w + x + px];
+		fl sumX = 0, sumY = 0;
+		const fl mean = sum / (fl)(roi * roi);
+		sum = 0;
+		for (int y = 0; y < roi; y++)
+			for (int x = 0; x < roi; x++) {
+				fl v = img[(size_t)(y + py) * w + x + px] - mean;
+				v = v * v;
+				sum += v;
+				sumX += v * (fl)x;
+				sumY += v * (fl)y;
+			}
+		if (sum == 0) continue;
+		px = (int)((fl)px + (sumX / sum - (fl)(roi / 2)));
+		py = (int)((fl)py + (sumY / sum - (fl)(roi / 2)));
+		if (px < 0 || py < 0 || px > w - 1 - roi || py > h - 1 - roi) continue;
+		int accepted = 1;
+		for (int c = 0; c < nk; c++) {
+			const int dx = kept[2 * c] - px, dy = kept[2 * c + 1] - py;
+			if ((fl)(dx * dx + dy * dy) < mind * mind) {
+				memmove(kept + 2 * c, kept + 2 * c + 2, sizeof(int) * 2 * (size_t)(nk - c - 1));
+				nk--;
+				accepted = 0;
+				break;
+			}
+		}
+		if (accepted) {
+			kept[2 * nk] = px;
+			kept[2 * nk + 1] = py;
+			nk++;
+		}
+	}
+	for (int i = 0; i < nk && i < max_out; i++) {
+		xy[2 * i] = kept[2 * i];
+		xy[2 * i + 1] = kept[2 * i + 1];
+	}
+	free(kept);
+	free(pts);
+	free(map);
+	return nk;
+}

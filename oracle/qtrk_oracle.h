@@ -149,6 +149,12 @@ void oq_build_lut(const OConfig* cfg, const void* images, int pdt, int count, in
                   int fourier);
 /* FFT2D::Apply and FourierRadialProfile (cputrack/cpu_tracker.cpp:920-963), in place on img */
 void oq_power_spectrum_2d(fl* img, int w, int h);
+
+/* Bead finder (cputrack/BeadFinder.cpp:163-217): the correlation map (real part, [H][W] at the
+ * frame's next power-of-two size) and the positions found. */
+void oq_bead_xcor_map(const fl* img, int w, int h, int sx, int sy, int roi, fl* map);
+int oq_find_beads(const fl* img, int w, int h, int sx, int sy, int roi, fl img_distance, fl similarity, int* xy,
+                  int max_out);
 void oq_fourier_radial_profile(fl* img, int w, int h, fl* dst, int nbins, int nang);
 void oq_finalize_lut(float* zlut, int rows, int res);
 

@@ -549,6 +549,28 @@ def generate_images_from_lut(luts, pos, lut_index, width, height, minr, maxr, sp
     return out
 
 
+def find_beads(img, sample_corner, roi, img_rel_dist, acceptance, want_sample=False):
+    """QTrkFindBeads (cputrack/qtrk_c_api.cpp:33-58) on a 2-D float frame: the beads' ROI corners as
+    an (n, 2) int32 array (x, y), in the library's order; with want_sample also the roi x roi sample
+    block it copied out."""
+    im = np.ascontiguousarray(img, np.float32)
+    h, w = im.shape
+    idata = ImageData(_fptr(im), w, h)
+    smp = np.zeros((roi, roi), np.float32)
+    sdata = ImageData(_fptr(smp), roi, roi)
+    n = C.c_int(0)
+    L = lib()
+    p = L.QTrkFindBeads(C.byref(idata), int(sample_corner[0]), int(sample_corner[1]), int(roi),
+                        float(img_rel_dist), float(acceptance), C.byref(n), C.byref(sdata) if want_sample else None)
+    check()
+    try:
+        arr = C.cast(p, C.POINTER(ROIPosition * max(n.value, 1))).contents
+        out = np.array([(arr[i].x, arr[i].y) for i in range(n.value)], np.int32).reshape(-1, 2)
+    finally:
+        L.QTrkFreeROIPositions(p)
+    return (out, smp) if want_sample else out
+
+
 class LUTMatch:
     """Grouped radial Z-LUT match of many profiles per bead (include/qtrk_amd.h)."""
 

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqtrk.so")
 
-SOURCES = ["localize.hip", "lutmatch.hip", "lutmodel.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp",
+SOURCES = ["localize.hip", "beadfinder.hip", "lutmatch.hip", "lutmodel.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp",
            "lutmodel_host.cpp", "pipeline_kernels.hip", "pipeline.cpp", "tracker_common.cpp", "result_manager.cpp"]
 
 HIPCC_FLAGS = [

@@ -2,6 +2,7 @@
 // include/qtrk_amd.h).  Every entry point forwards to the tracker inside a
 // guard that turns exceptions into the thread's last error.
 #include "QueuedTracker.h"
+#include "beadfinder.hpp"
 #include "errors.hpp"
 #include "hip_tracker.hpp"
 #include "qtrk_amd.h"
@@ -235,11 +236,24 @@ QTRK_API void QTrkGetConfigValues(QueuedTracker* q, char* dst, int maxStrLen)
 }
 
 QTRK_API void QTrkFreeROIPositions(ROIPosition* data) { delete[] data; }
-QTRK_API ROIPosition* QTrkFindBeads(ImageData*, int, int, int, float, float, int* beadcount, ImageData*)
+// QTrkFindBeads (cputrack/qtrk_c_api.cpp:33-58): the sample is the roi x roi block of the frame at
+// (smpCornerPosX, smpCornerPosY), copied into `sample` when its size is roi x roi; the result is
+// a new[] array for QTrkFreeROIPositions.  Returns NULL with *beadcount = 0 on an error.
+QTRK_API ROIPosition* QTrkFindBeads(ImageData* imgData, int smpCornerPosX, int smpCornerPosY, int roi,
+                                    float imgRelDist, float acceptance, int* beadcount, ImageData* sample)
 {
 	if (beadcount) *beadcount = 0;
-	set_last_error("QTrkFindBeads: bead finding is not part of this library");
-	return nullptr;
+	ROIPosition* out = nullptr;
+	GUARD({
+		if (!imgData || !beadcount) fail("null argument");
+		float* smp = (sample && sample->data && sample->w == roi && sample->h == roi) ? sample->data : nullptr;
+		std::vector<ROIPosition> found = find_beads(imgData->data, imgData->w, imgData->h, smpCornerPosX,
+		                                            smpCornerPosY, roi, imgRelDist, acceptance, smp);
+		out = new ROIPosition[found.size() ? found.size() : 1];
+		std::copy(found.begin(), found.end(), out);
+		*beadcount = (int)found.size();
+	});
+	return out;
 }
 
 // ------------------------------------------------------------ device batch

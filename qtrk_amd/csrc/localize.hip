@@ -24,6 +24,7 @@
 // The 2D Gaussian MLE alone on ROIs of up to 1,024 pixels runs instead in
 // gauss_wave_kernel, one wavefront per ROI (the same computation, every sum
 // inside the wavefront).
+#include "fft_device.hpp"
 #include "localize.hpp"
 #include "peakfit.hpp"
 
@@ -165,12 +166,6 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 
 size_t localize_lds_bytes(const LocalizeParams& p) { return (size_t)lds_layout(p).total; }
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b)
-{
-	return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 
 // Truncation toward zero with the hardware's defined saturation (out-of-range
 // values clamp to INT_MIN/INT_MAX, NaN gives 0); unlike a C++ cast it has no
@@ -505,32 +500,6 @@ __device__ __forceinline__ void wave_sync()
 	__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 	__builtin_amdgcn_wave_barrier();
 	__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// One radix-4 or radix-2 butterfly of kissfft's decimation in time (the same operations as
-// fft_stages): G is the group, k the butterfly, m the group's sub-length, k1 = k * fstride.
-__device__ __forceinline__ void bfly_pow2(float2* G, int r, int k, int m, int k1, const float2* tw, float nii)
-{
-	if (r == 4) {
-		const float2 s0 = cmul(G[k + m], tw[k1]);
-		const float2 s1 = cmul(G[k + 2 * m], tw[k1 << 1]);
-		const float2 s2 = cmul(G[k + 3 * m], tw[k1 + (k1 << 1)]);
-		float2 f0 = G[k];
-		const float2 s5 = csub(f0, s1);
-		f0 = cadd(f0, s1);
-		const float2 s3 = cadd(s0, s2);
-		float2 s4 = csub(s0, s2);
-		s4 = make_float2(s4.y * nii, -s4.x * nii);
-		G[k + 2 * m] = csub(f0, s3);
-		G[k] = cadd(f0, s3);
-		G[k + m] = cadd(s5, s4);
-		G[k + 3 * m] = csub(s5, s4);
-	} else {
-		const float2 tt = cmul(G[m + k], tw[k1]);
-		const float2 f0 = G[k];
-		G[m + k] = csub(f0, tt);
-		G[k] = cadd(f0, tt);
-	}
 }
 
 // One power-of-two transform (QI's lengths: radices 4 and 2, group and twiddle indices by

@@ -174,6 +174,20 @@ def make_workload(name, n=None, seed=None, **overrides):
                 render_luts=render, periods=periods, rois=rois, jobs=jobs, truth=pos)
 
 
+def bead_frame(w, h, centers, period=6.0, sigma=8.0, bg=100.0, amp=60.0, noise=0.0, seed=0):
+    """A w x h float32 camera frame with radially symmetric beads (a cosine of the radius, of the
+    given period, under a Gaussian envelope) centred at `centers` (float x, y), on a flat
+    background, plus Gaussian noise of the given sigma."""
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    img = np.full((h, w), bg, np.float64)
+    for cx, cy in centers:
+        r = np.hypot(xx - cx, yy - cy)
+        img += amp * np.cos(2 * np.pi * r / period) * np.exp(-(r / sigma) ** 2)
+    if noise:
+        img += np.random.default_rng(seed).normal(0, noise, img.shape)
+    return img.astype(np.float32)
+
+
 def gaussian_spots(pos, sigma, I0, bg, w, h):
     """Pixel-integrated Gaussian spots, GenerateGaussianSpotImage (cputrack/utils.cpp:393-402), float64."""
     from scipy.special import erf
