@@ -433,7 +433,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--workload", default="realtime100",
-                    choices=["realtime100", "plain64", "queue", "precision", "mle", "batching"])
+                    choices=["realtime100", "plain64", "queue", "precision", "mle", "batching", "findbeads"])
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -454,8 +454,48 @@ def main():
     if a.workload == "queue":
         queue_bench(a)
         return
+    if a.workload == "findbeads":
+        findbeads_bench(a)
+        return
 
     localize_bench(a, a.workload)
+
+
+def findbeads_bench(a):
+    """QTrkFindBeads on a 1,600 x 1,200 float frame with 120 beads and a 40 x 40 sample, padded
+    to 2,048 x 2,048: GPU transforms and re-centring plus the host scan, timed
+    per call from host memory; the oracle's restatement (one core) is the CPU baseline."""
+    from qtrk_amd import api, synth
+
+    w, h, roi = 1600, 1200, 40
+    rng = np.random.default_rng(1600)
+    cent = [(roi + roi // 2 + c * 2 * roi + rng.uniform(-3, 3), roi + roi // 2 + r * 2 * roi + rng.uniform(-3, 3))
+            for r in range(6) for c in range(20)]
+    img = synth.bead_frame(w, h, cent, noise=3.0, seed=1600)
+    sx, sy = int(cent[0][0]) - roi // 2, int(cent[0][1]) - roi // 2
+    for _ in range(max(1, a.warmup)):
+        got = api.find_beads(img, (sx, sy), roi, 0.8, 0.45)
+    steps = max(1, min(a.steps, 20))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = api.find_beads(img, (sx, sy), roi, 0.8, 0.45)
+    t = (time.perf_counter() - t0) / steps
+    out = {"metric": "QTrkFindBeads calls/sec (1600x1200 float frame, 120 beads)", "value": round(1 / t, 2),
+           "unit": "calls/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup, "ms_per_step": round(t * 1e3, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "findbeads: 1600x1200 f32 frame padded to 2048x2048, roi 40, "
+                                  "img_rel_dist 0.8, acceptance 0.45", "beads_found": int(len(got)),
+                      "beads_planted": len(cent)}}
+    if not a.no_cpu:
+        from oracle.bind import Oracle
+        o = Oracle()
+        t0 = time.perf_counter()
+        ref = o.find_beads(img, sx, sy, roi, 0.8, 0.45)
+        tc = time.perf_counter() - t0
+        out["config"]["equal_to_oracle"] = bool(np.array_equal(ref, got))
+        out["cpu_baseline"] = {"value": round(1 / tc, 3), "unit": "calls/s", "cores": 1, "kind": "port",
+                               "sample": "one call of the same frame through the oracle's restatement"}
+    print(json.dumps(out), flush=True)
 
 
 def queue_bench(a):

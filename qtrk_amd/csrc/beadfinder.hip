@@ -43,7 +43,7 @@ constexpr int kChunk = 4096;  // floats staged per step of the ordered sum
 // stage the data into LDS one chunk ahead; lane 0 of wave 0 adds.
 __global__ __launch_bounds__(kThreads) void ordered_mean_kernel(const float* data, int n, float* out)
 {
-	__shared__ float buf[2][kChunk];
+	__shared__ __align__(16) float buf[2][kChunk];
 	const int tid = threadIdx.x;
 	const int nchunks = (n + kChunk - 1) / kChunk;
 	if (tid >= 64)
@@ -57,17 +57,31 @@ __global__ __launch_bounds__(kThreads) void ordered_mean_kernel(const float* dat
 			for (int i = tid - 64; i < len; i += kThreads - 64) buf[(c + 1) & 1][i] = data[nxt + i];
 		}
 		if (tid == 0) {
+			// 16 values a step, the next 16 read while these are added: the additions wait on
+			// each other only, not on the LDS
 			const float* b = buf[c & 1];
-			const int len = min(kChunk, n - c * kChunk);
-			int i = 0;
-			for (; i + 8 <= len; i += 8) {
-				float v[8];
-#pragma unroll
-				for (int u = 0; u < 8; u++) v[u] = b[i + u];
-#pragma unroll
-				for (int u = 0; u < 8; u++) s += v[u];
+			const float4* b4 = (const float4*)b;
+			const int len = min(kChunk, n - c * kChunk), nq = len >> 4;
+			float4 a0, a1, a2, a3;
+			if (nq > 0) {
+				a0 = b4[0];
+				a1 = b4[1];
+				a2 = b4[2];
+				a3 = b4[3];
 			}
-			for (; i < len; i++) s += b[i];
+			for (int q = 0; q < nq; q++) {
+				const int nx = q + 1 < nq ? 4 * (q + 1) : 4 * q;
+				const float4 c0 = b4[nx], c1 = b4[nx + 1], c2 = b4[nx + 2], c3 = b4[nx + 3];
+				s += a0.x; s += a0.y; s += a0.z; s += a0.w;
+				s += a1.x; s += a1.y; s += a1.z; s += a1.w;
+				s += a2.x; s += a2.y; s += a2.z; s += a2.w;
+				s += a3.x; s += a3.y; s += a3.z; s += a3.w;
+				a0 = c0;
+				a1 = c1;
+				a2 = c2;
+				a3 = c3;
+			}
+			for (int i = nq << 4; i < len; i++) s += b[i];
 		}
 		__syncthreads();
 	}
