@@ -23,6 +23,17 @@ QTRK_API void QTrkClearLastError(void);
 QTRK_API void QTrkSetConfigValue(QueuedTracker* qtrk, const char* name, const char* value);
 QTRK_API void QTrkGetConfigValues(QueuedTracker* qtrk, char* dst, int maxStrLen);
 
+/* QueuedTracker::SetImageZLUT / GetImageZLUT / GetImageZLUTSize
+ * (cputrack/QueuedTracker.h:58-60), which the reference's C ABI does not export and
+ * LabVIEW calls through the C++ interface (cputrack/lv_queuetrk_api.cpp:159-175).
+ * dims = [beads, planes, height, width]; src null gives a zeroed image LUT; the radial
+ * LUT is set alongside from radial_zlut ([beads][planes][zlut_radialsteps], null:
+ * zeroed).  BuildLUT with BUILDLUT_IMAGELUT adds into plane `plane` of it.  Set
+ * returns 1 (0 on an error). */
+QTRK_API int QTrkSetImageZLUT(QueuedTracker* qtrk, float* src, float* radial_zlut, int* dims);
+QTRK_API void QTrkGetImageZLUT(QueuedTracker* qtrk, float* dst);
+QTRK_API void QTrkGetImageZLUTSize(QueuedTracker* qtrk, int* dims);
+
 /* Name, compute units and clock of the GPU the library runs on. */
 QTRK_API int QTrkGetDeviceInfo(char* dst, int maxStrLen);
 

@@ -74,6 +74,7 @@ class Oracle:
             "oq_localize_trace": (None, [P, C.c_int, P, C.c_int, C.c_int, P, P, C.c_int, P, C.c_int, P, P, P]),
             "oq_build_lut": (None, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int, C.c_int]),
             "oq_power_spectrum_2d": (None, [P, C.c_int, C.c_int]),
+            "oq_build_image_lut": (None, [P, P, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int, C.c_int, C.c_int]),
             "oq_bead_xcor_map": (None, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]),
             "oq_find_beads": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fl, fl, P, C.c_int]),
             "oq_fourier_radial_profile": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int]),
@@ -321,6 +322,16 @@ class Oracle:
         self.L.oq_build_lut(C.byref(cfg), _p(images), pdt, len(images), int(plane), None if kp is None else _p(kp),
                             1 if normalize else 0, None if off is None else _p(off),
                             None if gain is None else _p(gain), _p(zlut), zlut.shape[1], 1 if fourier else 0)
+
+    def build_image_lut(self, cfg, images, plane, image_lut, known_pos=None):
+        """Adds plane `plane` of every bead's image LUT (float32 [beads][planes][h][w]) in place."""
+        images = np.ascontiguousarray(images)
+        pdt = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[images.dtype]
+        kp = None if known_pos is None else np.ascontiguousarray(known_pos, np.float32)
+        assert image_lut.dtype == np.float32 and image_lut.flags.c_contiguous
+        _, planes, ih, iw = image_lut.shape
+        self.L.oq_build_image_lut(C.byref(cfg), _p(images), pdt, len(images), int(plane),
+                                  None if kp is None else _p(kp), None, None, _p(image_lut), planes, ih, iw)
 
     def finalize_lut(self, zlut):
         self.L.oq_finalize_lut(_p(zlut), zlut.shape[0] * zlut.shape[1], zlut.shape[2])

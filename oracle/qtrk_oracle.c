@@ -1644,3 +1644,63 @@ int oq_find_beads(const fl* img, int w, int h, int sx, int sy, int roi, fl img_d
 	free(map);
 	return nk;
 }
+
+/* Interpolate (cputrack/utils.h:40-58): int conversion by truncation (x86 gives INT_MIN for NaN and
+ * out-of-range values, i.e. "outside"), 0 outside. */
+static fl interp_ref(const fl* img, int w, int h, fl x, fl y)
+{
+	if (!(x > -2147483648.0 && x < 2147483648.0 && y > -2147483648.0 && y < 2147483648.0)) return 0;
+	const int rx = (int)x, ry = (int)y;
+	if (rx < 0 || ry < 0 || rx + 1 >= w || ry + 1 >= h) return 0;
+	const fl v00 = img[w * ry + rx], v10 = img[w * ry + rx + 1];
+	const fl v01 = img[w * (ry + 1) + rx], v11 = img[w * (ry + 1) + rx + 1];
+	const fl v0 = v00 + (v10 - v00) * (x - (fl)rx), v1 = v01 + (v11 - v01) * (x - (fl)rx);
+	return v0 + (v1 - v0) * (y - (fl)ry);
+}
+
+/* BuildLUT's BUILDLUT_IMAGELUT part (cputrack/QueuedCPUTracker.cpp:582-624): for bead i, its position
+ * (the known one, or COM then QI as oq_build_lut) and the tracker's mean (0 with a known position,
+ * where ComputeMeanAndCOM is not called), then image_lut[i][plane][y][x] += Interpolate at
+ * (pos.x - iw/2 + x, pos.y - ih/2 + y) - mean. */
+void oq_build_image_lut(const OConfig* cfg, const void* images, int pdt, int count, int plane, const float* known_pos,
+                        const float* calib_offset, const float* calib_gain, float* image_lut, int planes, int ih,
+                        int iw)
+{
+	OTracker t;
+	tracker_setup(&t, cfg, 0, 0, count, 1, 0);
+	OWorker wk;
+	oq_worker_init(&wk, &t);
+	size_t roibytes = (size_t)cfg->width * cfg->height * pdt_bytes(pdt);
+	for (int i = 0; i < count; i++) {
+		oq_set_image(&wk, (const char*)images + roibytes * i, cfg->width * pdt_bytes(pdt), pdt);
+		if (calib_offset || calib_gain) {
+			size_t idx = (size_t)cfg->width * cfg->height * i;
+			for (int k = 0; k < wk.w * wk.h; k++) {
+				if (calib_offset && !calib_gain) wk.img[k] = wk.img[k] + calib_offset[idx + k] * 1.0f;
+				if (calib_gain && !calib_offset) wk.img[k] = wk.img[k] * calib_gain[idx + k] * 1.0f;
+				if (calib_gain && calib_offset) wk.img[k] = (wk.img[k] + calib_offset[idx + k] * 1.0f) * calib_gain[idx + k] * 1.0f;
+			}
+		}
+		fl px, py, mean = 0;
+		if (known_pos) {
+			px = known_pos[2 * i];
+			py = known_pos[2 * i + 1];
+		} else {
+			fl cx, cy;
+			int bh;
+			oq_com(&wk, 0, &cx, &cy);
+			mean = wk.mean;
+			oq_qi(&wk, cx, cy, cfg->qi_iterations, cfg->qi_radialsteps, cfg->qi_angstepspq, cfg->qi_angstep_factor,
+			      (fl)cfg->qi_minradius, (fl)cfg->qi_maxradius, 0, &px, &py, &bh, 0);
+		}
+		float* dst = image_lut + ((size_t)i * planes + plane) * ih * iw;
+		const fl x0 = px - (fl)(iw / 2) * (fl)1.0f, y0 = py - (fl)(ih / 2) * (fl)1.0f;
+		for (int y = 0; y < ih; y++)
+			for (int x = 0; x < iw; x++) {
+				const fl v = interp_ref(wk.img, wk.w, wk.h, x0 + (fl)x * (fl)1.0f, y0 + (fl)y * (fl)1.0f);
+				dst[y * iw + x] = (float)(dst[y * iw + x] + (v - mean));
+			}
+	}
+	oq_worker_free(&wk);
+	free(t.qi_window);
+}

@@ -152,6 +152,9 @@ void oq_power_spectrum_2d(fl* img, int w, int h);
 
 /* Bead finder (cputrack/BeadFinder.cpp:163-217): the correlation map (real part, [H][W] at the
  * frame's next power-of-two size) and the positions found. */
+void oq_build_image_lut(const OConfig* cfg, const void* images, int pdt, int count, int plane, const float* known_pos,
+                        const float* calib_offset, const float* calib_gain, float* image_lut, int planes, int ih,
+                        int iw);
 void oq_bead_xcor_map(const fl* img, int w, int h, int sx, int sy, int roi, fl* map);
 int oq_find_beads(const fl* img, int w, int h, int sx, int sy, int roi, fl img_distance, fl similarity, int* xy,
                   int max_out);

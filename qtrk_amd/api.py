@@ -153,6 +153,9 @@ SIGNATURES = {
     "QTrkGetProfileReport": (None, [_P, C.c_char_p, C.c_int]),
     "QTrkGetWarnings": (None, [_P, C.c_char_p, C.c_int]),
     "QTrkFreeROIPositions": (None, [_P]),
+    "QTrkSetImageZLUT": (C.c_int, [_P, _FP, _FP, _IP]),
+    "QTrkGetImageZLUT": (None, [_P, _FP]),
+    "QTrkGetImageZLUTSize": (None, [_P, _IP]),
     "QTrkFindBeads": (_P, [C.POINTER(ImageData), C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _IP,
                            C.POINTER(ImageData)]),
     "ComputeRadialProfile": (None, [_FP, C.c_int, C.c_int, C.c_float, C.c_float, Vector2f,
@@ -335,6 +338,26 @@ class QTrkInstance:
         lut = np.ascontiguousarray(lut, dtype=np.float32)
         assert lut.ndim == 3 and lut.shape[2] == self.cfg.zlut_radialsteps, lut.shape
         self._call("QTrkSetRadialZLUT", _ptr(lut), lut.shape[0], lut.shape[1])
+
+    def set_image_zlut(self, image_lut, radial=None):
+        """image_lut: float32 [beads][planes][height][width], or its 4 dims to allocate a zeroed one;
+        radial: float32 [beads][planes][zlut_radialsteps] or None (zeroed)."""
+        if isinstance(image_lut, tuple):
+            dims, src = np.array(image_lut, np.int32), None
+        else:
+            src = np.ascontiguousarray(image_lut, np.float32)
+            dims = np.array(src.shape, np.int32)
+        rad = None if radial is None else np.ascontiguousarray(radial, np.float32)
+        return bool(self._call("QTrkSetImageZLUT", None if src is None else _fptr(src),
+                               None if rad is None else _fptr(rad), dims.ctypes.data_as(_IP)))
+
+    def get_image_zlut(self):
+        dims = np.zeros(4, np.int32)
+        self._call("QTrkGetImageZLUTSize", dims.ctypes.data_as(_IP))
+        out = np.zeros(tuple(int(d) for d in dims), np.float32)
+        if out.size:
+            self._call("QTrkGetImageZLUT", _fptr(out))
+        return out
 
     def get_radial_zlut_size(self):
         c, p, r = C.c_int(), C.c_int(), C.c_int()

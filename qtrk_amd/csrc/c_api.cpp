@@ -222,6 +222,15 @@ QTRK_API void QTrkGetProfileReport(QueuedTracker* q, char* dst, int maxStrLen)
 }
 QTRK_API void QTrkGetWarnings(QueuedTracker* q, char* dst, int maxStrLen) { GUARD(copy_str(q->GetWarnings(), dst, maxStrLen)); }
 
+QTRK_API int QTrkSetImageZLUT(QueuedTracker* q, float* src, float* radial_zlut, int* dims)
+{
+	int ok = 0;
+	GUARD(ok = q->SetImageZLUT(src, radial_zlut, dims) ? 1 : 0);
+	return ok;
+}
+QTRK_API void QTrkGetImageZLUT(QueuedTracker* q, float* dst) { GUARD(q->GetImageZLUT(dst)); }
+QTRK_API void QTrkGetImageZLUTSize(QueuedTracker* q, int* dims) { GUARD(q->GetImageZLUTSize(dims)); }
+
 QTRK_API void QTrkSetConfigValue(QueuedTracker* q, const char* name, const char* value)
 {
 	GUARD(q->SetConfigValue(name ? name : "", value ? value : ""));

@@ -867,7 +867,9 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
 	WaitAllLocked(lk);
-	if (buildFlags & BUILDLUT_IMAGELUT) fail("BUILDLUT_IMAGELUT is not implemented");
+	const bool ilut = (buildFlags & BUILDLUT_IMAGELUT) && ilut_dims[2] * ilut_dims[3] > 0;
+	if (ilut && (ilut_dims[0] < zcount || plane >= ilut_dims[1]))
+		fail("BUILDLUT_IMAGELUT: the image LUT has fewer beads or planes than the build needs");
 	if ((buildFlags & BUILDLUT_FOURIER) && fx_fft.n == 0)
 		fail("BUILDLUT_FOURIER needs a ROI of at most " + std::to_string(kMaxFFTLen) + " pixels a side");
 	if (zcount <= 0) fail("BuildLUT needs a LUT allocated with SetRadialZLUT");
@@ -896,12 +898,61 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 		upload(d_build_pos, known_pos, sizeof(vector2f) * zcount);
 		P.known_pos = d_build_pos.as<float>();
 	}
+	if (ilut) {
+		P.image_lut = d_image_lut.as<float>();
+		P.ilut_planes = ilut_dims[1];
+		P.ilut_h = ilut_dims[2];
+		P.ilut_w = ilut_dims[3];
+	}
 	HIP_CHECK(launch_localize(P, aux_stream()));
 	HIP_CHECK(hipStreamSynchronize(aux_stream()));
 	h_zlut_stale = true;
 }
 
-// Normalizes every plane profile (cputrack/QueuedCPUTracker.cpp:648-659).
+// QueuedCPUTracker::GetImageZLUTSize / GetImageZLUT / SetImageZLUT
+// (cputrack/QueuedCPUTracker.cpp:519-531, 548-570): dims = [beads, planes, height,
+// width]; a null src gives a zeroed image LUT; the radial LUT is set alongside with
+// [beads][planes] of radial_zlut (null: zeroed).
+void QueuedHIPTracker::GetImageZLUTSize(int* dims)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	for (int i = 0; i < 4; i++) dims[i] = ilut_dims[i];
+}
+
+void QueuedHIPTracker::GetImageZLUT(float* dst)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	WaitAllLocked(lk);
+	const size_t n = (size_t)ilut_dims[0] * ilut_dims[1] * ilut_dims[2] * ilut_dims[3];
+	if (n && d_image_lut.p) HIP_CHECK(hipMemcpy(dst, d_image_lut.p, n * 4, hipMemcpyDeviceToHost));
+}
+
+bool QueuedHIPTracker::SetImageZLUT(float* src, float* radial_zlut, int* dims)
+{
+	if (!dims) fail("null dims");
+	for (int i = 0; i < 4; i++)
+		if (dims[i] < 0) fail("image LUT dimensions must be >= 0");
+	{
+		std::unique_lock<std::mutex> lk(mtx);
+		HIP_CHECK(hipSetDevice(dev));
+		WaitAllLocked(lk);
+		const size_t n = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
+		d_image_lut.release();
+		for (int i = 0; i < 4; i++) ilut_dims[i] = dims[i];
+		if (n) {
+			d_image_lut.ensure(n * 4);
+			if (src) HIP_CHECK(hipMemcpy(d_image_lut.p, src, n * 4, hipMemcpyHostToDevice));
+			else HIP_CHECK(hipMemset(d_image_lut.p, 0, n * 4));
+		}
+	}
+	SetRadialZLUT(radial_zlut, dims[0], dims[1]);
+	return true;
+}
+
+// Normalizes every plane profile (cputrack/QueuedCPUTracker.cpp:648-659).  The reference
+// also computes the image LUT's first and second differences across planes there
+// (:661-696); no interface reads them back, so they are not computed.
 void QueuedHIPTracker::FinalizeLUT()
 {
 	std::unique_lock<std::mutex> lk(mtx);

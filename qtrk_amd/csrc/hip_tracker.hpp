@@ -55,6 +55,9 @@ public:
 	void BeginLUT(uint flags) override;
 	void BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, int plane, vector2f* known_pos = 0) override;
 	void FinalizeLUT() override;
+	void GetImageZLUTSize(int* dims) override;
+	void GetImageZLUT(float* dst) override;
+	bool SetImageZLUT(float* src, float* radial_zlut, int* dims) override;
 	int GetResultCount() override;
 	int FetchResults(LocalizationResult* results, int maxResults) override;
 	int GetQueueLength(int* maxQueueLen = 0) override;
@@ -142,6 +145,9 @@ private:
 	LocMode_t localizeMode = 0;
 	uint buildFlags = 0;
 	DeviceBuffer d_build_rois, d_build_pos;
+	// the image LUT, [beads][planes][height][width] as ilut_dims (QueuedTracker.h's order)
+	DeviceBuffer d_image_lut;
+	int ilut_dims[4] = { 0, 0, 0, 0 };
 	PinnedBuffer h_build_rois;
 
 	int batchSize = 1024, numSlots = 4;

@@ -352,6 +352,21 @@ __device__ __forceinline__ bool circle_inside(float cx, float cy, float r, float
 	return cx - a >= 1.0f && cx + a <= Wf - 2.0f && cy - a >= 1.0f && cy + a <= Hf - 2.0f;
 }
 
+// Interpolate (cputrack/utils.h:40-58): the reference's int conversion truncates toward zero
+// (and gives INT_MIN, so "outside", for NaN and out-of-range values); 0 outside the ROI.
+template <typename SPix>
+__device__ __forceinline__ float interpolate_ref(const SPix* img, int W, int H, float x, float y)
+{
+	if (!(x > -2147483648.0f && x < 2147483648.0f && y > -2147483648.0f && y < 2147483648.0f)) return 0.0f;
+	const int rx = (int)x, ry = (int)y;
+	if (rx < 0 || ry < 0 || rx + 1 >= W || ry + 1 >= H) return 0.0f;
+	const float v00 = (float)img[W * ry + rx], v10 = (float)img[W * ry + rx + 1];
+	const float v01 = (float)img[W * (ry + 1) + rx], v11 = (float)img[W * (ry + 1) + rx + 1];
+	const float fx = x - (float)rx, fy = y - (float)ry;
+	const float v0 = v00 + (v10 - v00) * fx, v1 = v01 + (v11 - v01) * fx;
+	return v0 + (v1 - v0) * fy;
+}
+
 __device__ __forceinline__ void wave_argmax(float& v, int& i)
 {
 	for (int off = 32; off > 0; off >>= 1) {
@@ -1229,6 +1244,19 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		}
 		cx = px;
 		cy = py;
+	}
+
+	// ---- BUILDLUT_IMAGELUT (cputrack/QueuedCPUTracker.cpp:606-624): the ROI sampled 1 px apart
+	// on an ilut_w x ilut_h grid centred on the build position, minus the mean, added into the
+	// bead's plane; before the profile, while the ROI is in LDS unchanged
+	if (build && P.image_lut) {
+		const int iw = P.ilut_w, ih = P.ilut_h;
+		float* dst = P.image_lut + ((size_t)roi * P.ilut_planes + P.build_plane) * ih * iw;
+		const float x0 = cx - (float)(iw / 2) * 1.0f, y0 = cy - (float)(ih / 2) * 1.0f;
+		for (int i = tid; i < iw * ih; i += kBlock) {
+			const int y = i / iw, x = i - y * iw;
+			dst[i] += interpolate_ref(img, W, H, x0 + (float)x * 1.0f, y0 + (float)y * 1.0f) - mean;
+		}
 	}
 
 	// ---- radial profile (cputrack/utils.cpp:275-320) and normalisation (:230-261), LUT

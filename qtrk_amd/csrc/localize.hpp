@@ -72,6 +72,10 @@ struct LocalizeParams {
 	int build_plane;
 	const float* known_pos;  // [n][2] or null
 	float* build_lut;
+	// BUILDLUT_IMAGELUT: image_lut[roi][ilut_planes][ilut_h][ilut_w], plane build_plane
+	// accumulated; null when the flag is off
+	float* image_lut;
+	int ilut_planes, ilut_h, ilut_w;
 
 	// QI
 	int qi_iters, qi_nr, qi_A;
