@@ -51,17 +51,14 @@ class CopyPool {
 public:
 	explicit CopyPool(int helpers, bool streaming = false) : helpers_(helpers < 0 ? 0 : helpers), streaming_(streaming)
 	{
-		for (int i = 0; i < helpers_; i++) threads_.emplace_back([this] { Run(); });
-	}
-	~CopyPool()
-	{
-		{
-			std::lock_guard<std::mutex> g(m_);
-			stop_ = true;
+		try {
+			for (int i = 0; i < helpers_; i++) threads_.emplace_back([this] { Run(); });
+		} catch (...) {
+			Stop();  // the threads already started must end before the members go
+			throw;
 		}
-		cv_.notify_all();
-		for (auto& t : threads_) t.join();
 	}
+	~CopyPool() { Stop(); }
 	CopyPool(const CopyPool&) = delete;
 	CopyPool& operator=(const CopyPool&) = delete;
 
@@ -97,6 +94,17 @@ public:
 
 private:
 	static constexpr size_t kMinSplit = 1 << 20;
+
+	void Stop()
+	{
+		{
+			std::lock_guard<std::mutex> g(m_);
+			stop_ = true;
+		}
+		cv_.notify_all();
+		for (auto& t : threads_) t.join();
+		threads_.clear();
+	}
 
 	void CopyPart(int p)
 	{
