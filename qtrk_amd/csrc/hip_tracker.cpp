@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <sstream>
+#include <thread>
 
 namespace qtrk {
 
@@ -350,9 +351,12 @@ QueuedHIPTracker::Slot* QueuedHIPTracker::AcquireSlot(std::unique_lock<std::mute
 		for (Slot* s : slots)
 			if (s->state == Slot::Launched && (!oldest || s->seq < oldest->seq)) oldest = s;
 		if (!oldest) fail("no batch slot available");
+		// polled rather than hipEventSynchronize: the caller is blocked either way, and a poll
+		// resumes filling as soon as the batch is done instead of after a wake-up
 		hipEvent_t ev = oldest->ev_done;
 		lk.unlock();
-		hipError_t e = hipEventSynchronize(ev);
+		hipError_t e;
+		while ((e = hipEventQuery(ev)) == hipErrorNotReady) std::this_thread::yield();
 		lk.lock();
 		HIP_CHECK(e);
 	}
