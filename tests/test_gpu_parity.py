@@ -385,6 +385,34 @@ def test_schedule_frame_packs_frames_into_batches(gpu):
         assert np.array_equal(r[key], ref[key]), key
 
 
+def test_frame_copy_threads_config(gpu):
+    """frame_copy_threads sets how many threads copy a frame into pinned memory: results do not
+    depend on it (a 1.5 MB row block, copied by 1 and by 8 threads), and it is range-checked."""
+    w = synth.make_workload("realtime100", n=32)
+    frame = np.zeros((1000, 1600), np.uint16)
+    pos = [(50 * (i % 8) * 4, 120 * (i // 8)) for i in range(32)]
+    for i, (x, y) in enumerate(pos):
+        frame[y:y + 100, x:x + 100] = w["rois"][i]
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_radial_zlut(w["luts"][:32])
+    trk.set_localization_mode(w["mode"])
+    assert trk.get_config_values()["frame_copy_threads"] == "5"
+    job = api.make_jobs(1, frame=np.array([3]), zlut_index=np.array([0]))[0]
+    out = []
+    for th in ("1", "8"):
+        trk.set_config_value("frame_copy_threads", th)
+        assert trk.get_config_values()["frame_copy_threads"] == th
+        assert trk.schedule_frame(frame, pos, job) == 32
+        trk.wait_for(32)
+        r = trk.fetch_results(32)
+        out.append(r[np.argsort(r["zlutIndex"])])
+    for bad in ("0", "65"):
+        with pytest.raises(api.QTrkError, match="frame_copy_threads"):
+            trk.set_config_value("frame_copy_threads", bad)
+    trk.free()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_build_lut_matches_oracle(gpu, oracle):
     w = synth.make_workload("plain64", n=8)
     cfg = w["cfg"]
