@@ -85,6 +85,9 @@ class Oracle:
             "oq_fisher_ref": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl, fl, fl, fl,
                                      fl]),
             "oq_erf": (fl, [fl]),
+            "oq_qm_expf": (C.c_float, [C.c_float]),
+            "oq_qm_erff": (C.c_float, [C.c_float]),
+            "oq_qm_check": (C.c_long, [C.c_int, C.c_float, C.c_int, P, P, P]),
             "oq_lut_compare_weighted": (fl, [P, P, C.c_int, C.c_int, fl, fl, fl, P, P]),
             "oq_zbias_apply": (C.c_float, [P, C.c_int, C.c_int, C.c_float]),
             "oq_compute_zbias_table": (None, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P]),
@@ -112,6 +115,14 @@ class Oracle:
         c = OConfig()
         C.memmove(C.byref(c), bytes(raw)[:C.sizeof(OConfig)], C.sizeof(OConfig))
         return c
+
+    # -- qmath.h (the Gaussian MLE's exp and erf)
+    def qm_check(self, which, amax, stride):
+        """Every stride-th float of magnitude <= amax through qm_expf (which 0) or qm_erff (1):
+        (count, max ulps from the exact value, max ulps from glibc's float function, worst x)."""
+        a, b, w = C.c_double(), C.c_double(), C.c_float()
+        n = self.L.oq_qm_check(which, amax, stride, C.byref(a), C.byref(b), C.byref(w))
+        return n, a.value, b.value, w.value
 
     # -- primitives
     def interpolate(self, img, x, y):

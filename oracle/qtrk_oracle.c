@@ -7,6 +7,7 @@
  */
 #define _GNU_SOURCE
 #include "qtrk_oracle.h"
+#include "qmath.h" /* qtrk_amd/csrc: the exp/erf the GPU's Gaussian MLE evaluates */
 
 #include <math.h>
 #include <limits.h>
@@ -27,6 +28,8 @@
 #define FABS fabs
 #define FACOS acos
 #define FERF erf
+#define GEXP exp
+#define GERF erf
 #else
 #define FC(x) ((fl)(x##f))
 #define FSQRT sqrtf
@@ -38,6 +41,12 @@
 #define FABS fabsf
 #define FACOS acosf
 #define FERF erff
+/* The 2D Gaussian MLE's exp and erf: qmath.h's, the functions the GPU kernel
+ * evaluates, so that both round the same.  The reference binds the C library's
+ * erff and expf (cputrack/cpu_tracker.cpp:547-553 on MSVC); qmath.h's are within
+ * 1 ulp (exp) and 2 ulp (erf) of the exact values, as a C library's are. */
+#define GEXP qm_expf
+#define GERF qm_erff
 #endif
 
 #define MIN_RADPROFILE_SMP_COUNT 4 /* cputrack/QueuedTracker.h:18 */
@@ -1419,8 +1428,13 @@ void oq_fisher_ref(fl* F, int w, int h, const float* zlut, int planes, int steps
 
 /* erf as the reference build binds it: a float argument resolves to the C
  * library's erff (the A&S template in cputrack/utils.h:395-417 loses overload
- * resolution to the non-template float overload). */
-fl oq_erf(fl x) { return FERF(x); }
+ * resolution to the non-template float overload); here qmath.h's qm_erff stands
+ * in for the C library's (FP64 build: erf). */
+fl oq_erf(fl x) { return GERF(x); }
+
+/* qmath.h's functions, exported for their tests */
+float oq_qm_expf(float x) { return qm_expf(x); }
+float oq_qm_erff(float x) { return qm_erff(x); }
 
 /* GenerateGaussianSpotImage, cputrack/utils.cpp:393-402 */
 void oq_gaussian_spot(fl* img, int w, int h, fl px, fl py, fl sigma, fl I0, fl bg)
@@ -1455,8 +1469,8 @@ void oq_gauss2d_mle(const fl* img, int w, int h, fl mean, fl x0, fl y0, int iter
 				fl DX = FC(0.5) * oq_erf(Xe0) - FC(0.5) * oq_erf(Xe1);
 				fl DY = FC(0.5) * oq_erf(Ye0) - FC(0.5) * oq_erf(Ye1);
 				fl mu = bg + I0 * DX * DY;
-				fl dmx = I0 * s2pi * (FEXP(-Xe1 * Xe1) - FEXP(-Xe0 * Xe0)) * DY;
-				fl dmy = I0 * s2pi * (FEXP(-Ye1 * Ye1) - FEXP(-Ye0 * Ye0)) * DX;
+				fl dmx = I0 * s2pi * (GEXP(-Xe1 * Xe1) - GEXP(-Xe0 * Xe0)) * DY;
+				fl dmy = I0 * s2pi * (GEXP(-Ye1 * Ye1) - GEXP(-Ye0 * Ye0)) * DX;
 				fl dmI = DX * DY;
 				fl dmB = 1;
 				fl smp = img[y * w + x];
@@ -1465,8 +1479,8 @@ void oq_gauss2d_mle(const fl* img, int w, int h, fl mean, fl x0, fl y0, int iter
 				dy += dmy * f;
 				dI += dmI * f;
 				dB += dmB * f;
-				fl d2x = I0 * s2pi3 * ((x - px - FC(0.5)) * FEXP(-Xe1 * Xe1) - (x - px + FC(0.5)) * FEXP(-Xe0 * Xe0)) * DY;
-				fl d2y = I0 * s2pi3 * ((y - py - FC(0.5)) * FEXP(-Ye1 * Ye1) - (y - py + FC(0.5)) * FEXP(-Ye0 * Ye0)) * DX;
+				fl d2x = I0 * s2pi3 * ((x - px - FC(0.5)) * GEXP(-Xe1 * Xe1) - (x - px + FC(0.5)) * GEXP(-Xe0 * Xe0)) * DY;
+				fl d2y = I0 * s2pi3 * ((y - py - FC(0.5)) * GEXP(-Ye1 * Ye1) - (y - py + FC(0.5)) * GEXP(-Ye0 * Ye0)) * DX;
 				ddx += d2x * f - dmx * dmx * smp / (mu * mu);
 				ddy += d2y * f - dmy * dmy * smp / (mu * mu);
 				ddI += -dmI * dmI * smp / (mu * mu);
@@ -1703,4 +1717,50 @@ void oq_build_image_lut(const OConfig* cfg, const void* images, int pdt, int cou
 	}
 	oq_worker_free(&wk);
 	free(t.qi_window);
+}
+
+/* ------------------------------------------------------ qmath.h checks */
+
+/* The spacing of floats at |v| (a power of two; 2^-149 among the subnormals). */
+static double ulp_at(double v)
+{
+	v = fabs(v);
+	if (v < 0x1p-126) return 0x1p-149;
+	int e;
+	frexp(v, &e); /* v = m 2^e, 0.5 <= m < 1 */
+	return ldexp(1.0, e - 24);
+}
+
+/* Every stride-th float of magnitude up to amax, both signs: the largest error of
+ * qm_expf (which = 0) or qm_erff (which = 1) in ulps of the exact value (glibc's
+ * double exp/erf stands for it) and the largest difference from glibc's float
+ * expf/erff in ulps.  Returns the number of floats checked. */
+long oq_qm_check(int which, float amax, int stride, double* max_exact, double* max_libc, float* worst)
+{
+	double me = 0, ml = 0;
+	float wx = 0;
+	long n = 0;
+	unsigned top;
+	memcpy(&top, &amax, 4);
+	for (int sign = 0; sign < 2; sign++)
+		for (unsigned u = 0; u <= top; u += (unsigned)stride) {
+			const unsigned bits = u | (sign ? 0x80000000u : 0u);
+			float x;
+			memcpy(&x, &bits, 4);
+			const double t = which == 0 ? exp((double)x) : erf((double)x);
+			const float q = which == 0 ? qm_expf(x) : qm_erff(x);
+			const float c = which == 0 ? expf(x) : erff(x);
+			const float tf = (float)t;
+			double d;
+			if (isinf(tf) || isinf(q)) d = (isinf(tf) && q == tf) ? 0 : 1e9;
+			else d = fabs((double)q - t) / ulp_at(t);
+			if (d > me) { me = d; wx = x; }
+			const double dl = (q == c) ? 0 : (isinf(q) || isinf(c)) ? 1e9 : fabs((double)q - (double)c) / ulp_at(c);
+			if (dl > ml) ml = dl;
+			n++;
+		}
+	*max_exact = me;
+	*max_libc = ml;
+	*worst = wx;
+	return n;
 }

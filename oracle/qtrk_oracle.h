@@ -166,6 +166,9 @@ void oq_bspline_weights(fl* w, fl t);
 void oq_generate_image_from_lut(fl* img, int w, int h, const float* zlut, int planes, int steps,
                                 fl minr, fl maxr, fl px, fl py, fl pz, int spline, int ovs);
 fl oq_erf(fl x);
+float oq_qm_expf(float x);
+float oq_qm_erff(float x);
+long oq_qm_check(int which, float amax, int stride, double* max_exact, double* max_libc, float* worst);
 void oq_gaussian_spot(fl* img, int w, int h, fl px, fl py, fl sigma, fl I0, fl bg);
 /* LUTProfileCompareAdjustedWeights, QuadrantAlign, ComputeXCorInterpolated (cputrack/cpu_tracker.cpp) */
 fl oq_lut_compare_weighted(const fl* prof, const float* lut, int planes, int res, fl minr, fl maxr, fl z_estim,

@@ -4,8 +4,10 @@ The library is one shared object compiled by hipcc from qtrk_amd/csrc/ and
 placed in-tree at qtrk_amd/libqtrk.so.  Every HIP source is compiled with
 -ffp-contract=off (the kernels reproduce the reference CPU tracker's rounding,
 which has no fused multiply-adds) and with packed-FP32 instruction selection
-disabled.
+disabled.  The sources compile in parallel into qtrk_amd/_build/ and are then
+linked; every call recompiles all of them.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -14,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libqtrk.so")
+OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["localize.hip", "beadfinder.hip", "lutmatch.hip", "lutmodel.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp",
            "lutmodel_host.cpp", "pipeline_kernels.hip", "pipeline.cpp", "tracker_common.cpp", "result_manager.cpp"]
@@ -22,7 +25,6 @@ HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-fPIC",
-    "-shared",
     "-std=c++17",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -39,20 +41,31 @@ def hipcc():
     return "hipcc"
 
 
-def build_library(verbose=False):
-    """Compiles every source into qtrk_amd/libqtrk.so (always rebuilds)."""
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + [
-        "-I" + os.path.join(ROOT, "include"),
-        "-I" + CSRC,
-    ] + srcs + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libqtrk.so")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd[-3:]))
+    return r
+
+
+def build_library(verbose=False, jobs=None):
+    """Compiles every source into qtrk_amd/libqtrk.so (always rebuilds)."""
+    os.makedirs(OBJDIR, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+    objs, cmds = [], []
+    for s in SOURCES:
+        o = os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o")
+        objs.append(o)
+        cmds.append([hipcc()] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, s), "-o", o])
+    jobs = jobs or min(len(cmds), max(1, min(os.cpu_count() or 1, 16)))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for f in [ex.submit(_run, c, verbose) for c in cmds]:
+            f.result()
+    tmp = LIB + ".tmp"
+    _run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

@@ -23,10 +23,13 @@
 //
 // The 2D Gaussian MLE alone on ROIs of up to 1,024 pixels runs instead in
 // gauss_wave_kernel, one wavefront per ROI (the same computation, every sum
-// inside the wavefront).
+// inside the wavefront).  Its exp and erf are qmath.h's, the same source the
+// CPU oracle compiles, so the model terms round as they do there.
 #include "fft_device.hpp"
 #include "localize.hpp"
 #include "peakfit.hpp"
+#define QM_FN __device__ static inline
+#include "qmath.h"
 
 #include <cmath>
 #include <type_traits>
@@ -1197,8 +1200,8 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				const float q = row ? py : px;
 				const float e0 = ((float)i - q + 0.5f) * s2;
 				const float e1 = ((float)i - q - 0.5f) * s2;
-				const float D = 0.5f * erff(e0) - 0.5f * erff(e1);
-				const float x1e = expf(-e1 * e1), x0e = expf(-e0 * e0);
+				const float D = 0.5f * qm_erff(e0) - 0.5f * qm_erff(e1);
+				const float x1e = qm_expf(-e1 * e1), x0e = qm_expf(-e0 * e0);
 				const float C1 = Is * (x1e - x0e);
 				const float C2 = Is3 * (((float)i - q - 0.5f) * x1e - ((float)i - q + 0.5f) * x0e);
 				if (row) {
@@ -1626,8 +1629,8 @@ __global__ void __launch_bounds__(kBlock) gauss_wave_kernel(LocalizeParams P)
 			const float q = row ? py : px;
 			const float e0 = ((float)i - q + 0.5f) * s2;
 			const float e1 = ((float)i - q - 0.5f) * s2;
-			const float D = 0.5f * erff(e0) - 0.5f * erff(e1);
-			const float x1e = expf(-e1 * e1), x0e = expf(-e0 * e0);
+			const float D = 0.5f * qm_erff(e0) - 0.5f * qm_erff(e1);
+			const float x1e = qm_expf(-e1 * e1), x0e = qm_expf(-e0 * e0);
 			const float C1 = Is * (x1e - x0e);
 			const float C2 = Is3 * (((float)i - q - 0.5f) * x1e - ((float)i - q + 0.5f) * x0e);
 			if (row) {

@@ -508,14 +508,28 @@ def test_unsupported_mode_raises(gpu):
     trk.free()
 
 
-def test_gaussian2d_mle(gpu, oracle, oracle64):
-    """LT_Gaussian2D (cputrack/cpu_tracker.cpp:514-589) on sigma=1.5 spots in 16x16 ROIs.
+def _gauss_compare(tag, got, ref, r64):
+    """x/y of the GPU against the FP32 oracle: the same NaN set and every finite value within
+    1e-3 px (measured: all bit-identical).  Both evaluate qmath.h's erf and exp, so the model
+    terms round the same; the FP32-FP64 oracle spread (the reference's own sensitivity to
+    rounding) is printed beside."""
+    for k in "xy":
+        nan = np.isnan(ref[k])
+        assert np.array_equal(np.isnan(got[k]), nan), k
+        d = np.abs(got[k][~nan] - ref[k][~nan])
+        same = int((got[k][~nan] == ref[k][~nan]).sum())
+        spread = np.nanmax(np.abs(ref[k] - r64[k]))
+        print(f"[{tag}] {k}: {same}/{d.size} bit-identical, max |GPU-oracle| {d.max():.2e} px, {int(nan.sum())} NaN; "
+              f"oracle FP32-FP64 spread {spread:.2e}", flush=True)
+        assert d.max() <= XY_TOL, k
 
-    The reference's independent (diagonal) Newton steps do not converge on every ROI, so the
-    result is sensitive to rounding: the FP32 and FP64 oracles themselves differ by up to ~0.06 px
-    on this workload.  The GPU takes erff/expf from the device math library instead of the host
-    CRT, so its agreement with the FP32 oracle is bounded by that same spread; the bulk agrees
-    to 1e-3 px and the precision against the truth is the same."""
+
+def test_gaussian2d_mle(gpu, oracle, oracle64):
+    """LT_Gaussian2D (cputrack/cpu_tracker.cpp:514-589) on sigma=1.5 spots in 16x16 ROIs: COM
+    bit-exact and x/y within 1e-3 px of the oracle on every ROI.  The reference's independent
+    (diagonal) Newton steps do not converge on every ROI, so its result is sensitive to the
+    rounding of its erf/exp (its FP32 and FP64 restatements differ by up to ~0.06 px here); the
+    kernel and the oracle share qmath.h's erf/exp for that reason."""
     w = synth.make_workload("gauss16", n=2048)
     ref, _, _ = _oracle(oracle, w, luts=None)
     r64, _, _ = oracle64.localize_trace(oracle64.config_from_bytes(bytes(w["cfg"])), w["mode"], None,
@@ -527,16 +541,10 @@ def test_gaussian2d_mle(gpu, oracle, oracle64):
     _report("gauss16", got, ref)
     assert np.array_equal(got["frame"], ref["frame"])
     assert np.array_equal(got["comx"], ref["comx"]) and np.array_equal(got["comy"], ref["comy"])
-    for k in "xy":
-        d = np.abs(got[k] - ref[k])
-        spread = np.abs(ref[k] - r64[k]).max()
-        print(f"[gauss16] {k}: p99 |GPU-oracle| {np.percentile(d, 99):.2e}, max {d.max():.2e}; "
-              f"oracle FP32-FP64 spread {spread:.2e}")
-        assert np.percentile(d, 99) <= XY_TOL
-        assert d.max() <= spread + XY_TOL
+    _gauss_compare("gauss16", got, ref, r64)
     e_gpu = np.hypot(got["x"] - w["truth"][:, 0], got["y"] - w["truth"][:, 1])
     e_ref = np.hypot(ref["x"] - w["truth"][:, 0], ref["y"] - w["truth"][:, 1])
-    assert abs(np.sqrt(np.mean(e_gpu ** 2)) - np.sqrt(np.mean(e_ref ** 2))) < 1e-3
+    assert abs(np.sqrt(np.nanmean(e_gpu ** 2)) - np.sqrt(np.nanmean(e_ref ** 2))) < 1e-4
 
 
 def _gauss_sizes_run(oracle, W, H, n=512):
@@ -563,10 +571,8 @@ def _gauss_sizes_run(oracle, W, H, n=512):
 
 @pytest.mark.parametrize("size", [(20, 12), (12, 20)], ids=["20x12", "12x20"])
 def test_gaussian2d_mle_sizes(gpu, oracle, oracle64, size):
-    """LT_Gaussian2D on non-square ROIs where the reference's diagonal Newton is well
-    conditioned (its FP32 and FP64 restatements agree to 2.5e-4 px at the 99th percentile):
-    the one-wavefront kernel and the workgroup kernel, each against the oracle with the 16x16
-    case's bars."""
+    """LT_Gaussian2D on non-square ROIs: the one-wavefront kernel and the workgroup kernel, each
+    against the oracle at the 16x16 case's bar (every finite x/y within 1e-3 px)."""
     W, H = size
     w, ref, wave, wg = _gauss_sizes_run(oracle, W, H)
     r64, _, _ = oracle64.localize_trace(oracle64.config_from_bytes(bytes(w["cfg"])), w["mode"], None, w["rois"],
@@ -575,26 +581,23 @@ def test_gaussian2d_mle_sizes(gpu, oracle, oracle64, size):
         _report(f"gauss {W}x{H} {tag}", got, ref)
         for k in ("comx", "comy", "imageMean"):
             assert np.array_equal(got[k], ref[k]), k
-        for k in "xy":
-            d = np.abs(got[k] - ref[k])
-            spread = np.abs(ref[k] - r64[k]).max()
-            print(f"[gauss {W}x{H} {tag}] {k}: p99 |GPU-oracle| {np.percentile(d, 99):.2e}, max {d.max():.2e}; "
-                  f"oracle FP32-FP64 spread {spread:.2e}", flush=True)
-            assert np.percentile(d, 99) <= XY_TOL
-            assert d.max() <= spread + XY_TOL
+        _gauss_compare(f"gauss {W}x{H} {tag}", got, ref, r64)
     assert np.all(wave["z"] == 0) and np.all(wave["error"] == 0)
 
 
 @pytest.mark.parametrize("size", [(24, 24), (32, 32)], ids=["24x24", "32x32"])
-def test_gaussian2d_wavefront_matches_workgroup(gpu, oracle, size):
+def test_gaussian2d_wavefront_matches_workgroup(gpu, oracle, oracle64, size):
     """Larger ROIs up to the one-wavefront kernel's 1,024 pixels: its x/y against the workgroup
-    kernel's (the same NaN set, within 1e-4 px), COM and image mean bit-exact against the
-    oracle.  No x/y bar against the oracle here: on these sizes the reference's undamped
-    diagonal Newton has not converged after 10 iterations from its start (I0 = mean W H / 2),
-    so its own FP32 and FP64 restatements differ by up to 0.07 px (32x32) and a few percent
-    of ROIs diverge to NaN."""
+    kernel's (the same NaN set, within 1e-4 px), COM and image mean bit-exact, and both against
+    the oracle at the 1e-3 px bar.  On these sizes the reference's undamped diagonal Newton has
+    not converged after 10 iterations from its start (I0 = mean W H / 2): its own FP32 and FP64
+    restatements differ by up to 0.07 px (32x32) and a few percent of ROIs diverge to NaN, so the
+    agreement rests on the shared erf/exp and the same operation order."""
     W, H = size
     w, ref, wave, wg = _gauss_sizes_run(oracle, W, H)
+    r64, _, _ = oracle64.localize_trace(oracle64.config_from_bytes(bytes(w["cfg"])), w["mode"], None, w["rois"],
+                                        w["jobs"])
+    _gauss_compare(f"gauss {W}x{H} wavefront", wave, ref, r64)
     _report(f"gauss {W}x{H} wavefront", wave, ref)
     for k in ("comx", "comy", "imageMean"):
         assert np.array_equal(wave[k], ref[k]) and np.array_equal(wave[k], wg[k]), k
