@@ -348,10 +348,6 @@ def mle_bench(a):
     print(json.dumps(out), flush=True)
 
 
-# the batching workload: ROI size, ROIs per batch, beads, LUT planes, radial steps per tracker
-BATCHING = [(32, 2048, 64, 100, 32), (64, 4096, 256, 150, 64), (96, 1024, 64, 200, 96), (128, 512, 32, 300, 96)]
-
-
 def batching_bench(a):
     """4 trackers (ROI 32/64/96/128 px; 2,048/4,096/1,024/512 uint16 ROIs per batch; own LUTs of
     100-300 planes x 32-96 steps) served by one hipGraph launch per batch: COM -> QI (5) -> BF16
@@ -359,6 +355,7 @@ def batching_bench(a):
     with p50/p99 latency."""
     from qtrk_amd import api, synth
 
+    BATCHING = synth.BATCHING
     rng = np.random.default_rng(0x51AB0004)
     p = api.Pipeline()
     trks, works = [], []

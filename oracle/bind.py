@@ -96,6 +96,10 @@ class Oracle:
                                              fl, P]),
             "oq_gaussian_spot": (None, [P, C.c_int, C.c_int, fl, fl, fl, fl, fl]),
             "oq_gauss2d_mle": (None, [P, C.c_int, C.c_int, fl, fl, fl, C.c_int, fl, P]),
+            "oq_quadrant_profile_img": (None, [P, C.c_int, C.c_int, fl, C.c_int, C.c_int, C.c_int, C.c_int, fl, fl,
+                                               fl, fl, P]),
+            "oq_com_qi_img": (None, [P, P, fl, C.c_int, P]),
+            "oq_schedule_frame": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_int, C.c_int, P, P, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(L, k)
@@ -289,6 +293,37 @@ class Oracle:
                                  None if zc is None else _p(zc), _p(rois), pdt, _p(jobs), len(jobs), _p(out),
                                  _p(best), _p(prof))
         return out, best, prof
+
+    def quadrant_profile(self, img, mean, A, angular_steps, q, nr, minr, maxr, cx, cy):
+        """ComputeQuadrantProfile of quadrant q (0..3) with ComputeQI's table of A directions."""
+        img = np.ascontiguousarray(img, self.fl)
+        out = np.zeros(nr, self.fl)
+        self.L.oq_quadrant_profile_img(_p(img), img.shape[1], img.shape[0], mean, A, angular_steps, q, nr, minr, maxr,
+                                       cx, cy, _p(out))
+        return out
+
+    def com_qi(self, cfg, img, bgcorr=0.0, window=True):
+        """ComputeMeanAndCOM(bgcorr) then ComputeQI with cfg's settings, with or without the radial bin
+        window: (x, y, boundary hit)."""
+        img = np.ascontiguousarray(img, self.fl)
+        out = np.zeros(3, self.fl)
+        self.L.oq_com_qi_img(C.byref(cfg), _p(img), bgcorr, int(window), _p(out))
+        return float(out[0]), float(out[1]), bool(out[2])
+
+    def schedule_frame(self, cfg, frame, positions, job):
+        """QueuedTracker::ScheduleFrame: (ROIs [n][H][W] in the frame's pixel type, jobs) of the
+        ROIs it queues, in order.  frame: 2-D array, any row pitch (its strides[0])."""
+        frame = np.asarray(frame)
+        assert frame.strides[1] == frame.itemsize
+        pdt = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[frame.dtype]
+        pos = np.ascontiguousarray(positions, np.int32).reshape(-1, 2)
+        job = np.ascontiguousarray(np.asarray(job, JOB).reshape(1))
+        rois = np.zeros((len(pos), cfg.height, cfg.width), frame.dtype)
+        jobs = np.zeros(len(pos), JOB)
+        base = frame.__array_interface__["data"][0]
+        n = self.L.oq_schedule_frame(C.byref(cfg), C.c_void_p(base), frame.strides[0], frame.shape[1],
+                                     frame.shape[0], _p(pos), len(pos), pdt, _p(job), _p(rois), _p(jobs))
+        return rois[:n], jobs[:n]
 
     def power_spectrum_2d(self, img):
         """FFT2D::Apply: |FFT2(img)|^2 with the zero frequency at (w/2, h/2)."""

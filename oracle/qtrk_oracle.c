@@ -1764,3 +1764,77 @@ long oq_qm_check(int which, float amax, int stride, double* max_exact, double* m
 	*worst = wx;
 	return n;
 }
+
+/* ------------------------------------------------------- frame extraction */
+
+/* QueuedTracker::ScheduleFrame (cputrack/QueuedTracker.cpp:91-111) with the copy that
+ * QueuedCPUTracker::ScheduleLocalization makes of each ROI (cputrack/QueuedCPUTracker.cpp:
+ * 435-456): ROIs with x < 0, y < 0, x + W > width or y + H > height are skipped; a kept ROI
+ * starts at byte pitch*y + x*bpp of the frame, its W*bpp bytes per row are copied into
+ * rois_out (W x H pixels each, in the order kept), and its job is *job with zlutIndex =
+ * i + job->zlutIndex.  positions holds (x, y) int pairs.  Returns the number kept. */
+int oq_schedule_frame(const OConfig* cfg, const void* img, int pitch, int width, int height, const int* positions,
+                      int numROI, int pdt, const OJob* job, void* rois_out, OJob* jobs_out)
+{
+	const int bpp = pdt == 0 ? 1 : pdt == 1 ? 2 : 4;
+	const int W = cfg->width, H = cfg->height;
+	const size_t row = (size_t)W * bpp;
+	const unsigned char* src = (const unsigned char*)img;
+	unsigned char* dst = (unsigned char*)rois_out;
+	int count = 0;
+	for (int i = 0; i < numROI; i++) {
+		const int x = positions[2 * i], y = positions[2 * i + 1];
+		if (x < 0 || y < 0 || x + W > width || y + H > height) continue;
+		const unsigned char* roi = src + (size_t)pitch * y + (size_t)x * bpp;
+		for (int r = 0; r < H; r++) memcpy(dst + ((size_t)count * H + r) * row, roi + (size_t)pitch * r, row);
+		jobs_out[count] = *job;
+		jobs_out[count].zlutIndex = i + job->zlutIndex;
+		count++;
+	}
+	return count;
+}
+
+/* ------------------------------------------ pieces exposed for the pins */
+
+static void img_worker(OWorker* wk, const fl* img, int w, int h)
+{
+	memset(wk, 0, sizeof(*wk));
+	wk->w = w;
+	wk->h = h;
+	wk->img = (fl*)malloc(sizeof(fl) * (size_t)w * h);
+	memcpy(wk->img, img, sizeof(fl) * (size_t)w * h);
+}
+
+/* ComputeQuadrantProfile (cputrack/cpu_tracker.cpp:631-677) of quadrant q on an image, with
+ * ComputeQI's direction table of A entries and `mean` as the fallback. */
+void oq_quadrant_profile_img(const fl* img, int w, int h, fl mean, int A, int angularSteps, int q, int nr, fl minr,
+                             fl maxr, fl cx, fl cy, fl* dst)
+{
+	OWorker wk;
+	img_worker(&wk, img, w, h);
+	wk.mean = mean;
+	ensure_qdirs(&wk, A);
+	oq_quadrant_profile(&wk, dst, nr, angularSteps, q, minr, maxr, cx, cy, 0);
+	oq_worker_free(&wk);
+}
+
+/* ComputeMeanAndCOM(bgcorr) then ComputeQI with cfg's QI settings, weighted by the radial bin
+ * window when `window` (as ProcessJob, cputrack/QueuedCPUTracker.cpp:306) or not (as BuildLUT,
+ * :601-603).  out3 = x, y, boundary hit. */
+void oq_com_qi_img(const OConfig* cfg, const fl* img, fl bgcorr, int window, fl* out3)
+{
+	OWorker wk;
+	img_worker(&wk, img, cfg->width, cfg->height);
+	fl* wnd = (fl*)malloc(sizeof(fl) * (size_t)(cfg->qi_radialsteps > 0 ? cfg->qi_radialsteps : 1));
+	oq_radial_bin_window(cfg->qi_radialsteps, wnd);
+	fl cx, cy, x, y;
+	int bh;
+	oq_com(&wk, bgcorr, &cx, &cy);
+	oq_qi(&wk, cx, cy, cfg->qi_iterations, cfg->qi_radialsteps, cfg->qi_angstepspq, cfg->qi_angstep_factor,
+	      (fl)cfg->qi_minradius, (fl)cfg->qi_maxradius, window ? wnd : 0, &x, &y, &bh, 0);
+	out3[0] = x;
+	out3[1] = y;
+	out3[2] = (fl)bh;
+	free(wnd);
+	oq_worker_free(&wk);
+}

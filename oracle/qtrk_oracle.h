@@ -193,6 +193,15 @@ void oq_fisher_ref(fl* F, int w, int h, const float* zlut, int planes, int steps
                    fl pz, fl dx, fl dy, fl dz, fl maxValue);
 void oq_gauss2d_mle(const fl* img, int w, int h, fl mean, fl x0, fl y0, int iters, fl sigma,
                     fl* out4);
+/* QueuedTracker::ScheduleFrame (cputrack/QueuedTracker.cpp:91-111): the ROIs it queues, copied
+ * as QueuedCPUTracker::ScheduleLocalization copies them, and their jobs; returns the count */
+/* Pieces of the path on a float image, for the oracle's own pins: the quadrant profile with
+ * ComputeQI's direction table of A entries, and COM then QI with or without the bin window. */
+void oq_quadrant_profile_img(const fl* img, int w, int h, fl mean, int A, int angularSteps, int q, int nr, fl minr,
+                             fl maxr, fl cx, fl cy, fl* dst);
+void oq_com_qi_img(const OConfig* cfg, const fl* img, fl bgcorr, int window, fl* out3);
+int oq_schedule_frame(const OConfig* cfg, const void* img, int pitch, int width, int height, const int* positions,
+                      int numROI, int pdt, const OJob* job, void* rois_out, OJob* jobs_out);
 
 #ifdef __cplusplus
 }

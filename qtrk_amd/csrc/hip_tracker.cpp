@@ -162,6 +162,7 @@ QueuedHIPTracker::~QueuedHIPTracker()
 		std::unique_lock<std::mutex> lk(mtx);
 		for (Slot* s : slots)
 			if (s->state == Slot::Launched) hipEventSynchronize(s->ev_done);
+		slots_cv.wait(lk, [&] { return waiters == 0; });
 	}
 	FreeSlots();
 }
@@ -247,7 +248,8 @@ void QueuedHIPTracker::AllocSlots()
 		HIP_CHECK(hipEventCreate(&s->ev_start));
 		HIP_CHECK(hipEventCreate(&s->ev_k0));
 		HIP_CHECK(hipEventCreate(&s->ev_k1));
-		HIP_CHECK(hipEventCreate(&s->ev_done));
+		// blocking synchronisation: a long wait on a full queue sleeps instead of spinning
+		HIP_CHECK(hipEventCreateWithFlags(&s->ev_done, hipEventBlockingSync));
 		size_t rb = (size_t)cfg.width * cfg.height * 2;
 		s->h_rois.ensure(rb * batchSize);
 		s->h_jobs.ensure(sizeof(DevJob) * batchSize);
@@ -334,32 +336,76 @@ void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void*
 
 // ------------------------------------------------------------- queueing
 
-QueuedHIPTracker::Slot* QueuedHIPTracker::AcquireSlot(std::unique_lock<std::mutex>& lk)
+// A free slot, marked Filling, after harvesting finished batches; nullptr when every slot is busy.
+QueuedHIPTracker::Slot* QueuedHIPTracker::TakeFreeSlot()
 {
-	while (true) {
-		Harvest();
-		for (Slot* s : slots)
-			if (s->state == Slot::Free) {
-				s->state = Slot::Filling;
-				s->n = 0;
-				s->frame = false;
-				return s;
-			}
-		// Queue full: block like the reference (cputrack/QueuedCPUTracker.cpp:437-440)
-		// until the oldest launched batch completes.
-		Slot* oldest = nullptr;
-		for (Slot* s : slots)
-			if (s->state == Slot::Launched && (!oldest || s->seq < oldest->seq)) oldest = s;
-		if (!oldest) fail("no batch slot available");
-		// polled rather than hipEventSynchronize: the caller is blocked either way, and a poll
-		// resumes filling as soon as the batch is done instead of after a wake-up
-		hipEvent_t ev = oldest->ev_done;
-		lk.unlock();
-		hipError_t e;
-		while ((e = hipEventQuery(ev)) == hipErrorNotReady) std::this_thread::yield();
-		lk.lock();
-		HIP_CHECK(e);
+	Harvest();
+	for (Slot* s : slots)
+		if (s->state == Slot::Free) {
+			s->state = Slot::Filling;
+			s->n = 0;
+			s->frame = false;
+			return s;
+		}
+	return nullptr;
+}
+
+// Waits for `ev` with the lock released.  The event is polled for up to kSpinWait (a batch
+// that is about to finish resumes the caller at once) and then waited on with a blocking
+// synchronisation, so a long wait does not keep a core busy.  `waiters` keeps the slots (and
+// their events) alive meanwhile: code that frees them waits until it is back to zero.  Other
+// threads may change `filling` and the slot states while this waits.
+void QueuedHIPTracker::WaitEventLocked(std::unique_lock<std::mutex>& lk, hipEvent_t ev)
+{
+	waiters++;
+	lk.unlock();
+	const auto t0 = std::chrono::steady_clock::now();
+	hipError_t e;
+	while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
+		if (std::chrono::steady_clock::now() - t0 > kSpinWait) {
+			e = hipEventSynchronize(ev);
+			break;
+		}
+		std::this_thread::yield();
 	}
+	lk.lock();
+	if (--waiters == 0) slots_cv.notify_all();
+	HIP_CHECK(e);
+}
+
+// Queue full: block like the reference (cputrack/QueuedCPUTracker.cpp:437-440) until the
+// oldest launched batch completes.  Callers re-examine `filling` afterwards.
+void QueuedHIPTracker::WaitOldestLocked(std::unique_lock<std::mutex>& lk)
+{
+	Slot* oldest = nullptr;
+	for (Slot* s : slots)
+		if (s->state == Slot::Launched && (!oldest || s->seq < oldest->seq)) oldest = s;
+	if (!oldest) fail("no batch slot available");
+	WaitEventLocked(lk, oldest->ev_done);
+}
+
+// Flushes, waits for every batch, and waits until no thread is blocked on a slot's event, so the
+// slots can be freed or remade.  Returns with the lock held, no slot in use and `quiescing` set:
+// scheduling threads wait until the caller clears it (EndQuiesceLocked).
+void QueuedHIPTracker::QuiesceLocked(std::unique_lock<std::mutex>& lk)
+{
+	quiescing = true;
+	while (true) {
+		if (filling) {
+			if (filling->n > 0) Launch(filling);
+			else filling->state = Slot::Free;
+			filling = nullptr;
+		}
+		WaitAllLocked(lk);
+		if (waiters == 0 && !filling) return;
+		slots_cv.wait(lk, [&] { return waiters == 0; });
+	}
+}
+
+void QueuedHIPTracker::EndQuiesceLocked()
+{
+	quiescing = false;
+	slots_cv.notify_all();
 }
 
 void QueuedHIPTracker::ScheduleLocalization(void* data, int pitch, QTRK_PixelDataType pdt, const LocalizationJob* jobInfo)
@@ -367,12 +413,22 @@ void QueuedHIPTracker::ScheduleLocalization(void* data, int pitch, QTRK_PixelDat
 	if ((int)pdt < 0 || (int)pdt > 2) fail("invalid pixel data type");
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
-	if (filling && filling->n > 0 && (filling->pdt != (int)pdt || filling->frame)) {
-		Launch(filling);
-		filling = nullptr;
+	// several threads may schedule: a thread that waited for a slot finds `filling` as the
+	// others left it
+	while (true) {
+		if (quiescing) {  // the slots are being remade (SetConfigValue)
+			slots_cv.wait(lk, [&] { return !quiescing; });
+			continue;
+		}
+		if (filling && filling->n > 0 && (filling->pdt != (int)pdt || filling->frame)) {
+			Launch(filling);
+			filling = nullptr;
+		}
+		if (filling || (filling = TakeFreeSlot())) break;
+		WaitOldestLocked(lk);
 	}
-	if (!filling) filling = AcquireSlot(lk);
 	Slot* s = filling;
+	s->frame = false;
 	s->pdt = (int)pdt;
 	const size_t row = (size_t)cfg.width * PDT_BytesPerPixel(pdt);
 	const size_t rb = row * cfg.height;
@@ -444,16 +500,28 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 	const uint8_t* img = (const uint8_t*)imgptr;
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
-	if (filling && filling->n > 0 &&
-	    !(filling->frame && filling->pdt == (int)pdt && filling->frame_pitch == pitch)) {
-		Launch(filling);  // ROIs queued before the frame go first
-		filling = nullptr;
-	}
 	if (!copier) copier.reset(new CopyPool(frameCopyThreads - 1, true));
 	size_t c0 = 0;
 	while (c0 < sel.size()) {
+		if (quiescing) {  // the slots are being remade (SetConfigValue)
+			slots_cv.wait(lk, [&] { return !quiescing; });
+			continue;
+		}
+		// ROIs queued before the frame (or another thread's) go first, in their own launch
+		if (filling && filling->n > 0 &&
+		    !(filling->frame && filling->pdt == (int)pdt && filling->frame_pitch == pitch)) {
+			Launch(filling);
+			filling = nullptr;
+		}
 		if (!filling) {
-			filling = AcquireSlot(lk);
+			Slot* f = TakeFreeSlot();
+			if (!f) {
+				WaitOldestLocked(lk);  // `filling` may be another thread's frame slot afterwards
+				continue;
+			}
+			filling = f;
+		}
+		if (filling->n == 0) {
 			filling->frame = true;
 			filling->pdt = (int)pdt;
 			filling->frame_pitch = pitch;
@@ -653,10 +721,7 @@ void QueuedHIPTracker::WaitAllLocked(std::unique_lock<std::mutex>& lk)
 		for (Slot* s : slots)
 			if (s->state == Slot::Launched) { ev = s->ev_done; break; }
 		if (!ev) return;
-		lk.unlock();
-		hipError_t e = hipEventSynchronize(ev);
-		lk.lock();
-		HIP_CHECK(e);
+		WaitEventLocked(lk, ev);
 	}
 }
 
@@ -976,9 +1041,13 @@ void QueuedHIPTracker::SetConfigValue(std::string name, std::string value)
 	if (name == "batch_size" || name == "num_streams") {
 		int v = atoi(value.c_str());
 		if (v < 1) fail(name + " must be >= 1");
-		Flush();
 		std::unique_lock<std::mutex> lk(mtx);
-		WaitAllLocked(lk);
+		HIP_CHECK(hipSetDevice(dev));
+		struct End {
+			QueuedHIPTracker* t;
+			~End() { t->EndQuiesceLocked(); }
+		} end{ this };
+		QuiesceLocked(lk);
 		FreeSlots();
 		if (name == "batch_size") batchSize = v;
 		else numSlots = v;

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <chrono>
+#include <condition_variable>
 #include <deque>
 #include <memory>
 #include <mutex>
@@ -109,7 +111,11 @@ private:
 	void BuildTables();
 	void AllocSlots();
 	void FreeSlots();
-	Slot* AcquireSlot(std::unique_lock<std::mutex>& lk);
+	Slot* TakeFreeSlot();
+	void WaitEventLocked(std::unique_lock<std::mutex>& lk, hipEvent_t ev);
+	void WaitOldestLocked(std::unique_lock<std::mutex>& lk);
+	void QuiesceLocked(std::unique_lock<std::mutex>& lk);
+	void EndQuiesceLocked();
 	void Launch(Slot* s);
 	void Harvest();
 	void WaitAllLocked(std::unique_lock<std::mutex>& lk);
@@ -159,6 +165,12 @@ private:
 	std::unique_ptr<CopyPool> copier;  // made on the first frame
 
 	mutable std::mutex mtx;
+	// threads blocked on a slot's event with mtx released (WaitEventLocked); while `quiescing`
+	// (the slots are being remade) scheduling threads wait; both are signalled on slots_cv
+	int waiters = 0;
+	bool quiescing = false;
+	std::condition_variable slots_cv;
+	static constexpr std::chrono::microseconds kSpinWait{ 200 };
 	std::mutex res_mtx;
 	std::deque<LocalizationResult> results;
 

@@ -78,6 +78,10 @@ def scaled_model_lut(planes, steps, size):
     return (np.exp(-r[None, :] ** 2 / (2 * w * w)) * (1 + np.cos(2 * np.pi * r[None, :] / T)) / 2).astype(np.float32)
 
 
+# The batching workload's trackers: ROI size, ROIs per batch, beads, LUT planes, radial steps
+BATCHING = [(32, 2048, 64, 100, 32), (64, 4096, 256, 150, 64), (96, 1024, 64, 200, 96), (128, 512, 32, 300, 96)]
+
+
 def batching_tracker(size, n, beads, planes, steps, rng, render=True):
     """One tracker of the batching workload: size x size uint16 ROIs (n = beads x frames, frame-major),
     per-tracker image LUT of `planes` x `steps` (scaled_model_lut), bead centre +-size/16 px, z in the

@@ -94,6 +94,39 @@ def test_workload_parity(gpu, oracle, oracle64, name, n):
     assert e["x"] < XY_TOL and e["y"] < XY_TOL and e["z"] < Z_TOL
 
 
+def test_workload_parity_angstep_factor(gpu, oracle):
+    """realtime100 at qi_angstep_factor = 1.3 (the reference's own benchmark setting,
+    cputrack-test/Benchmark.cpp:25): per iteration int(A / 1.3^(iters - k)) angular samples
+    (clamped to >= 4) at table stride (int)(A / samples) (cputrack/cpu_tracker.cpp:314, 356,
+    642-657); bit-exact indexing and best plane, x/y/z within the bars."""
+    w = synth.make_workload("realtime100", n=512, qi_angstep_factor=1.3)
+    ref, rbest, rprof = _oracle(oracle, w)
+    trk = _tracker(w)
+    got, best, prof = _batch(trk, w["rois"], w["jobs"])
+    trk.free()
+    _report("realtime100 factor 1.3", got, ref)
+    _assert_parity(got, ref, best, rbest)
+    np.testing.assert_allclose(prof, rprof, atol=1e-5)
+
+
+@pytest.mark.parametrize("spec", synth.BATCHING, ids=[f"{s[0]}px" for s in synth.BATCHING])
+def test_batching_tracker_sizes_parity(gpu, oracle, spec):
+    """The batching workload's four trackers (ROI 32/64/96/128 px, LUTs of 100/150/200/300 planes x
+    32/64/96/96 steps, uint16 Poisson ROIs) through the tracker's per-ROI kernel, 256 ROIs each,
+    against the oracle's ProcessJob (cputrack/QueuedCPUTracker.cpp:273-361)."""
+    size, _, beads, planes, steps = spec
+    w = synth.batching_tracker(size, 256, min(beads, 64), planes, steps, np.random.default_rng(0x51AB0004 + size))
+    wl = dict(w, mode=synth.MODE_QI_Z, luts=w["lut"])
+    ref, rbest, _ = _oracle(oracle, wl)
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_radial_zlut(w["lut"])
+    trk.set_localization_mode(synth.MODE_QI_Z)
+    got, best, _ = _batch(trk, w["rois"], w["jobs"])
+    trk.free()
+    _report(f"batching {size}px", got, ref)
+    _assert_parity(got, ref, best, rbest)
+
+
 def test_queue_api_matches_batch_and_is_lifo(gpu, oracle):
     w = synth.make_workload("plain64", n=300)
     trk = _tracker(w)
@@ -128,6 +161,71 @@ def test_small_batches_many_streams_block_and_complete(gpu, oracle):
     ref, _, _ = _oracle(oracle, w)
     _assert_parity(r, ref)
     trk.free()
+
+
+def test_concurrent_schedulers_share_a_small_queue(gpu, oracle):
+    """Threads scheduling into one tracker at once (the reference guards its queue with a mutex,
+    cputrack/QueuedCPUTracker.cpp:435-456, cudatrack/QueuedCUDATracker.cu:431-446): batches of 8
+    on 2 streams, so the threads block on the full queue and find each other's filling slot when
+    they wake; two threads schedule single ROIs and one whole frames, while the main thread
+    changes num_streams and batch_size (the slots are remade with threads waiting).  Every ROI's
+    result arrives exactly once, the tracker becomes idle, and every result matches the oracle."""
+    import threading
+    import time
+
+    w = synth.make_workload("plain64", n=400)
+    trk = _tracker(w)
+    trk.set_config_value("batch_size", "8")
+    trk.set_config_value("num_streams", "2")
+    errors = []
+    frames = []
+    for k in range(25):  # frames of 4 ROIs: ROIs 300 + 4k .. 303 + 4k
+        f = np.zeros((70, 280), np.float32)
+        for i in range(4):
+            f[3:67, 4 + 68 * i:68 + 68 * i] = w["rois"][300 + 4 * k + i]
+        frames.append((f, [(4 + 68 * i, 3) for i in range(4)],
+                       api.make_jobs(1, frame=np.array([1000 + k]), zlut_index=np.array([0]))[0]))
+
+    def rois(lo, hi):
+        try:
+            for i in range(lo, hi):
+                trk.schedule_localization(w["rois"][i], w["jobs"][i])
+        except Exception as e:  # noqa: BLE001 - reported by the main thread
+            errors.append(e)
+
+    def frame_thread():
+        try:
+            for f, pos, job in frames:
+                assert trk.schedule_frame(f, pos, job) == 4
+        except Exception as e:  # noqa: BLE001
+            errors.append(e)
+
+    ths = [threading.Thread(target=rois, args=(0, 150)), threading.Thread(target=rois, args=(150, 300)),
+           threading.Thread(target=frame_thread)]
+    for t in ths:
+        t.start()
+    time.sleep(0.002)
+    trk.set_config_value("num_streams", "3")
+    trk.set_config_value("batch_size", "5")
+    for t in ths:
+        t.join(60)
+        assert not t.is_alive()
+    assert not errors, errors
+    trk.wait_for(400, timeout=60)
+    t0 = time.time()
+    while not trk.is_idle():
+        assert time.time() - t0 < 10, "tracker never became idle"
+        time.sleep(0.001)
+    r = trk.fetch_results(1000)
+    trk.free()
+    assert len(r) == 400
+    got = _by_index(r)
+    keys = set(zip(got["frame"].tolist(), got["zlutIndex"].tolist()))
+    assert len(keys) == 400
+    calls = [(w["rois"][i], w["jobs"][i]) for i in range(300)] + frames
+    ref = _oracle_frames(oracle, w, w["luts"], calls)
+    _report("concurrent", got, ref)
+    _assert_parity(got, ref)
 
 
 def test_pixel_formats_and_pitch(gpu, oracle):
@@ -242,8 +340,30 @@ def test_calibration_images(gpu, oracle):
     _assert_parity(got, ref, best, rbest)
 
 
-def test_schedule_frame_extraction(gpu):
-    w = synth.make_workload("plain64", n=12)
+def _by_index(r):
+    return r[np.lexsort((r["zlutIndex"], r["frame"]))]
+
+
+def _oracle_frames(oracle, w, luts, calls, mode=None):
+    """The oracle's QueuedTracker::ScheduleFrame on each call (frame, positions, job) or per-ROI call
+    (roi array, job), then ProcessJob on the ROIs it queued; results sorted by (frame, zlutIndex)."""
+    cfg = oracle.config_from_bytes(bytes(w["cfg"]))
+    out = []
+    for c in calls:
+        if len(c) == 2:
+            rois, jobs = np.asarray(c[0])[None], np.asarray(c[1], api.JOB_DTYPE).reshape(1)
+        else:
+            rois, jobs = oracle.schedule_frame(cfg, c[0], c[1], c[2])
+        if len(jobs):
+            out.append(oracle.localize_trace(cfg, w["mode"] if mode is None else mode, luts, rois, jobs)[0])
+    return _by_index(np.concatenate(out))
+
+
+def test_schedule_frame_extraction(gpu, oracle):
+    """QTrkScheduleFrame against the oracle's ScheduleFrame (cputrack/QueuedTracker.cpp:91-111):
+    ROIs cut from the frame at their positions, those outside skipped (including one a pixel
+    past the right and bottom edges; ROIs flush with them are kept), zlutIndex = i + base."""
+    w = synth.make_workload("plain64", n=14)
     H, W = 300, 410
     frame = np.zeros((H, W), np.float32)
     pos = []
@@ -251,37 +371,30 @@ def test_schedule_frame_extraction(gpu):
         x, y = 10 + (i % 4) * 95, 20 + (i // 4) * 90
         frame[y:y + 64, x:x + 64] = w["rois"][i]
         pos.append((x, y))
-    pos += [(-1, 5), (W - 63, 0), (0, H - 63)]     # outside the frame: skipped
+    frame[H - 64:, W - 64:] = w["rois"][12]
+    frame[0:64, 0:64] = w["rois"][13]
+    pos += [(-1, 5), (W - 63, 0), (0, H - 63), (W - 64, H - 64), (0, 0)]  # 3 skipped, 2 flush kept
     trk = _tracker(w)
     job = np.zeros(1, api.JOB_DTYPE)[0]
     job["frame"] = 77
     job["zlutIndex"] = 0
     n = trk.schedule_frame(frame, pos, job)
-    assert n == 12
-    trk.wait_for(12)
-    r = trk.fetch_results(12)
-    r = r[np.argsort(r["zlutIndex"])]
-    assert list(r["zlutIndex"]) == list(range(12)) and np.all(r["frame"] == 77)
-    jobs = w["jobs"].copy()
-    jobs["frame"] = 77
-    jobs["zlutIndex"] = np.arange(12)
-    luts = np.repeat(w["luts"], 12, axis=0)
-    trk2 = api.QTrkInstance(w["settings"])
-    trk2.set_radial_zlut(luts)
-    trk2.set_localization_mode(w["mode"])
-    ref, _, _ = _batch(trk2, w["rois"], jobs, diag=False)
-    trk2.free()
+    assert n == 14
+    trk.wait_for(14)
+    r = _by_index(trk.fetch_results(14))
     trk.free()
-    # the single LUT applies to bead 0 only; compare the xy part for all, z for bead 0
-    assert np.array_equal(r["x"], ref["x"]) and np.array_equal(r["y"], ref["y"])
-    assert r["z"][0] == ref["z"][0]
+    assert list(r["zlutIndex"]) == list(range(12)) + [15, 16] and np.all(r["frame"] == 77)
+    ref = _oracle_frames(oracle, w, w["luts"], [(frame, pos, job)])
+    _report("frame extraction", r, ref)
+    _assert_parity(r, ref)
 
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32], ids=["u8", "u16", "f32"])
-def test_schedule_frame_chunks_pitch_and_formats(gpu, dtype):
+def test_schedule_frame_chunks_pitch_and_formats(gpu, oracle, dtype):
     """The frame is uploaded once per batch_size ROIs and the kernel reads each ROI at its offset in
-    it: every result equals the per-ROI path, with a padded frame pitch, ROIs outside the frame
-    skipped, zlutIndex = i + the job's zlutIndex, and more ROIs than one batch."""
+    it: every result matches the oracle's ScheduleFrame + ProcessJob, with a padded frame pitch,
+    ROIs outside the frame skipped, zlutIndex = i + the job's zlutIndex, and more ROIs than one
+    batch."""
     n, base = 40, 5
     w = synth.make_workload("realtime100", n=n)
     rois = w["rois"]
@@ -314,22 +427,17 @@ def test_schedule_frame_chunks_pitch_and_formats(gpu, dtype):
     r = trk.fetch_results(n)
     r = r[np.argsort(r["zlutIndex"])]
     assert list(r["zlutIndex"]) == [base + i for i in keep]
-    jobs = api.make_jobs(n, frame=np.full(n, 9), zlut_index=np.array([base + i for i in keep]))
-    trk2 = api.QTrkInstance(w["settings"])
-    trk2.set_radial_zlut(luts)
-    trk2.set_localization_mode(w["mode"])
-    ref, _, _ = _batch(trk2, rois, jobs, diag=False)
-    trk2.free()
     trk.free()
-    for k in ("frame", "zlutIndex", "x", "y", "z", "comx", "comy", "error", "imageMean"):
-        assert np.array_equal(r[k], ref[k]), k
+    ref = _oracle_frames(oracle, w, luts, [(frame, pos, job)])
+    _report(f"frame chunks {np.dtype(dtype).name}", r, ref)
+    _assert_parity(r, ref)
 
 
-def test_schedule_frame_packs_frames_into_batches(gpu):
+def test_schedule_frame_packs_frames_into_batches(gpu, oracle):
     """Frames of one pixel type and pitch share a batch slot, their row blocks one after another,
     until it holds batch_size ROIs (the rest goes on to the next slot) or a block would outgrow
     it; a per-ROI call, another pitch or another pixel type launches the filling slot first.
-    Every result equals the per-ROI path."""
+    Every result matches the oracle's ScheduleFrame + ProcessJob on the same calls."""
     w = synth.make_workload("realtime100", n=40)
     src = w["rois"]
     src8 = np.clip(src // 64, 0, 255).astype(np.uint8)
@@ -347,12 +455,13 @@ def test_schedule_frame_packs_frames_into_batches(gpu):
     trk.set_config_value("batch_size", "16")
     trk.set_radial_zlut(luts)
     trk.set_localization_mode(w["mode"])
-    k, meta = 0, []   # meta: (frame number, is u8) per ROI, in zlutIndex order
+    k, calls = 0, []
     for f, st in enumerate(steps):
         if st[0] == "roi":
             for _ in range(st[1]):
-                trk.schedule_localization(src[k], api.make_jobs(1, frame=np.array([f]), zlut_index=np.array([k]))[0])
-                meta.append((f, False))
+                job = api.make_jobs(1, frame=np.array([f]), zlut_index=np.array([k]))[0]
+                trk.schedule_localization(src[k], job)
+                calls.append((src[k], job))
                 k += 1
             continue
         dtype, H, W, pad, pos = st
@@ -363,31 +472,21 @@ def test_schedule_frame_packs_frames_into_batches(gpu):
             frame[y:y + 100, x:x + 100] = (src8 if dtype == np.uint8 else src)[k + i]
         job = api.make_jobs(1, frame=np.array([f]), zlut_index=np.array([k]))[0]
         assert trk.schedule_frame(frame, pos, job) == len(pos)
-        meta += [(f, dtype == np.uint8)] * len(pos)
+        calls.append((frame, pos, job))
         k += len(pos)
     trk.wait_for(k)
-    r = trk.fetch_results(k)
-    r = r[np.argsort(r["zlutIndex"])]
-    assert list(r["zlutIndex"]) == list(range(k))
-    frames = np.array([m[0] for m in meta])
-    is8 = np.array([m[1] for m in meta])
-    trk2 = api.QTrkInstance(w["settings"])
-    trk2.set_radial_zlut(luts)
-    trk2.set_localization_mode(w["mode"])
-    ref = np.zeros(k, r.dtype)
-    for sel, rois in ((~is8, src[:k][~is8]), (is8, src8[:k][is8])):
-        idx = np.nonzero(sel)[0]
-        jobs = api.make_jobs(len(idx), frame=frames[idx], zlut_index=idx)
-        ref[idx] = _batch(trk2, np.ascontiguousarray(rois), jobs, diag=False)[0]
-    trk2.free()
+    r = _by_index(trk.fetch_results(k))
     trk.free()
-    for key in ("frame", "zlutIndex", "x", "y", "z", "comx", "comy", "error", "imageMean"):
-        assert np.array_equal(r[key], ref[key]), key
+    assert sorted(r["zlutIndex"]) == list(range(k))
+    ref = _oracle_frames(oracle, w, luts, calls)
+    _report("frames packed", r, ref)
+    _assert_parity(r, ref)
 
 
-def test_frame_copy_threads_config(gpu):
+def test_frame_copy_threads_config(gpu, oracle):
     """frame_copy_threads sets how many threads copy a frame into pinned memory: results do not
-    depend on it (a 1.5 MB row block, copied by 1 and by 8 threads), and it is range-checked."""
+    depend on it (a 1.5 MB row block, copied by 1 and by 8 threads) and match the oracle's
+    ScheduleFrame + ProcessJob, and it is range-checked."""
     w = synth.make_workload("realtime100", n=32)
     frame = np.zeros((1000, 1600), np.uint16)
     pos = [(50 * (i % 8) * 4, 120 * (i // 8)) for i in range(32)]
@@ -411,6 +510,7 @@ def test_frame_copy_threads_config(gpu):
             trk.set_config_value("frame_copy_threads", bad)
     trk.free()
     assert np.array_equal(out[0], out[1])
+    _assert_parity(_by_index(out[0]), _oracle_frames(oracle, w, w["luts"][:32], [(frame, pos, job)]))
 
 
 def test_build_lut_matches_oracle(gpu, oracle):

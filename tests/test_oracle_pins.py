@@ -310,3 +310,126 @@ def test_queued_cpu_tracker_restatement(oracle):
     assert t > 0 and len(q) == len(single)
     q = q[np.argsort(q["frame"])]
     assert np.array_equal(q, single)
+
+
+# --------------------------------------------- BuildLUT composition (f2)
+
+def _plain64_cfg(oracle, **kw):
+    s = dict(width=64, height=64, qi_minradius=2, qi_roi_coverage=0.875, qi_radial_coverage=1.25,
+             qi_angular_coverage=1.46, qi_angstep_factor=1, qi_iterations=5, zlut_minradius=2,
+             zlut_roi_coverage=0.875, zlut_radial_coverage=1.25, zlut_angular_coverage=0.7)
+    s.update(kw)
+    return oracle.config(**s)
+
+
+def test_build_lut_known_position_closed_form(oracle):
+    """BuildLUT with known positions (cputrack/QueuedCPUTracker.cpp:580-645) on the quadratic bowl
+    f = A + B |p - c|^2: the angular mean about a known position k is A + B (r^2 + |k - c|^2) (the
+    cross term averages out over the uniform directions), and bilinear sampling adds t(1 - t) per
+    axis, so each bin lies in A + B (r_i^2 + d^2 + [0, 0.5]) with r_i = minr + i (maxr - minr) / n.
+    The row is added into its plane (twice -> doubled) and other planes stay zero."""
+    cfg = _plain64_cfg(oracle)
+    nb = cfg.zlut_radialsteps
+    A, B, c = 100.0, 0.5, (31.3, 32.6)
+    y, x = np.mgrid[0:64, 0:64].astype(np.float64)
+    img = (A + B * ((x - c[0]) ** 2 + (y - c[1]) ** 2)).astype(np.float32)
+    k = np.array([[28.0, 35.0]], np.float32)
+    zlut = np.zeros((1, 3, nb), np.float32)
+    oracle.build_lut(cfg, img[None], 1, zlut, known_pos=k)
+    r = cfg.zlut_minradius + np.arange(nb) * (cfg.zlut_maxradius - cfg.zlut_minradius) / nb
+    d2 = float(((k[0] - np.array(c)) ** 2).sum())
+    lo, hi = A + B * (r * r + d2), A + B * (r * r + d2 + 0.5)
+    row = zlut[0, 1].astype(np.float64)
+    tol = 2e-6 * hi
+    assert np.all(row >= lo - tol) and np.all(row <= hi + tol)
+    assert np.all(zlut[0, 0] == 0) and np.all(zlut[0, 2] == 0)
+    first = zlut[0, 1].copy()
+    oracle.build_lut(cfg, img[None], 1, zlut, known_pos=k)
+    assert np.array_equal(zlut[0, 1], first + first)
+
+
+def test_build_lut_position_is_com_then_qi_without_window(oracle):
+    """Without known positions BuildLUT profiles at ComputeQI(ComputeMeanAndCOM()) with neither the
+    configured COM background correction nor the QI radial bin window (cputrack/
+    QueuedCPUTracker.cpp:598-603); ProcessJob applies both (:291, :306).  On an asymmetric image the
+    window moves QI's result, and the row equals the profile at the unwindowed position only."""
+    cfg = _plain64_cfg(oracle, com_bgcorrection=1.5)
+    nb, na = cfg.zlut_radialsteps, cfg.zlut_angularsteps
+    y, x = np.mgrid[0:64, 0:64].astype(np.float64)
+    img = (50 + 4000 * np.exp(-((x - 32.4) ** 2 + (y - 31.7) ** 2) / 32)
+           + 1500 * np.exp(-((x - 50.0) ** 2 + (y - 36.0) ** 2) / 6)).astype(np.float32)
+    pw = oracle.com_qi(cfg, img, 0.0, window=True)
+    pn = oracle.com_qi(cfg, img, 0.0, window=False)
+    pb = oracle.com_qi(cfg, img, 1.5, window=False)
+    assert abs(pw[0] - pn[0]) + abs(pw[1] - pn[1]) > 1e-2
+    zlut = np.zeros((1, 1, nb), np.float32)
+    oracle.build_lut(cfg, img[None], 0, zlut)
+    prof = lambda p: oracle.radial_profile(img, nb, na, cfg.zlut_minradius, cfg.zlut_maxradius, p[0], p[1],
+                                           float(img.mean()), False)
+    assert np.array_equal(zlut[0, 0], prof(pn))
+    assert not np.array_equal(zlut[0, 0], prof(pw))
+    if pb[:2] != pn[:2]:
+        assert not np.array_equal(zlut[0, 0], prof(pb))
+
+
+# ------------------------------------- sample-count thresholds (a6, a9)
+
+def _affine(w, h, a=10.0, b=0.7, c=-0.3):
+    y, x = np.mgrid[0:h, 0:w].astype(np.float64)
+    return (a + b * x + c * y).astype(np.float32), (a, b, c)
+
+
+def test_quadrant_profile_takes_four_samples(oracle):
+    """ComputeQuadrantProfile averages when at least MIN_RADPROFILE_SMP_COUNT = 4 samples are inside
+    (n >= 4, cputrack/cpu_tracker.cpp:670) and falls back to the mean below.  Quadrant 0, 4 angles
+    (pi/2 (j + 0.5)/4), radius 10 on a 40x40 affine image: centre x 28.5 keeps all 4 (average,
+    exact for an affine image), 29.5 puts the first angle's sample at x = 39.31 >= W - 1 (3 left:
+    the fallback)."""
+    img, (a, b, c) = _affine(40, 40)
+    ang = np.float32(0.5 * np.float64(np.float32(3.141593)) * (np.arange(4) + 0.5) / 4).astype(np.float64)
+    for cx, inside in ((28.5, 4), (29.5, 3)):
+        xs = cx + np.cos(ang) * 10
+        assert int((xs < 39).sum()) == inside
+        p = oracle.quadrant_profile(img, -999.0, 4, 4, 0, 1, 10.0, 20.0, cx, 20.0)
+        if inside == 4:
+            want = a + b * np.mean(xs) + c * np.mean(20.0 + np.sin(ang) * 10)
+            assert abs(p[0] - want) < 1e-3
+        else:
+            assert p[0] == np.float32(-999.0)
+
+
+def test_radial_profile_needs_more_than_four_samples(oracle):
+    """::ComputeRadialProfile averages only when more than 4 samples are inside (n > 4,
+    cputrack/utils.cpp:313): 5 directions 2 pi j/5 at radius 10 on a 40x40 affine image, centre x
+    28.5 (all 5 inside: the average) and 29.5 (the j = 0 sample at x = 39.5 is outside, 4 left: the
+    fallback)."""
+    img, (a, b, c) = _affine(40, 40)
+    ang = (2 * np.float32(3.141593) * np.arange(5) / np.float32(5)).astype(np.float64)
+    for cx, inside in ((28.5, 5), (29.5, 4)):
+        xs = cx + np.cos(ang) * 10
+        assert int((xs < 39).sum()) == inside
+        p = oracle.radial_profile(img, 1, 5, 10.0, 20.0, cx, 20.0, -999.0, False)
+        if inside == 5:
+            want = a + b * np.mean(xs) + c * np.mean(20.0 + np.sin(ang) * 10)
+            assert abs(p[0] - want) < 1e-3
+        else:
+            assert p[0] == np.float32(-999.0)
+
+
+# ------------------------------------ QI angular index at factor != 1 (a6)
+
+def test_quadrant_profile_angular_index_truncates_the_step(oracle):
+    """With fewer angular samples than table directions (qi_angstep_factor > 1), sample a uses
+    direction (int)angstepf * a, the step truncated before the multiply (cputrack/cpu_tracker.cpp:
+    655-657), not (int)(angstepf * a).  64 directions, 17 samples (64 / 1.3^5 = 17.2): step 3, so
+    directions 0..48 -- the other rule reaches 60.  Exact on an affine image."""
+    img, (a, b, c) = _affine(100, 100, b=0.9, c=0.4)
+    A, n, r, cx, cy = 64, 17, 30.0, 50.0, 50.0
+    ang = np.float32(0.5 * np.float64(np.float32(3.141593)) * (np.arange(A) + 0.5) / A).astype(np.float64)
+    step = 64 / np.float32(17)
+    val = lambda j: a + b * np.mean(cx + np.cos(ang[j]) * r) + c * np.mean(cy + np.sin(ang[j]) * r)
+    want = val(int(step) * np.arange(n))
+    other = val((step * np.arange(n)).astype(int))
+    assert abs(want - other) > 1.0
+    p = oracle.quadrant_profile(img, 0.0, A, n, 0, 1, r, r + 1, cx, cy)
+    assert abs(p[0] - want) < 1e-3

@@ -128,3 +128,65 @@ def test_pipeline_errors(gpu):
     finally:
         p.free()
         bare.free()
+
+
+@pytest.mark.gpu
+def test_pipeline_batching_workload_against_oracle(gpu, oracle):
+    """The batching workload at its stated shape: 4 trackers (ROI 32/64/96/128 px with 2,048/4,096/
+    1,024/512 uint16 ROIs, LUTs of 100/150/200/300 planes x 32/64/96/96 steps) in one autotuned graph
+    (COM -> QI(5) -> BF16 grouped match -> 5-iteration MLE).  Per tracker, 160 ROIs spread over the
+    batch against the oracle's ProcessJob (cputrack/QueuedCPUTracker.cpp:273-361): indexing and
+    boundary flags bit-exact, x/y within 1e-3 px, z within 1e-3 planes.  All 7,680 records through
+    properties: each job's record in job order, finite positions inside the ROI, finite z,
+    finite positive CRLBs for the MLE, and a second launch giving identical output."""
+    rng = np.random.default_rng(0x51AB0004)
+    p = api.Pipeline()
+    tr = []
+    try:
+        for size, n, beads, planes, steps in synth.BATCHING:
+            w = synth.batching_tracker(size, n, beads, planes, steps, rng)
+            trk = api.QTrkInstance(w["settings"])
+            trk.set_radial_zlut(w["lut"])
+            trk.set_localization_mode(synth.MODE_QI_Z)
+            tr.append((w, trk))
+            s = p.add(trk, n, api.QTrkU16, api.MATCH_BF16, w["model_lut"], 0.0, w["cfg"].zlut_maxradius, 5)
+            p.set_jobs(s, w["jobs"])
+            p.upload(s, w["rois"])
+        p.finalize(True)
+        print(p.report(), flush=True)
+        p.launch()
+        print("graph ms", p.sync(), flush=True)
+        outs = [p.download(s) for s in range(len(tr))]
+        p.launch()
+        p.sync()
+        for s, (res, th, cr) in enumerate(outs):
+            again = p.download(s)
+            assert np.array_equal(again[0], res) and np.array_equal(again[1], th) and np.array_equal(again[2], cr)
+        for (w, _), (res, th, cr) in zip(tr, outs):
+            size, planes = w["cfg"].width, w["model_lut"].shape[0]
+            jobs = w["jobs"]
+            assert np.array_equal(res["frame"], jobs["frame"]) and np.array_equal(res["zlutIndex"], jobs["zlutIndex"])
+            for k in ("x", "y", "comx", "comy"):
+                assert np.all(np.isfinite(res[k])) and np.all((res[k] >= 0) & (res[k] < size)), k
+            # the 7-point fit's vertex can leave the plane range where the scores are flat (as the
+            # reference's does): finite, with the count outside the range printed
+            assert np.all(np.isfinite(res["z"]))
+            print(f"{size}px: z in [{res['z'].min():.2f}, {res['z'].max():.2f}], "
+                  f"{int(((res['z'] < 0) | (res['z'] > planes - 1)).sum())} outside [0, {planes - 1}]", flush=True)
+            ok = np.all(np.isfinite(th), 1) & np.all(np.isfinite(cr) & (cr > 0), 1)
+            print(f"{size}px: {int((~ok).sum())} of {len(ok)} MLE fits without a finite CRLB", flush=True)
+            assert ok.mean() >= 0.99
+            idx = np.linspace(0, len(jobs) - 1, 160).astype(int)
+            ref, _, _ = oracle.localize_trace(oracle.config_from_bytes(bytes(w["cfg"])), synth.MODE_QI_Z, w["lut"],
+                                              w["rois"][idx], jobs[idx])
+            got = res[idx]
+            for k in ("frame", "zlutIndex", "error"):
+                assert np.array_equal(got[k], ref[k]), k
+            d = {k: float(np.abs(got[k].astype(np.float64) - ref[k]).max()) for k in ("x", "y", "z", "comx", "comy")}
+            same = {k: int((got[k] == ref[k]).sum()) for k in ("x", "y", "z")}
+            print(f"{size}px vs oracle (160 ROIs): max |d| {d}, bit-identical {same}", flush=True)
+            assert max(d["x"], d["y"], d["comx"], d["comy"]) <= 1e-3 and d["z"] <= 1e-3
+    finally:
+        p.free()
+        for _, trk in tr:
+            trk.free()
