@@ -69,14 +69,45 @@ def copy_probe(mib=256, reps=3):
                       "launches": reps}), flush=True)
 
 
-def cpu_baseline(w, seconds_target=12.0):
+# Worker threads of the CPU leg: the host cores this process may run on, up to the per-GPU CPU
+# share of the measurement hosts (16 cores per GPU; their schedulers cap a GPU job's threads there)
+CPU_SHARE = 16
+
+
+def host_info():
+    """nproc, the cores this process may use, and the CPU model (lscpu's "Model name")."""
+    model = None
+    try:
+        import subprocess
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"nproc": os.cpu_count(), "affinity": aff, "model": model}
+
+
+def cpu_baseline(w, seconds_target=20.0):
+    """The oracle's QueuedCPUTracker restatement (job queue, worker threads, per-ROI ProcessJob) on
+    the host cores, on a bounded sample of the same ROIs: ~seconds_target of single-core CPU work
+    spread over the worker threads; the clock runs from the first job queued until every result is in (the reference's
+    speed-test clock).  A single-thread run of 256 ROIs sizes the sample and is reported too."""
     from oracle.bind import Oracle
 
     o = Oracle()
     cfg = o.config_from_bytes(bytes(w["cfg"]))
-    threads = min(os.cpu_count() or 1, 16)
+    host = host_info()
+    threads = max(1, min(host["affinity"], CPU_SHARE))
     n = len(w["jobs"])
-    # calibrate on a short single-thread run, then size the all-core sample to ~seconds_target of CPU work
     k1 = 256
     t1, _ = o.queued_run(cfg, w["mode"], w["luts"], w["rois"][:k1], w["jobs"][:k1], 1)
     per_loc = t1 / k1
@@ -86,8 +117,10 @@ def cpu_baseline(w, seconds_target=12.0):
     jobs["frame"] = np.arange(len(jobs))
     t, _ = o.queued_run(cfg, w["mode"], w["luts"], rois, jobs, threads)
     return {"value": round(len(jobs) / t, 1), "unit": "localizations/s", "cores": threads, "kind": "port",
-            "sample": f"{len(jobs)} ROIs ({reps}x the {n}-ROI batch), {threads} worker threads, oracle "
-                      f"QueuedCPUTracker restatement; 1 thread: {k1 / t1:.0f} loc/s"}
+            "one_thread": round(k1 / t1, 1), "host": host,
+            "sample": f"{len(jobs)} ROIs ({reps}x the {n}-ROI batch) on {threads} worker threads (the host's "
+                      f"cores available to this process, up to its {CPU_SHARE}-core per-GPU share), oracle "
+                      f"QueuedCPUTracker restatement; one thread: 256 ROIs"}
 
 
 def precision_profiles(beads, frames, planes, seed):
@@ -428,6 +461,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--rois", type=int, default=None, help="ROIs per step (realtime100: 4096, plain64: 1000)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-frames", action="store_true", help="skip the PCIe-inclusive frame-path rate")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--workload", default="realtime100",
                     choices=["realtime100", "plain64", "queue", "precision", "mle", "batching", "findbeads"])
@@ -495,57 +529,84 @@ def findbeads_bench(a):
     print(json.dumps(out), flush=True)
 
 
+def _camera_frames(w, per=256, side=16):
+    """The realtime100 batch as 16 camera frames of 1,600 x 1,600 u16, each holding a 16 x 16 grid of
+    100 x 100 ROIs (one per bead): (frames, ROI positions, one job per frame)."""
+    from qtrk_amd import api
+
+    n = len(w["jobs"])
+    nf = n // per
+    frames = np.zeros((nf, side * 100, side * 100), np.uint16)
+    for k in range(n):
+        f, i = divmod(k, per)
+        r, c = divmod(i, side)
+        frames[f, r * 100:(r + 1) * 100, c * 100:(c + 1) * 100] = w["rois"][k]
+    pos = np.array([[(i % side) * 100, (i // side) * 100] for i in range(per)], np.int32)
+    jobs = [api.make_jobs(1, frame=np.array([f], np.uint32), zlut_index=np.array([0], np.int32))[0]
+            for f in range(nf)]
+    return frames, pos, jobs
+
+
+def frame_path_rates(trk, frames, pos, jobs, steps, warmup):
+    """Localizations/s through QTrkScheduleFrame from host frames (copies into pinned memory and over
+    PCIe included), two ways:
+      * drained: the clock of the reference's speed tests (cputrack-test/SharedTests.h:300-314,
+        cudatrack-test/test.cu:310-371) -- from the first schedule call until GetResultCount() == N
+        after Flush, N = one pass of all frames; min/median/max over `steps` passes;
+      * stream: the frames of `steps` passes scheduled back to back with results fetched as they
+        arrive, then Flush and the last results (a camera stream's sustained rate).
+    Returns (dict, the last drained pass's results)."""
+    nf = len(frames)
+    n = nf * len(pos)
+
+    def one_pass():
+        for f in range(nf):
+            trk.schedule_frame(frames[f], pos, jobs[f])
+        trk.wait_for(n)
+        return trk.fetch_results(n)
+
+    def stream(k):
+        got = 0
+        for _ in range(k):
+            for f in range(nf):
+                trk.schedule_frame(frames[f], pos, jobs[f])
+            got += len(trk.fetch_results())
+        trk.wait_for(n * k - got)
+        got += len(trk.fetch_results())
+        return got
+
+    for _ in range(max(1, warmup)):
+        one_pass()
+    stream(max(1, warmup))
+    t0 = time.perf_counter()
+    got = stream(steps)
+    wall = time.perf_counter() - t0
+    assert got == n * steps, got
+    passes = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        res = one_pass()
+        passes.append(time.perf_counter() - t)
+    med = float(np.median(passes))
+    return {"drained_locs_per_s": round(n / med, 1),
+            "drained_pass_ms_min_median_max": [round(v * 1e3, 3) for v in (min(passes), med, max(passes))],
+            "stream_locs_per_s": round(n * steps / wall, 1),
+            "pcie_gbs_stream": round(frames.nbytes * steps / wall / 1e9, 2),
+            "host_bytes_per_pass": int(frames.nbytes)}, res
+
+
 def queue_bench(a):
     """The realtime100 ROIs through the drop-in queue API from host memory: 16 camera frames of
     1,600 x 1,600 u16 per step, each holding a 16 x 16 grid of the batch's 100 x 100 ROIs (one per
-    bead), handed to QTrkScheduleFrame with the 256 ROI positions.  Timed as a camera stream: the
-    frames of all steps are scheduled one after another and the results fetched as they arrive,
-    then Flush and the last results.  The rate includes the frames' copies into pinned memory and
-    over PCIe.  A drained pass (16 frames, Flush, wait for all 4,096 results) is timed as well."""
+    bead), handed to QTrkScheduleFrame with the 256 ROI positions (frame_path_rates)."""
     from qtrk_amd import api, synth
 
     w = synth.make_workload("realtime100", n=4096)
     trk = api.QTrkInstance(w["settings"])
     trk.set_radial_zlut(w["luts"])
     trk.set_localization_mode(w["mode"])
-    per, side = 256, 16
-    frames = np.zeros((16, side * 100, side * 100), np.uint16)
-    for k in range(4096):
-        f, i = divmod(k, per)
-        r, c = divmod(i, side)
-        frames[f, r * 100:(r + 1) * 100, c * 100:(c + 1) * 100] = w["rois"][k]
-    pos = np.array([[(i % side) * 100, (i // side) * 100] for i in range(per)], np.int32)
-    jobs = [api.make_jobs(1, frame=np.array([f], np.uint32), zlut_index=np.array([0], np.int32))[0]
-            for f in range(16)]
-
-    def one_pass():
-        for f in range(16):
-            trk.schedule_frame(frames[f], pos, jobs[f])
-        trk.wait_for(4096)
-        return trk.fetch_results(4096)
-
-    def stream(steps):
-        got = 0
-        for _ in range(steps):
-            for f in range(16):
-                trk.schedule_frame(frames[f], pos, jobs[f])
-            got += len(trk.fetch_results())
-        trk.wait_for(4096 * steps - got)
-        got += len(trk.fetch_results())
-        return got
-
-    for _ in range(a.warmup):
-        one_pass()
-    stream(a.warmup)
-    t0 = time.perf_counter()
-    got = stream(a.steps)
-    wall = time.perf_counter() - t0
-    assert got == 4096 * a.steps, got
-    passes = []
-    for _ in range(a.steps):
-        t = time.perf_counter()
-        res = one_pass()
-        passes.append((time.perf_counter() - t) * 1e3)
+    frames, pos, jobs = _camera_frames(w)
+    rates, res = frame_path_rates(trk, frames, pos, jobs, a.steps, a.warmup)
     # the same localizations as the device-resident batch (results are LIFO: order by frame, bead)
     b = api.DeviceBatch(trk, 4096, api.QTrkU16)
     b.upload(w["rois"], w["jobs"])
@@ -556,18 +617,14 @@ def queue_bench(a):
     trk.free()
     res = res[np.lexsort((res["zlutIndex"], res["frame"]))]
     same = bool(np.array_equal(res["x"], ref["x"]) and np.array_equal(res["z"], ref["z"]))
-    value = 4096 * a.steps / wall
+    value = rates["stream_locs_per_s"]
     out = {"metric": "localizations/sec through QTrkScheduleFrame from host frames (PCIe included)",
-           "value": round(value, 1), "unit": "localizations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
-           "ms_per_step": round(wall / a.steps * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "value": value, "unit": "localizations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "ms_per_step": round(4096 / value * 1e3, 4), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-           "config": {"workload": "realtime100 ROIs as 16 frames of 1600x1600 u16 (256 ROIs each) per step, "
-                                  "QTrkScheduleFrame streamed, results fetched as they arrive",
-                      "host_bytes_per_step": int(frames.nbytes),
-                      "pcie_gbs": round(frames.nbytes * a.steps / wall / 1e9, 2),
-                      "drained_pass_ms_min_median_max": [round(float(v), 3) for v in
-                                                         (min(passes), np.median(passes), max(passes))],
-                      "results_equal_device_batch": same}}
+           "config": dict({"workload": "realtime100 ROIs as 16 frames of 1600x1600 u16 (256 ROIs each) per step, "
+                                       "QTrkScheduleFrame streamed, results fetched as they arrive",
+                           "results_equal_device_batch": same}, **rates)}
     print(json.dumps(out), flush=True)
 
 
@@ -646,6 +703,13 @@ def localize_bench(a, name):
                                          "peak": VALU_PEAK_WAVE_INSTR / 1e9, "unit": "G wave-instr/s",
                                          "frac": round(valu / kern_s / VALU_PEAK_WAVE_INSTR, 4)}
     b.free()
+    if realtime and n == 4096 and not a.no_frames:
+        # the same batch from host memory through the drop-in C ABI (PCIe included): reported
+        # beside the device-resident value, never as it
+        frames, pos, jobs = _camera_frames(w)
+        rates, _ = frame_path_rates(trk, frames, pos, jobs, 5, 1)
+        out["pcie_inclusive"] = dict({"path": "QTrkScheduleFrame: 16 host frames of 1600x1600 u16, 256 ROIs each",
+                                      "unit": "localizations/s"}, **rates)
     trk.free()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)

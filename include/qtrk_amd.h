@@ -18,7 +18,9 @@ QTRK_API void QTrkClearLastError(void);
  * which the reference's C ABI leaves commented out (cputrack/qtrk_c_api.h:193-196).
  * Keys: "batch_size" (ROIs per GPU launch), "num_streams" (batches in flight),
  * "frame_copy_threads" (threads, the caller's included, that copy a
- * QTrkScheduleFrame frame into pinned memory; default 5).
+ * QTrkScheduleFrame frame into pinned memory; default 5), "queue_spin_us" (how long a
+ * schedule call blocked on a full queue polls the oldest batch before it sleeps in a
+ * blocking wait; default 2000).
  * GetConfigValues writes "key=value" lines. */
 QTRK_API void QTrkSetConfigValue(QueuedTracker* qtrk, const char* name, const char* value);
 QTRK_API void QTrkGetConfigValues(QueuedTracker* qtrk, char* dst, int maxStrLen);

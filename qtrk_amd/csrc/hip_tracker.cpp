@@ -350,19 +350,20 @@ QueuedHIPTracker::Slot* QueuedHIPTracker::TakeFreeSlot()
 	return nullptr;
 }
 
-// Waits for `ev` with the lock released.  The event is polled for up to kSpinWait (a batch
-// that is about to finish resumes the caller at once) and then waited on with a blocking
+// Waits for `ev` with the lock released.  The event is polled for up to spinWait (a batch that
+// is about to finish resumes the caller at once) and then waited on with a blocking
 // synchronisation, so a long wait does not keep a core busy.  `waiters` keeps the slots (and
 // their events) alive meanwhile: code that frees them waits until it is back to zero.  Other
 // threads may change `filling` and the slot states while this waits.
 void QueuedHIPTracker::WaitEventLocked(std::unique_lock<std::mutex>& lk, hipEvent_t ev)
 {
 	waiters++;
+	const auto spin = spinWait;
 	lk.unlock();
 	const auto t0 = std::chrono::steady_clock::now();
 	hipError_t e;
 	while ((e = hipEventQuery(ev)) == hipErrorNotReady) {
-		if (std::chrono::steady_clock::now() - t0 > kSpinWait) {
+		if (std::chrono::steady_clock::now() - t0 > spin) {
 			e = hipEventSynchronize(ev);
 			break;
 		}
@@ -1052,6 +1053,11 @@ void QueuedHIPTracker::SetConfigValue(std::string name, std::string value)
 		if (name == "batch_size") batchSize = v;
 		else numSlots = v;
 		AllocSlots();
+	} else if (name == "queue_spin_us") {
+		int v = atoi(value.c_str());
+		if (v < 0 || v > 1000000) fail("queue_spin_us must be in 0..1000000");
+		std::unique_lock<std::mutex> lk(mtx);
+		spinWait = std::chrono::microseconds(v);
 	} else if (name == "frame_copy_threads") {
 		int v = atoi(value.c_str());
 		if (v < 1 || v > 64) fail("frame_copy_threads must be in 1..64");
@@ -1067,6 +1073,7 @@ QueuedTracker::ConfigValueMap QueuedHIPTracker::GetConfigValues()
 	m["batch_size"] = std::to_string(batchSize);
 	m["num_streams"] = std::to_string(numSlots);
 	m["frame_copy_threads"] = std::to_string(frameCopyThreads);
+	m["queue_spin_us"] = std::to_string((long long)spinWait.count());
 	return m;
 }
 

@@ -170,7 +170,8 @@ private:
 	int waiters = 0;
 	bool quiescing = false;
 	std::condition_variable slots_cv;
-	static constexpr std::chrono::microseconds kSpinWait{ 200 };
+	// how long a wait on a batch polls its event before a blocking synchronisation (queue_spin_us)
+	std::chrono::microseconds spinWait{ 2000 };
 	std::mutex res_mtx;
 	std::deque<LocalizationResult> results;
 
