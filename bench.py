@@ -12,6 +12,7 @@ sample of the same workload.
     python bench.py --workload precision   # grouped Z-LUT match, exact vs BF16/FP16 matrix cores
     python bench.py --workload mle         # LUT-model MLE + Fisher/CRLB, and the 2D Gaussian MLE
     python bench.py --workload batching    # 4 trackers in one hipGraph per batch, p50/p99 latency
+    python bench.py --workload mixed       # fused FP32 Z-LUT compare vs grouped BF16/FP16 match (device batch)
 """
 import argparse
 import json
@@ -464,7 +465,7 @@ def main():
     ap.add_argument("--no-frames", action="store_true", help="skip the PCIe-inclusive frame-path rate")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--workload", default="realtime100",
-                    choices=["realtime100", "plain64", "queue", "precision", "mle", "batching", "findbeads"])
+                    choices=["realtime100", "plain64", "queue", "precision", "mle", "batching", "findbeads", "mixed"])
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -487,6 +488,9 @@ def main():
         return
     if a.workload == "findbeads":
         findbeads_bench(a)
+        return
+    if a.workload == "mixed":
+        mixed_bench(a)
         return
 
     localize_bench(a, a.workload)
@@ -625,6 +629,62 @@ def queue_bench(a):
            "config": dict({"workload": "realtime100 ROIs as 16 frames of 1600x1600 u16 (256 ROIs each) per step, "
                                        "QTrkScheduleFrame streamed, results fetched as they arrive",
                            "results_equal_device_batch": same}, **rates)}
+    print(json.dumps(out), flush=True)
+
+
+def mixed_bench(a):
+    """The realtime100 batch (4,096 ROIs of 100x100 u16, 256 beads x 16 frames) with the Z-LUT match
+    done two ways, on the default 200 x 64 LUTs and on 500 x 128 LUTs (the precision workload's size):
+      * fused FP32: one localize_kernel per batch, each ROI's LUT scores in the reference
+        arithmetic (FP64 sums) inside its workgroup -- the headline path;
+      * split BF16 / FP16: the device batch with QTrkBatchSetMatchPrecision -- the kernel in
+        profile-only form, the ROIs grouped by bead on the GPU, then each bead's 16 profiles matched
+        together on the matrix cores, the 8 best planes and the fit window rescored in the reference
+        arithmetic, FP64 fit.
+    Per form: loc/s over consecutive batches, and the error against the truth (bias = mean,
+    sigma = standard deviation; x, y in px, z in planes) with the count of results identical to
+    the fused form's."""
+    from qtrk_amd import api, synth
+
+    cases = {}
+    for label, over, planes in (("lut_200x64", {}, 200), ("lut_500x128", {"zlut_radial_coverage": 2.6224}, 500)):
+        w = synth.make_workload("realtime100", n=4096, planes=planes, **over)
+        n = len(w["jobs"])
+        trk = api.QTrkInstance(w["settings"])
+        trk.set_radial_zlut(w["luts"])
+        trk.set_localization_mode(w["mode"])
+        b = api.DeviceBatch(trk, n, api.QTrkU16)
+        b.upload(w["rois"], w["jobs"])
+        forms, fused = {}, None
+        for name, prec in (("fused_fp32", api.MATCH_FP32), ("split_bf16", api.MATCH_BF16),
+                           ("split_fp16", api.MATCH_FP16)):
+            b.set_match_precision(prec)
+            b.run(n, a.warmup)
+            b.sync()
+            t0 = time.perf_counter()
+            b.run(n, a.steps)
+            b.sync()
+            sec = (time.perf_counter() - t0) / a.steps
+            res = b.download(n)
+            if fused is None:
+                fused = res
+            t = w["truth"]
+            e = np.stack([res["x"] - t[:, 0], res["y"] - t[:, 1], res["z"] - t[:, 2]], 1).astype(np.float64)
+            forms[name] = {"locs_per_s": round(n / sec, 1), "ms_per_batch": round(sec * 1e3, 4),
+                           "bias_xyz": np.round(e.mean(0), 5).tolist(), "sigma_xyz": np.round(e.std(0), 5).tolist(),
+                           "identical_to_fused": int(np.sum((res["x"] == fused["x"]) & (res["y"] == fused["y"])
+                                                            & (res["z"] == fused["z"])))}
+        b.free()
+        trk.free()
+        cases[label] = {"luts": f"{w['luts'].shape[0]} beads x {w['luts'].shape[1]} planes x {w['luts'].shape[2]} steps",
+                        "forms": forms}
+    big = cases["lut_500x128"]["forms"]
+    out = {"metric": "localizations/sec, realtime100 ROIs: fused FP32 Z-LUT compare vs grouped BF16/FP16 match",
+           "value": big["split_bf16"]["locs_per_s"], "unit": "localizations/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": big["split_bf16"]["ms_per_batch"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16", "data": "synthetic",
+           "config": {"workload": "realtime100: 4096 ROIs of 100x100 u16, 256 beads x 16 frames, COM + QI(5) + Z "
+                                  "(value: the 500 x 128 LUT case, split BF16)", "cases": cases}}
     print(json.dumps(out), flush=True)
 
 

@@ -65,6 +65,17 @@ QTRK_API void* QTrkBatchDeviceResults(QTrkBatch* b);
  * previous launch's last ones leave idle; all write the same (identical)
  * results.  HIP events on the batch stream bracket all of them. */
 QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats);
+/* How a run matches the radial profiles against the Z-LUT (modes with LT_LocalizeZ):
+ * QTRK_MATCH_FP32 (the default) scores each ROI's profile inside its localization
+ * workgroup, in the reference arithmetic.  QTRK_MATCH_BF16 / QTRK_MATCH_FP16 run the
+ * localization in profile-only form, group the ROIs by bead (job.zlutIndex) on the GPU and
+ * match each bead's profiles together as QTrkLUTMatch does (matrix-core scores, the 8 best
+ * planes and the fit window rescored in the reference arithmetic, FP64 fit): results equal the
+ * FP32 form's whenever the exact best plane is among those 8.  Not with LT_ZLUTAlign,
+ * LT_LocalizeZWeighted or LT_FourierLUT, and the compare profile
+ * (QTrkEnableRadialZLUTCompareProfile) is not recorded.  The first call converts the
+ * tracker's LUT (again after the LUT or its weights change).  0 on success. */
+QTRK_API int QTrkBatchSetMatchPrecision(QTrkBatch* b, int precision);
 /* Waits for the batch stream; returns the event-timed milliseconds of the
  * last QTrkBatchRun (all repeats), or a negative value on error. */
 QTRK_API float QTrkBatchSync(QTrkBatch* b);
@@ -218,8 +229,10 @@ QTRK_API int QTrkLUTModelSync(QTrkLUTModel* m, void* stream);
  * finalize (autotune = true times the launch-shape candidates of each stage on
  * those ROIs and keeps the fastest; then the graph is captured); then per
  * batch: fill the ROI buffers (upload or QTrkPipelineDeviceROIs), launch,
- * sync, download.  Trackers must not change their settings or LUT after
- * finalize.  No counterpart in the reference (one tracker, one queue).
+ * sync, download.  Trackers must not change their settings after finalize; a
+ * tracker's Z-LUT or radial weights changed after it (SetRadialZLUT, SetRadialWeights,
+ * BuildLUT, FinalizeLUT) makes QTrkPipelineLaunch fail until the pipeline is finalized
+ * again, which converts the new LUT.  No counterpart in the reference (one tracker, one queue).
  */
 typedef struct QTrkPipeline QTrkPipeline;
 

@@ -195,6 +195,7 @@ SIGNATURES = {
     "QTrkBatchDeviceJobs": (_P, [_P]),
     "QTrkBatchDeviceResults": (_P, [_P]),
     "QTrkBatchRun": (C.c_int, [_P, C.c_int, C.c_int]),
+    "QTrkBatchSetMatchPrecision": (C.c_int, [_P, C.c_int]),
     "QTrkBatchSync": (C.c_float, [_P]),
     "QTrkBatchDownload": (C.c_int, [_P, _P, C.c_int]),
     "QTrkBatchEnableDiagnostics": (C.c_int, [_P, C.c_bool]),
@@ -540,6 +541,12 @@ class DeviceBatch:
         if lib().QTrkBatchDownload(self.h, _ptr(out), int(n)) != 0:
             check()
         return out
+
+    def set_match_precision(self, precision):
+        """MATCH_FP32 (per-ROI compare in the localization kernel) or MATCH_BF16 / MATCH_FP16
+        (profile kernel + grouped matrix-core match)."""
+        if lib().QTrkBatchSetMatchPrecision(self.h, int(precision)) != 0:
+            check()
 
     def enable_diagnostics(self, on=True):
         if lib().QTrkBatchEnableDiagnostics(self.h, bool(on)) != 0:

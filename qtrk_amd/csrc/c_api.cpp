@@ -270,7 +270,8 @@ QTRK_API ROIPosition* QTrkFindBeads(ImageData* imgData, int smpCornerPosX, int s
 struct QTrkBatch {
 	QueuedHIPTracker* trk = nullptr;
 	int maxROIs = 0, pdt = 0;
-	DeviceBuffer rois, jobs, results, best, prof;
+	int prec = kMatchFP32;  // QTrkBatchSetMatchPrecision
+	DeviceBuffer rois, jobs, results, best, prof, split;
 	bool diag = false;
 	hipStream_t stream = nullptr, stream2 = nullptr;  // repeats alternate between the two
 	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
@@ -342,6 +343,23 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			P.best_out = b->best.as<int>();
 			P.prof_out = b->prof.as<float>();
 		}
+		if (b->prec != kMatchFP32) {
+			// profile kernel + grouped match; the repeats run one after another on the batch
+			// stream, as they share the scratch
+			const std::string why = b->trk->SplitMatchUsable();
+			if (!why.empty()) fail("the grouped Z-LUT match cannot run: " + why);
+			const LutMatchTables* tab = b->trk->PrepareMatchTables(b->prec);
+			const int nb = b->trk->cfg.zlut_radialsteps;
+			b->split.ensure(QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, nb, tab->beads));
+			QueuedHIPTracker::SplitScratch sc = QueuedHIPTracker::SplitScratch::Carve(b->split.p, b->maxROIs, nb, tab->beads);
+			if (b->diag) sc.prof = b->prof.as<float>();
+			HIP_CHECK(hipEventRecord(b->e0, b->stream));
+			for (int r = 0; r < repeats; r++)
+				b->trk->LocalizeSplit(b->rois.p, b->pdt, b->jobs.as<DevJob>(), b->results.as<DevResult>(), n, b->prec,
+				                      tab, sc, b->diag ? b->best.as<int>() : nullptr, b->stream);
+			HIP_CHECK(hipEventRecord(b->e1, b->stream));
+			return;
+		}
 		// Repeats of the batch alternate between two streams, so that a launch's first
 		// workgroups fill the CUs the previous launch's last ones leave idle, as consecutive
 		// camera batches do; e0 .. e1 spans both.  They write the same (identical) results.
@@ -353,6 +371,23 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			HIP_CHECK(hipStreamWaitEvent(b->stream, b->e2, 0));
 		}
 		HIP_CHECK(hipEventRecord(b->e1, b->stream));
+	});
+	return ok ? 0 : -1;
+}
+
+QTRK_API int QTrkBatchSetMatchPrecision(QTrkBatch* b, int precision)
+{
+	bool ok = GUARD({
+		if (!b) fail("null batch");
+		if (precision < kMatchFP32 || precision > kMatchFP16) fail("unknown match precision");
+		HIP_CHECK(hipSetDevice(b->trk->Device()));
+		if (precision != kMatchFP32) {
+			const std::string why = b->trk->SplitMatchUsable();
+			if (!why.empty()) fail("the grouped Z-LUT match cannot run: " + why);
+			const LutMatchTables* tab = b->trk->PrepareMatchTables(precision);
+			b->split.ensure(QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, b->trk->cfg.zlut_radialsteps, tab->beads));
+		}
+		b->prec = precision;
 	});
 	return ok ? 0 : -1;
 }

@@ -467,7 +467,17 @@ void QueuedHIPTracker::Launch(Slot* s)
 		           s->d_results.as<DevResult>());
 	}
 	HIP_CHECK(hipEventRecord(s->ev_k0, s->stream));
-	HIP_CHECK(launch_localize(P, s->stream));
+	if (matchPrec != kMatchFP32 && SplitMatchUsable().empty()) {
+		// match_precision bf16/fp16: profile kernel + grouped match (modes it cannot take run fused)
+		const LutMatchTables* tab = MatchTablesLocked(matchPrec, s->stream);
+		const int nb = cfg.zlut_radialsteps;
+		s->d_split.ensure(SplitScratch::Bytes(batchSize, nb, tab->beads));
+		const SplitScratch sc = SplitScratch::Carve(s->d_split.p, batchSize, nb, tab->beads);
+		LocalizeSplit(P.src, P.pdt, P.jobs, P.results, s->n, matchPrec, tab, sc, nullptr, s->stream, P.roi_offset,
+		              P.pitch);
+	} else {
+		HIP_CHECK(launch_localize(P, s->stream));
+	}
 	HIP_CHECK(hipEventRecord(s->ev_k1, s->stream));
 	HIP_CHECK(hipMemcpyAsync(s->h_results.p, s->d_results.p, sizeof(DevResult) * s->n, hipMemcpyDeviceToHost, s->stream));
 	HIP_CHECK(hipEventRecord(s->ev_done, s->stream));
@@ -857,6 +867,7 @@ void QueuedHIPTracker::SetRadialZLUT(float* data, int count, int planes)
 		zcount = zplanes = 0;
 	}
 	h_zlut_stale = false;
+	lutVersion++;
 }
 
 void QueuedHIPTracker::GetRadialZLUT(float* dst)
@@ -885,6 +896,7 @@ void QueuedHIPTracker::SetRadialWeights(float* w)
 		has_zcmp = true;
 	} else
 		has_zcmp = false;
+	lutVersion++;
 }
 
 void QueuedHIPTracker::EnableRadialZLUTCompareProfile(bool enabled)
@@ -977,6 +989,7 @@ void QueuedHIPTracker::BuildLUT(void* data, int pitch, QTRK_PixelDataType pdt, i
 	HIP_CHECK(launch_localize(P, aux_stream()));
 	HIP_CHECK(hipStreamSynchronize(aux_stream()));
 	h_zlut_stale = true;
+	lutVersion++;
 }
 
 // QueuedCPUTracker::GetImageZLUTSize / GetImageZLUT / SetImageZLUT
@@ -1033,6 +1046,102 @@ void QueuedHIPTracker::FinalizeLUT()
 	HIP_CHECK(hipStreamSynchronize(aux_stream()));
 	HIP_CHECK(hipMemcpy(h_zlut.data(), d_zlut.p, h_zlut.size() * 4, hipMemcpyDeviceToHost));
 	h_zlut_stale = false;
+	lutVersion++;
+}
+
+// ------------------------------------------------- split localization
+
+const LutMatchTables* QueuedHIPTracker::PrepareMatchTables(int prec)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	if (prec < kMatchFP32 || prec > kMatchFP16) fail("unknown match precision");
+	if (zcount <= 0 || zplanes <= 0) fail("no radial Z-LUT");
+	WaitAllLocked(lk);
+	const LutMatchTables* t = MatchTablesLocked(prec, aux_stream());
+	HIP_CHECK(hipStreamSynchronize(aux_stream()));
+	return t;
+}
+
+const LutMatchTables* QueuedHIPTracker::MatchTablesLocked(int prec, hipStream_t stream)
+{
+	if (!matchTab || matchTabVersion != lutVersion || matchTab->beads != zcount || matchTab->planes != zplanes) {
+		if (!matchTab || matchTab->beads != zcount || matchTab->planes != zplanes)
+			matchTab.reset(new LutMatchTables(zcount, zplanes, cfg.zlut_radialsteps));
+		matchTab->set(d_zlut.as<float>(), has_zcmp ? d_zcmp.as<float>() : nullptr, true, stream);
+		matchTabVersion = lutVersion;
+	}
+	matchTab->prepare(prec, stream);
+	return matchTab.get();
+}
+
+const LutMatchTables* QueuedHIPTracker::CurrentMatchTables(int prec) const
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	if (!matchTab || matchTabVersion != lutVersion || matchTab->beads != zcount || matchTab->planes != zplanes)
+		return nullptr;
+	if (prec != kMatchFP32 && !(prec >= 0 && prec < 3 && matchTab->prepared[prec])) return nullptr;
+	return matchTab.get();
+}
+
+std::string QueuedHIPTracker::SplitMatchUsable() const
+{
+	if (!(localizeMode & LT_LocalizeZ)) return "the localization mode has no LT_LocalizeZ";
+	if (localizeMode & (LT_ZLUTAlign | LT_LocalizeZWeighted | LT_FourierLUT))
+		return "LT_ZLUTAlign, LT_LocalizeZWeighted and LT_FourierLUT need the per-ROI compare";
+	if (zcount <= 0 || zplanes <= 0) return "no radial Z-LUT";
+	return "";
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t QueuedHIPTracker::SplitScratch::Bytes(int n, int nb, int beads)
+{
+	return align256((size_t)n * nb * sizeof(float)) + align256((size_t)beads * sizeof(int)) +
+	       align256((size_t)(beads + 1) * sizeof(int)) + align256((size_t)n * sizeof(int));
+}
+
+QueuedHIPTracker::SplitScratch QueuedHIPTracker::SplitScratch::Carve(void* base, int n, int nb, int beads)
+{
+	SplitScratch s;
+	char* p = (char*)base;
+	s.prof = (float*)p;
+	p += align256((size_t)n * nb * sizeof(float));
+	s.counts = (int*)p;
+	p += align256((size_t)beads * sizeof(int));
+	s.start = (int*)p;
+	p += align256((size_t)(beads + 1) * sizeof(int));
+	s.rowidx = (int*)p;
+	return s;
+}
+
+void QueuedHIPTracker::LocalizeSplit(const void* rois, int pdt, const DevJob* jobs, DevResult* results, int n,
+                                     int prec, const LutMatchTables* tab, const SplitScratch& s, int* best_out,
+                                     hipStream_t stream, const int64_t* roi_offset, int pitch)
+{
+	if (n <= 0) return;
+	LocalizeParams P;
+	FillParams(P, pdt, n, rois, roi_offset, pitch > 0 ? pitch : cfg.width * PDT_BytesPerPixel((QTRK_PixelDataType)pdt),
+	           jobs, results);
+	P.zluts = nullptr;  // profiles only: z stays 0 (the fused kernel's value for ROIs without a LUT)
+	P.prof_out = s.prof;
+	P.best_out = best_out;  // -1 here; the match writes the best plane of every grouped ROI
+	P.cmp_out = nullptr;
+	HIP_CHECK(launch_localize(P, stream));
+	const int beads = tab->beads;
+	HIP_CHECK(launch_group_by_bead((const int*)((const char*)jobs + offsetof(DevJob, zlutIndex)),
+	                               (int)(sizeof(DevJob) / sizeof(int)), n, beads, s.counts, s.start, s.rowidx,
+	                               stream));
+	LutMatchParams p = tab->params(prec);
+	p.prof = s.prof;
+	p.start = s.start;
+	p.rowidx = s.rowidx;
+	p.z = (float*)((char*)results + offsetof(DevResult, z));
+	p.zstride = (int)(sizeof(DevResult) / sizeof(float));
+	p.best = best_out;
+	p.cand = nullptr;
+	// workgroups for the average bead; beads with more profiles loop over their tiles
+	HIP_CHECK(launch_lutmatch(p, prec, std::max(1, (n + beads - 1) / beads), stream, 0));
 }
 
 // ------------------------------------------------------------- config
@@ -1053,6 +1162,17 @@ void QueuedHIPTracker::SetConfigValue(std::string name, std::string value)
 		if (name == "batch_size") batchSize = v;
 		else numSlots = v;
 		AllocSlots();
+	} else if (name == "match_precision") {
+		int v = value == "fp32" ? kMatchFP32 : value == "bf16" ? kMatchBF16 : value == "fp16" ? kMatchFP16 : -1;
+		if (v < 0) fail("match_precision must be fp32, bf16 or fp16");
+		std::unique_lock<std::mutex> lk(mtx);
+		HIP_CHECK(hipSetDevice(dev));
+		if (filling && filling->n > 0) {  // ROIs queued before go with the old setting
+			Launch(filling);
+			filling = nullptr;
+		}
+		WaitAllLocked(lk);
+		matchPrec = v;
 	} else if (name == "queue_spin_us") {
 		int v = atoi(value.c_str());
 		if (v < 0 || v > 1000000) fail("queue_spin_us must be in 0..1000000");
@@ -1074,6 +1194,7 @@ QueuedTracker::ConfigValueMap QueuedHIPTracker::GetConfigValues()
 	m["num_streams"] = std::to_string(numSlots);
 	m["frame_copy_threads"] = std::to_string(frameCopyThreads);
 	m["queue_spin_us"] = std::to_string((long long)spinWait.count());
+	m["match_precision"] = matchPrec == kMatchBF16 ? "bf16" : matchPrec == kMatchFP16 ? "fp16" : "fp32";
 	return m;
 }
 

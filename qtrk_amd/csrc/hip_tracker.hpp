@@ -12,6 +12,7 @@
 #include "copy_pool.hpp"
 #include "devbuf.hpp"
 #include "localize.hpp"
+#include "lutmatch.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -88,6 +89,40 @@ public:
 	const float* DeviceZCmp() const { return has_zcmp ? d_zcmp.as<float>() : nullptr; }
 	int ZCount() const { return zcount; }
 	int ZPlanes() const { return zplanes; }
+	// Counts changes of the radial Z-LUT or its weights (set, built, normalised), so that copies
+	// of them (the grouped-match tables) can tell they are stale.
+	uint64_t LUTVersion() const { return lutVersion; }
+
+	// The grouped-match tables of the tracker's LUT and weights in precision `prec`
+	// (kMatchBF16/kMatchFP16 tile means and residuals; kMatchFP32 the FP32 copy): made, or remade
+	// after the LUT changed, by PrepareMatchTables (which allocates); CurrentMatchTables returns
+	// them only when current and prepared (null otherwise) and allocates nothing.
+	const LutMatchTables* PrepareMatchTables(int prec);
+	const LutMatchTables* CurrentMatchTables(int prec) const;
+
+	// Scratch of LocalizeSplit, carved from one block: each part 256-byte aligned.
+	struct SplitScratch {
+		float* prof = nullptr;  // [n][zlut_radialsteps] radial profiles
+		int* counts = nullptr;  // [beads] grouping counts, then fill cursors
+		int* start = nullptr;   // [beads + 1] first slot of each bead
+		int* rowidx = nullptr;  // [n] rows grouped by bead
+		static size_t Bytes(int n, int nb, int beads);
+		static SplitScratch Carve(void* base, int n, int nb, int beads);
+	};
+	// Localizes n ROIs in GPU memory (contiguous [n][height][width] of pdt) into results, with the
+	// Z-LUT match of all ROIs together instead of inside each ROI's workgroup: localize_kernel in
+	// profile-only form (COM -> QI -> the radial profile; z left 0), the rows grouped by bead, then
+	// the grouped match of precision `prec` with the tables `tab`, which writes each matched ROI's z
+	// into its result.  best_out ([n], or null) receives the best planes.  Asynchronous on
+	// `stream`; allocates nothing.  Needs LT_LocalizeZ without LT_ZLUTAlign, LT_LocalizeZWeighted
+	// and LT_FourierLUT (SplitMatchUsable).
+	// roi_offset/pitch: ROI i at rois + roi_offset[i] with rows pitch bytes apart (roi_offset null:
+	// contiguous ROIs, pitch 0: width * bytes per pixel).
+	void LocalizeSplit(const void* rois, int pdt, const DevJob* jobs, DevResult* results, int n, int prec,
+	                   const LutMatchTables* tab, const SplitScratch& s, int* best_out, hipStream_t stream,
+	                   const int64_t* roi_offset = nullptr, int pitch = 0);
+	// Empty when the current mode and LUT can take the split localization, else the reason.
+	std::string SplitMatchUsable() const;
 	bool HasCalibration() const { return has_off || has_gain; }
 
 private:
@@ -106,7 +141,11 @@ private:
 		DeviceBuffer d_off;
 		hipStream_t stream = nullptr;
 		hipEvent_t ev_start = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
+		DeviceBuffer d_split;  // LocalizeSplit's scratch, when match_precision is not fp32
 	};
+	// the grouped-match tables for a launch on `stream` (made or remade there when stale); the
+	// LUT setters wait for every batch first, so no batch in flight uses tables being remade
+	const LutMatchTables* MatchTablesLocked(int prec, hipStream_t stream);
 
 	void BuildTables();
 	void AllocSlots();
@@ -163,6 +202,11 @@ private:
 	// threads, the caller's included, that copy a frame's rows into pinned memory
 	int frameCopyThreads = 5;
 	std::unique_ptr<CopyPool> copier;  // made on the first frame
+
+	int matchPrec = kMatchFP32;  // "match_precision": how queued batches match the Z-LUT
+	uint64_t lutVersion = 0;
+	std::unique_ptr<LutMatchTables> matchTab;
+	uint64_t matchTabVersion = 0;
 
 	mutable std::mutex mtx;
 	// threads blocked on a slot's event with mtx released (WaitEventLocked); while `quiescing`

@@ -183,16 +183,12 @@ __host__ __device__ inline ExactLds exact_lds(int R, int planes)
 
 // ------------------------------------------------------------ matrix-core path
 
+// One tile of up to FT * 32 profiles of one bead (slots start[bead] + chunk * FT * 32 ...).
 template <int PREC, int FT, int KSMAX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void lutmatch_mfma_kernel(LutMatchParams P)
+__device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, int chunk, unsigned char* smem)
 {
 	typedef typename MfmaOp<PREC>::V V;
 	typedef typename MfmaOp<PREC>::E E;
-	extern __shared__ __align__(16) unsigned char smem[];
-
-	int bead, chunk;
-	xcd_map(blockIdx.x, P.chunks, bead, chunk);
-	if (bead >= P.beads) return;
 	const int s0 = P.start[bead] + chunk * FT * 32;
 	const int rem = P.start[bead + 1] - s0;
 	const int nf = rem < FT * 32 ? rem : FT * 32;
@@ -394,10 +390,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
 			return exact_score(pr, Lb + (size_t)k * R, wl, R);
 		});
 		const int row = row_s[tid];
-		P.z[row] = (float)zf;
-		P.best[row] = bi;
+		P.z[(size_t)row * P.zstride] = (float)zf;
+		if (P.best) P.best[row] = bi;
 		if (P.cand)
 			for (int c = 0; c < 8; c++) P.cand[(size_t)row * 8 + c] = cp[c];
+	}
+}
+
+// Workgroup (bead, chunk) takes the bead's profile tiles chunk, chunk + chunks, ... (one tile when the
+// grid was sized for the largest bead).
+template <int PREC, int FT, int KSMAX>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void lutmatch_mfma_kernel(LutMatchParams P)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	int bead, chunk;
+	xcd_map(blockIdx.x, P.chunks, bead, chunk);
+	if (bead >= P.beads) return;
+	const int count = P.start[bead + 1] - P.start[bead];
+	for (; chunk * FT * 32 < count; chunk += P.chunks) {
+		mfma_chunk<PREC, FT, KSMAX>(P, bead, chunk, smem);
+		__syncthreads();
 	}
 }
 
@@ -406,12 +418,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
 // Up to 16 profiles of one bead per workgroup; lane per plane, all 16 scores
 // accumulated together so each LUT row is read once per 16 profiles.
 template <bool WEIGHTED>
-__global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
+__device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, int chunk, unsigned char* smem)
 {
-	extern __shared__ __align__(16) unsigned char smem[];
-	int bead, chunk;
-	xcd_map(blockIdx.x, P.chunks, bead, chunk);
-	if (bead >= P.beads) return;
 	const int s0 = P.start[bead] + chunk * kMatchExactTile;
 	const int rem = P.start[bead + 1] - s0;
 	const int nf = rem < kMatchExactTile ? rem : kMatchExactTile;
@@ -498,9 +506,23 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 			if (bi < 0 || sc[0] != sc[0]) bi = 0;
 			const double zf = peak_fit<double>(bi, NP, kZW, [&](int k) { return sc[k]; });
 			const int row = row_s[f];
-			P.z[row] = (float)zf;
-			P.best[row] = bi;
+			P.z[(size_t)row * P.zstride] = (float)zf;
+			if (P.best) P.best[row] = bi;
 		}
+	}
+}
+
+template <bool WEIGHTED>
+__global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
+{
+	extern __shared__ __align__(16) unsigned char smem[];
+	int bead, chunk;
+	xcd_map(blockIdx.x, P.chunks, bead, chunk);
+	if (bead >= P.beads) return;
+	const int count = P.start[bead + 1] - P.start[bead];
+	for (; chunk * kMatchExactTile < count; chunk += P.chunks) {
+		exact_chunk<WEIGHTED>(P, bead, chunk, smem);
+		__syncthreads();
 	}
 }
 
@@ -636,6 +658,7 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 {
 	if (p.beads <= 0 || max_per_bead <= 0) return hipSuccess;
 	LutMatchParams q = p;
+	if (q.zstride <= 0) q.zstride = 1;
 	if (prec == kMatchFP32) {
 		q.chunks = (max_per_bead + kMatchExactTile - 1) / kMatchExactTile;
 		const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
@@ -669,6 +692,69 @@ size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per
 	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes).total;
 	const int FT = frame_tile ? frame_tile / 32 : 1;
 	return (size_t)mfma_lds(FT, res, ksteps).total;
+}
+
+// ------------------------------------------------------- grouping by bead
+
+// Profile rows grouped by bead on the GPU: counts[b] = rows of bead b (index in [0, beads); others
+// belong to no group), start = exclusive prefix of the counts, rowidx[start[b] ...] = bead b's rows.
+// Rows are placed with atomics, so their order within a bead is arbitrary: every profile's match
+// is computed on its own, so its result does not depend on its slot.
+__global__ void group_zero_kernel(int* counts, int n)
+{
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) counts[i] = 0;
+}
+
+__global__ void group_count_kernel(const int* bead_idx, int stride, int n, int beads, int* counts)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int b = bead_idx[(size_t)i * stride];
+	if (b >= 0 && b < beads) atomicAdd(&counts[b], 1);
+}
+
+// One workgroup: start[0..beads] and counts[] turned into each bead's fill cursor (= start[b]).
+__global__ __launch_bounds__(1024) void group_scan_kernel(int* counts, int beads, int* start)
+{
+	__shared__ int part[1024];
+	const int tid = threadIdx.x, per = (beads + 1023) / 1024, b0 = tid * per;
+	int s = 0;
+	for (int b = b0; b < b0 + per && b < beads; b++) s += counts[b];
+	part[tid] = s;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the per-thread sums
+		const int v = tid >= off ? part[tid - off] : 0;
+		__syncthreads();
+		part[tid] += v;
+		__syncthreads();
+	}
+	int run = tid ? part[tid - 1] : 0;
+	for (int b = b0; b < b0 + per && b < beads; b++) {
+		const int c = counts[b];
+		start[b] = run;
+		counts[b] = run;
+		run += c;
+	}
+	if (tid == 1023) start[beads] = part[1023];
+}
+
+__global__ void group_scatter_kernel(const int* bead_idx, int stride, int n, int beads, int* cursor, int* rowidx)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int b = bead_idx[(size_t)i * stride];
+	if (b >= 0 && b < beads) rowidx[atomicAdd(&cursor[b], 1)] = i;
+}
+
+hipError_t launch_group_by_bead(const int* bead_idx, int stride, int n, int beads, int* counts, int* start,
+                                int* rowidx, hipStream_t stream)
+{
+	if (beads <= 0) return hipSuccess;
+	group_zero_kernel<<<grid_for((size_t)beads), 256, 0, stream>>>(counts, beads);
+	if (n > 0) group_count_kernel<<<(n + 255) / 256, 256, 0, stream>>>(bead_idx, stride, n, beads, counts);
+	group_scan_kernel<<<1, 1024, 0, stream>>>(counts, beads, start);
+	if (n > 0) group_scatter_kernel<<<(n + 255) / 256, 256, 0, stream>>>(bead_idx, stride, n, beads, counts, rowidx);
+	return hipGetLastError();
 }
 
 }  // namespace qtrk

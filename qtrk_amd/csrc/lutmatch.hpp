@@ -51,8 +51,9 @@ struct LutMatchParams {
 	const int* rowidx;
 	int chunks;               // profile tiles per bead in the grid
 	// outputs, indexed by profile row
-	float* z;
-	int* best;
+	float* z;                 // z of row i at z[i * zstride] (zstride 0 or 1: dense)
+	int zstride;
+	int* best;                // or null
 	int* cand;                // [rows][8] candidate planes (matrix-core path), or null
 };
 
@@ -68,8 +69,16 @@ hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* fra
 // although a bead's LUT tiles are then read once per 32 profiles).
 hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream,
                            int frame_tile = 0);
+// The launch's grid has ceil(max_per_bead / tile) workgroups per bead; a workgroup also takes the
+// bead's later tiles (every ceil(max_per_bead / tile)-th), so a smaller max_per_bead than the
+// largest bead's count is correct, only slower.
 // Dynamic LDS bytes of that launch.
 size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile = 0);
+
+// Groups rows 0..n-1 by bead (bead of row i at bead_idx[i * stride]; rows of other indices are left
+// out): counts [beads] (scratch, ends as fill cursors), start [beads + 1], rowidx [n].
+hipError_t launch_group_by_bead(const int* bead_idx, int stride, int n, int beads, int* counts, int* start,
+                                int* rowidx, hipStream_t stream);
 
 // Device-resident LUT of a grouped match in every precision's format: the FP32
 // LUT and weights, and per precision (built on first use) the tile means,

@@ -95,6 +95,7 @@ int Pipeline::Add(QueuedHIPTracker* trk, int n, int pdt, int precision, const fl
 	t->match.reset(new LutMatchTables(t->beads, trk->ZPlanes(), nb));
 	t->match->set(trk->DeviceZLUT(), trk->DeviceZCmp(), true, s0);
 	t->match->prepare(precision, s0);
+	t->lut_version = trk->LUTVersion();
 	if (model_luts) {
 		if (model_count <= 0 || model_planes < 4 || model_steps < 2 || !(model_maxr > model_minr))
 			fail("invalid model LUT");
@@ -298,6 +299,12 @@ void Pipeline::Finalize(bool autotune)
 	for (auto& t : ts) {
 		if (t->trk->ZPlanes() != t->match->planes || t->trk->ZCount() != t->match->beads)
 			fail("a tracker's LUT changed size after it was added");
+		if (t->trk->LUTVersion() != t->lut_version) {  // LUT or weights set, built or normalised since
+			t->match->set(t->trk->DeviceZLUT(), t->trk->DeviceZCmp(), true, s0);
+			t->match->prepare(t->precision, s0);
+			HIP_CHECK(hipStreamSynchronize(s0));
+			t->lut_version = t->trk->LUTVersion();
+		}
 		if (autotune) tune(*t);
 		else {
 			t->lds_pix = 2;
@@ -333,6 +340,9 @@ void Pipeline::Finalize(bool autotune)
 void Pipeline::Launch()
 {
 	if (!finalized) fail("call QTrkPipelineFinalize first");
+	for (auto& t : ts)
+		if (t->trk->LUTVersion() != t->lut_version)
+			fail("a tracker's Z-LUT or radial weights changed after QTrkPipelineFinalize: finalize again");
 	HIP_CHECK(hipSetDevice(dev));
 	HIP_CHECK(hipEventRecord(e0, s0));
 	HIP_CHECK(hipGraphLaunch(exec, s0));

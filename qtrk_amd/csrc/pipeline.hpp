@@ -2,7 +2,8 @@
 //
 // Per tracker (its own ROI size, settings, LUT and pixel type) the chain is
 //   COM -> QI -> radial profile     (localize_kernel, the tracker's settings, no per-ROI LUT compare)
-//   -> grouped Z-LUT match           (lutmatch, the tracker's LUT, BF16/FP16/FP32, profiles grouped by bead)
+//   -> grouped Z-LUT match           (lutmatch, a converted copy of the tracker's LUT, BF16/FP16/FP32,
+//                                     profiles grouped by bead; remade at finalize when the LUT changed)
 //   -> start assembly -> LUT-model MLE (lutmodel_fit_kernel on the ROI pixels, the tracker's image LUT).
 // The chains of all trackers are captured as parallel branches of one graph,
 // so one hipGraphLaunch serves every tracker's batch.  Launch shapes (LDS
@@ -35,6 +36,7 @@ struct PipelineTracker {
 	QueuedHIPTracker* trk = nullptr;
 	int n = 0, pdt = 0, width = 0, height = 0, beads = 0;
 	int precision = kMatchBF16, iters = 5;
+	uint64_t lut_version = 0;  // the tracker's LUTVersion() the match tables were made from
 	bool mle = false;
 	DeviceBuffer rois, jobs, results, prof, z, best, cand, theta0, theta, crlb;
 	DeviceBuffer start, rowidx;

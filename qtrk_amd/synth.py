@@ -129,9 +129,10 @@ def _render(luts, lut_index, pos, cfg, peak, bg, rng, dtype):
     return np.clip(cnt, 0, info.max).astype(dtype)
 
 
-def make_workload(name, n=None, seed=None, **overrides):
+def make_workload(name, n=None, seed=None, planes=None, **overrides):
     """Returns dict(settings, mode, luts, rois, jobs, truth) for a named workload; `overrides`
-    replace QTrkSettings fields (the LUTs follow the resulting radial sampling).
+    replace QTrkSettings fields (the LUTs follow the resulting radial sampling).  realtime100 with
+    `planes` other than 200 spreads the same LUT family (and the z range) over that many planes.
 
     plain64:      1 bead, 100 planes x 32 steps, n (default 1000) float32 64x64 ROIs,
                   centre +-4 px, z in [5, 95], peak 10,000, background 20.
@@ -158,16 +159,18 @@ def make_workload(name, n=None, seed=None, **overrides):
         bg = np.full(n, 20.0)
         dtype = np.float32
     else:
-        nbeads, nframes, planes = 256, 16, 200
+        nbeads, nframes = 256, 16
+        planes = 200 if planes is None else planes
+        sc = 200.0 / planes
         n = nbeads * nframes if n is None else n
         periods = rng.uniform(7, 10, size=nbeads)
-        render = np.stack([analytic_lut(planes, cfg.zlut_radialsteps, cfg.zlut_minradius, cfg.zlut_maxradius, p)
-                           for p in periods])
+        render = np.stack([analytic_lut(planes, cfg.zlut_radialsteps, cfg.zlut_minradius, cfg.zlut_maxradius, p,
+                                        dperiod=0.04 * sc, dwidth=0.05 * sc) for p in periods])
         k = np.arange(n)
         bead = (k % nbeads).astype(np.int32)
         frame = (k // nbeads).astype(np.uint32)
         off = rng.uniform(-5, 5, size=(n, 2))
-        z = rng.uniform(10, 190, size=n)
+        z = rng.uniform(10 / sc, 190 / sc, size=n)
         peak = rng.uniform(5000, 20000, size=n)
         bg = np.full(n, 50.0)
         dtype = np.uint16

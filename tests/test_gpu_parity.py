@@ -127,6 +127,77 @@ def test_batching_tracker_sizes_parity(gpu, oracle, spec):
     _assert_parity(got, ref, best, rbest)
 
 
+@pytest.mark.parametrize("prec", [api.MATCH_BF16, api.MATCH_FP16], ids=["bf16", "fp16"])
+def test_batch_grouped_match_parity(gpu, oracle, prec):
+    """A device batch with the grouped Z-LUT match (QTrkBatchSetMatchPrecision): the localization
+    kernel in profile-only form, the ROIs grouped by bead on the GPU, each bead's profiles matched
+    together on the matrix cores with the 8 best planes and the fit window rescored in the
+    reference arithmetic.  Against the oracle's ProcessJob: indexing and best plane bit-exact, x/y/z
+    within the bars, profiles equal, with one bead holding 200 of the 1,024 ROIs (its workgroups
+    loop over several profile tiles) and ROIs of a bead outside the LUT (z 0, best plane -1, as in
+    the per-ROI form); after SetRadialWeights the next run matches with the new weights."""
+    w = synth.make_workload("realtime100", n=1024)
+    jobs = w["jobs"].copy()
+    jobs["zlutIndex"][:200] = 3
+    jobs["zlutIndex"][200:210] = 5000
+    trk = _tracker(w)
+    b = api.DeviceBatch(trk, 1024, api.QTrkU16)
+    b.set_match_precision(prec)
+    b.enable_diagnostics(True)
+    b.upload(w["rois"], jobs)
+    for weights in (None, np.linspace(0.5, 1.5, trk.cfg.zlut_radialsteps).astype(np.float32)):
+        if weights is not None:
+            trk.set_radial_weights(weights)
+        b.run(1024, 2)
+        b.sync()
+        got = b.download(1024)
+        best, prof = b.diagnostics(1024)
+        ref, rbest, rprof = _oracle(oracle, w, jobs=jobs, zcmp=weights)
+        _report(f"batch grouped match {'weighted' if weights is not None else 'plain'}", got, ref)
+        _assert_parity(got, ref, best, rbest)
+        np.testing.assert_allclose(prof, rprof, atol=1e-5)
+        assert np.all(got["z"][200:210] == 0) and np.all(best[200:210] == -1)
+    b.free()
+    trk.free()
+
+
+def test_queue_match_precision_bf16(gpu, oracle):
+    """match_precision = bf16: queued batches, from ScheduleLocalization and from ScheduleFrame's
+    pitched frame slots, run the profile kernel and the grouped matrix-core match; results equal
+    the oracle's ProcessJob, also after the radial weights change between batches."""
+    w = synth.make_workload("realtime100", n=300)
+    trk = _tracker(w)
+    trk.set_config_value("batch_size", "128")
+    trk.set_config_value("match_precision", "bf16")
+    assert trk.get_config_values()["match_precision"] == "bf16"
+    got = trk.localize_all(w["rois"], w["jobs"])
+    ref, _, _ = _oracle(oracle, w)
+    _report("queue bf16", got, ref)
+    _assert_parity(got, ref)
+    wts = np.linspace(0.3, 1.7, trk.cfg.zlut_radialsteps).astype(np.float32)
+    trk.set_radial_weights(wts)
+    frame = np.zeros((230, 1030), np.uint16)
+    big = np.zeros((230, 1100), np.uint16)
+    frame = big[:, :1030]
+    pos = [(3 + 102 * (i % 10), 10 + 110 * (i // 10)) for i in range(20)]
+    for i, (x, y) in enumerate(pos):
+        frame[y:y + 100, x:x + 100] = w["rois"][i]
+    job = api.make_jobs(1, frame=np.array([900]), zlut_index=np.array([0]))[0]
+    assert trk.schedule_frame(frame, pos, job) == 20
+    trk.wait_for(20)
+    r = _by_index(trk.fetch_results(20))
+    trk.free()
+    ref = _oracle_frames(oracle, w, w["luts"], [(frame, pos, job)], zcmp=wts)
+    _report("queue bf16 frames weighted", r, ref)
+    _assert_parity(r, ref)
+    with pytest.raises(api.QTrkError, match="match_precision"):
+        t2 = _tracker(w)
+        try:
+            t2.set_config_value("match_precision", "int8")
+        finally:
+            t2.free()
+
+
 def test_queue_api_matches_batch_and_is_lifo(gpu, oracle):
     w = synth.make_workload("plain64", n=300)
     trk = _tracker(w)
@@ -344,7 +415,7 @@ def _by_index(r):
     return r[np.lexsort((r["zlutIndex"], r["frame"]))]
 
 
-def _oracle_frames(oracle, w, luts, calls, mode=None):
+def _oracle_frames(oracle, w, luts, calls, mode=None, zcmp=None):
     """The oracle's QueuedTracker::ScheduleFrame on each call (frame, positions, job) or per-ROI call
     (roi array, job), then ProcessJob on the ROIs it queued; results sorted by (frame, zlutIndex)."""
     cfg = oracle.config_from_bytes(bytes(w["cfg"]))
@@ -355,7 +426,8 @@ def _oracle_frames(oracle, w, luts, calls, mode=None):
         else:
             rois, jobs = oracle.schedule_frame(cfg, c[0], c[1], c[2])
         if len(jobs):
-            out.append(oracle.localize_trace(cfg, w["mode"] if mode is None else mode, luts, rois, jobs)[0])
+            out.append(oracle.localize_trace(cfg, w["mode"] if mode is None else mode, luts, rois, jobs,
+                                             zcmp=zcmp)[0])
     return _by_index(np.concatenate(out))
 
 

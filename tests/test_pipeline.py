@@ -190,3 +190,37 @@ def test_pipeline_batching_workload_against_oracle(gpu, oracle):
         p.free()
         for _, trk in tr:
             trk.free()
+
+
+@pytest.mark.gpu
+def test_pipeline_lut_change_needs_finalize(gpu, oracle):
+    """A tracker's LUT changed after finalize (here its radial weights) makes the launch fail until
+    the pipeline is finalized again; then the grouped match scores against the new weights (its z
+    against the oracle's ProcessJob with those weights)."""
+    w = synth.batching_tracker(64, 128, 8, 80, 64, np.random.default_rng(5))
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_radial_zlut(w["lut"])
+    trk.set_localization_mode(synth.MODE_QI_Z)
+    p = api.Pipeline()
+    try:
+        s = p.add(trk, 128, api.QTrkU16, api.MATCH_BF16)
+        p.set_jobs(s, w["jobs"])
+        p.upload(s, w["rois"])
+        p.finalize(False)
+        p.launch()
+        p.sync()
+        wts = np.linspace(0.2, 2.0, trk.cfg.zlut_radialsteps).astype(np.float32)
+        trk.set_radial_weights(wts)
+        with pytest.raises(api.QTrkError, match="finalize again"):
+            p.launch()
+        p.finalize(False)
+        p.launch()
+        p.sync()
+        res = p.download(s)[0]
+        ref, _, _ = oracle.localize_trace(oracle.config_from_bytes(bytes(w["cfg"])), synth.MODE_QI_Z, w["lut"],
+                                          w["rois"], w["jobs"], zcmp=wts)
+        assert np.array_equal(res["x"], ref["x"])
+        assert float(np.abs(res["z"] - ref["z"]).max()) <= 1e-3
+    finally:
+        p.free()
+        trk.free()
