@@ -88,6 +88,27 @@ QTRK_API int QTrkBatchEnableDiagnostics(QTrkBatch* b, bool enable);
 QTRK_API int QTrkBatchDownloadDiagnostics(QTrkBatch* b, int* bestPlane, float* profiles, int n);
 
 /*
+ * No counterpart in the reference.  For callers that cannot allocate GPU memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc):
+ * QTrkBatchRun's localization (one run, the match precision of QTrkBatchSetMatchPrecision
+ * given here) of n ROIs on the caller's GPU buffers -- rois [n][height][width] of pdt,
+ * contiguous; jobs [n]; results [n] -- asynchronously on `stream` (a hipStream_t; NULL: the
+ * default stream).  It allocates nothing: QTRK_MATCH_FP32 needs no scratch, and the
+ * BF16/FP16 grouped match keeps its scratch (radial profiles, grouping by bead) in a
+ * fixed device array of QTRK_NOALLOC_SCRATCH_MB MiB that each GPU the library runs on
+ * holds until the process ends.  A call needing more -- 4*n*zlut_radialsteps + 4*n +
+ * 8*beads + 4 bytes, each part rounded up to 256 -- fails with an error, as does a
+ * BF16/FP16 call before QTrkBatchSetMatchPrecision has converted the tracker's current
+ * LUT for that precision.  Every call on one GPU shares that array: calls that overlap on
+ * one GPU (other streams or threads) race.  Results are as QTrkBatchRun leaves them in
+ * GPU memory, without the Z-bias correction QTrkBatchDownload applies.  0 on success.
+ */
+#define QTRK_NOALLOC_SCRATCH_MB 32
+QTRK_API int QTrkLocalizeDeviceNoAlloc(QueuedTracker* qtrk, const void* rois, QTRK_PixelDataType pdt,
+                                       const LocalizationJob* jobs, LocalizationResult* results, int n,
+                                       int precision, void* stream);
+
+/*
  * Z-bias table: per bead, the z error of the tracker's own localization of
  * its LUT rendered at biasPlanes depths, inverted on every later result
  * (QueuedTracker::ZLUTBiasCorrection, cputrack/QueuedTracker.cpp:115-140).

@@ -196,6 +196,7 @@ SIGNATURES = {
     "QTrkBatchDeviceResults": (_P, [_P]),
     "QTrkBatchRun": (C.c_int, [_P, C.c_int, C.c_int]),
     "QTrkBatchSetMatchPrecision": (C.c_int, [_P, C.c_int]),
+    "QTrkLocalizeDeviceNoAlloc": (C.c_int, [_P, _P, C.c_int, _P, _P, C.c_int, C.c_int, _P]),
     "QTrkBatchSync": (C.c_float, [_P]),
     "QTrkBatchDownload": (C.c_int, [_P, _P, C.c_int]),
     "QTrkBatchEnableDiagnostics": (C.c_int, [_P, C.c_bool]),
@@ -233,6 +234,7 @@ SIGNATURES = {
 }
 
 MATCH_FP32, MATCH_BF16, MATCH_FP16 = 0, 1, 2
+NOALLOC_SCRATCH_MB = 32  # QTRK_NOALLOC_SCRATCH_MB (include/qtrk_amd.h)
 
 _lib = None
 
@@ -509,6 +511,15 @@ class QTrkInstance:
         out = np.zeros(len(rois), RESULT_DTYPE)
         out[order] = res
         return out
+
+
+def localize_device_noalloc(trk, rois_ptr, pdt, jobs_ptr, results_ptr, n, precision=MATCH_FP32, stream=None):
+    """QTrkLocalizeDeviceNoAlloc on GPU buffers given as device addresses (ints); asynchronous on
+    `stream` (a hipStream_t handle as an int, or None for the default stream)."""
+    if lib().QTrkLocalizeDeviceNoAlloc(trk.h, C.c_void_p(rois_ptr), int(pdt), C.c_void_p(jobs_ptr),
+                                       C.c_void_p(results_ptr), int(n), int(precision),
+                                       None if stream is None else C.c_void_p(stream)) != 0:
+        check()
 
 
 class DeviceBatch:
