@@ -40,6 +40,9 @@ void QTrkComputedConfigUpdate(QTrkComputedConfig* cfg);
 QTrkComputedConfig QTrkComputeConfig(const QTrkSettings& s);
 int NearestPowerOf2(int v); /* cputrack/utils.cpp:644-652 */
 
+class CImageData; /* ImageData.h */
+typedef unsigned char uchar;
+
 class QueuedTracker {
 public:
 	QueuedTracker();
@@ -88,6 +91,23 @@ public:
 	virtual bool GetDebugImage(int ID, int* w, int* h, float** pData) { return false; }
 
 	QTrkComputedConfig cfg;
+
+	/* The reference's non-virtual members (cputrack/QueuedTracker.h:38, 87-92), which its
+	 * LabVIEW wrapper calls (cputrack/lv_queuetrk_api.cpp:461, 478, 613):
+	 *  - ScheduleImageData: a float ROI (cputrack/QueuedTracker.cpp:64-67);
+	 *  - ScheduleLocalization with the job's fields as arguments (:69-77; initial may be NULL);
+	 *  - ComputeZBiasCorrection: the table of QTrkComputeZLUTBiasTable, which becomes the active
+	 *    correction, copied into *result (w = bias planes, h = beads) when result is not NULL;
+	 *  - SetZLUTBiasCorrection / GetZLUTBiasCorrection: the table as an image, w = bias planes,
+	 *    h = beads; Get returns a new copy (delete it) or NULL when none is set;
+	 *  - DebugImage: GetDebugImage's image (none here: an empty image). */
+	void ScheduleImageData(ImageData* data, const LocalizationJob* jobInfo);
+	void ScheduleLocalization(uchar* data, int pitch, QTRK_PixelDataType pdt, uint frame, uint timestamp,
+	                          vector3f* initial, uint zlutIndex);
+	void ComputeZBiasCorrection(int bias_planes, CImageData* result, int smpPerPixel, bool useSplineInterp);
+	void SetZLUTBiasCorrection(const CImageData& data);
+	CImageData* GetZLUTBiasCorrection();
+	ImageData DebugImage(int ID);
 
 	/* Z-bias table: w = planes of the table, h = beads
 	 * (cputrack/QueuedTracker.cpp:115-140, 192-203). */

@@ -67,9 +67,19 @@ typedef enum QTRK_PixelDataType QTRK_PixelDataType;
 #define QTrkCUDA_UseBest -1
 
 /* vector2f / vector3f: plain float pairs/triples, 4-byte aligned as under the
- * reference's pack(4) (cputrack/std_incl.h:20-116) */
+ * reference's pack(4) (cputrack/std_incl.h:20-116).  In C++ they carry the
+ * reference's template names, vector2<float> and vector3<float>, so that C++
+ * member functions taking them link under the names that callers built against
+ * the reference's headers use. */
+#ifdef __cplusplus
+template <typename T> struct vector2 { T x, y; };
+template <typename T> struct vector3 { T x, y, z; };
+typedef vector2<float> vector2f;
+typedef vector3<float> vector3f;
+#else
 typedef struct vector2f { float x, y; } vector2f;
 typedef struct vector3f { float x, y, z; } vector3f;
+#endif
 
 #pragma pack(push, 1)
 
@@ -160,10 +170,16 @@ typedef struct RMFrameCounters {
 #pragma pack(pop)
 
 /* ImageData as marshalled by QTrkDotNet/QTrkDLL.cs (pack(4): pointer, w, h)
- * -- cputrack/utils.h:77-138 */
+ * -- cputrack/utils.h:77-138.  In C++ it is the reference's TImageData<float>
+ * (same layout, defined in ImageData.h). */
+#ifdef __cplusplus
+template <typename T> struct TImageData;
+typedef TImageData<float> ImageData;
+#else
 #pragma pack(push, 4)
 typedef struct ImageData { float* data; int w, h; } ImageData;
 #pragma pack(pop)
+#endif
 
 #ifdef __cplusplus
 class QueuedTracker;

@@ -70,5 +70,20 @@ def build_library(verbose=False, jobs=None):
     return LIB
 
 
+CHECK_SRC = os.path.join(ROOT, "tests", "cxx_surface_check.cpp")
+CHECK_BIN = os.path.join(ROOT, "tests", "_bin", "cxx_surface_check")
+
+
+def build_checks(verbose=False):
+    """The C++ check program of the tracker's C++ surface (tests/cxx_surface_check.cpp), linked
+    against the built library; run by the GPU tests."""
+    os.makedirs(os.path.dirname(CHECK_BIN), exist_ok=True)
+    cxx = os.environ.get("CXX", "g++")
+    _run([cxx, "-std=c++17", "-O1", "-I" + os.path.join(ROOT, "include"), CHECK_SRC, "-L" + HERE, "-lqtrk",
+          "-Wl,-rpath,$ORIGIN/../../qtrk_amd", "-o", CHECK_BIN], verbose)
+    return CHECK_BIN
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_checks(verbose=True))

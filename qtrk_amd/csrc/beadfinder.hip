@@ -20,6 +20,7 @@
 // The transforms use the same butterflies, twiddles and leaf order as the
 // localization kernel's (fft_device.hpp), so the correlation map equals the
 // reference's bit for bit, and with it every threshold decision.
+#include "ImageData.h"
 #include "beadfinder.hpp"
 #include "devbuf.hpp"
 #include "errors.hpp"

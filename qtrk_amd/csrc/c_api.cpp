@@ -1,6 +1,7 @@
 // c_api.cpp -- extern "C" boundary of libqtrk.so (include/qtrk_c_api.h,
 // include/qtrk_amd.h).  Every entry point forwards to the tracker inside a
 // guard that turns exceptions into the thread's last error.
+#include "ImageData.h"
 #include "QueuedTracker.h"
 #include "beadfinder.hpp"
 #include "errors.hpp"

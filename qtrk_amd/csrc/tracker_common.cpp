@@ -1,8 +1,10 @@
 // tracker_common.cpp -- implementation-independent parts of the tracker
 // interface: config derivation, frame -> ROI extraction, Z-bias correction,
 // and the utility exports of the reference's utils.h that the C# binding uses.
+#include "ImageData.h"
 #include "QueuedTracker.h"
 #include "errors.hpp"
+#include "hip_tracker.hpp"
 #include "qtrk_amd.h"
 
 #include <algorithm>
@@ -11,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <stdexcept>
 #include <thread>
 #include <vector>
 
@@ -110,6 +113,55 @@ void QueuedTracker::SetZLUTBiasCorrection(const float* table, int tablePlanes, i
 	zbias.assign(table, table + (size_t)tablePlanes * beads);
 	zbias_w = tablePlanes;
 	zbias_h = beads;
+}
+
+// The reference's non-virtual members on image types (cputrack/QueuedTracker.cpp:64-77, 143-203).
+void QueuedTracker::ScheduleImageData(ImageData* data, const LocalizationJob* jobInfo)
+{
+	ScheduleLocalization(data->data, data->pitch(), QTrkFloat, jobInfo);
+}
+
+void QueuedTracker::ScheduleLocalization(uchar* data, int pitch, QTRK_PixelDataType pdt, uint frame, uint timestamp,
+                                         vector3f* initial, uint zlutIndex)
+{
+	LocalizationJob j;
+	memset(&j, 0, sizeof(j));
+	j.frame = frame;
+	j.timestamp = timestamp;
+	if (initial) j.initialPos = *initial;
+	j.zlutIndex = (int)zlutIndex;
+	ScheduleLocalization((void*)data, pitch, pdt, &j);
+}
+
+void QueuedTracker::ComputeZBiasCorrection(int bias_planes, CImageData* result, int smpPerPixel, bool useSplineInterp)
+{
+	auto* t = dynamic_cast<qtrk::QueuedHIPTracker*>(this);
+	if (!t) throw std::runtime_error("ComputeZBiasCorrection: not a MI355X tracker");
+	int count = 0, planes = 0, steps = 0;
+	GetRadialZLUTSize(count, planes, steps);
+	std::vector<float> table((size_t)std::max(count, 0) * std::max(bias_planes, 0));
+	t->ComputeZBiasCorrection(bias_planes, table.data(), smpPerPixel, useSplineInterp);
+	if (result) *result = TImageData<float>(table.data(), bias_planes, count);
+}
+
+void QueuedTracker::SetZLUTBiasCorrection(const CImageData& data)
+{
+	SetZLUTBiasCorrection(data.data, data.w, data.h);
+}
+
+CImageData* QueuedTracker::GetZLUTBiasCorrection()
+{
+	if (zbias.empty()) return nullptr;
+	CImageData* c = new CImageData(zbias_w, zbias_h);
+	std::copy(zbias.begin(), zbias.end(), c->data);
+	return c;
+}
+
+ImageData QueuedTracker::DebugImage(int ID)
+{
+	ImageData img;
+	GetDebugImage(ID, &img.w, &img.h, &img.data);
+	return img;
 }
 
 // ---------------------------------------------------------------------------
