@@ -242,9 +242,6 @@ def precision_bench(a):
     print(json.dumps(res), flush=True)
 
 
-MLE_SETTINGS = dict(width=48, height=48, qi_minradius=1.0, qi_roi_coverage=1.0, qi_radial_coverage=1.4,
-                    qi_angular_coverage=1.5, qi_angstep_factor=1.0, qi_iterations=5, zlut_minradius=0.0,
-                    zlut_roi_coverage=1.0, zlut_radial_coverage=2.0, zlut_angular_coverage=0.7)
 # FLOPs of one model pass per pixel, counted from the fit's pixel model in lutmodel.hip (a fused
 # multiply-add counts 2): per sub-sample 20 (r^2 and its floor 2, reciprocal square root 1, scaled
 # radius 3, clamp 1, fraction 1, value 3, z-slope 3, slope 2, x/y slope 4), ovs^2 = 16 sub-samples,
@@ -271,28 +268,8 @@ def mle_bench(a):
     data = torch.poisson(model.forward(torch.from_numpy(truth)), generator=gen)
     # starting point: the tracker's COM + QI + Z-LUT position, a background estimate, and the
     # intensity from the photons above it over the model's sum at that position
-    st = api.default_settings(**MLE_SETTINGS)
-    trk = api.QTrkInstance(st)
-    trk.set_radial_zlut(synth.normalize_profiles(lut)[None])
-    trk.set_localization_mode(synth.MODE_QI_Z)
-    batch = api.DeviceBatch(trk, n, api.QTrkFloat)
-    batch.upload(data.cpu().numpy(), api.make_jobs(n))
-    batch.run(n)
-    batch.sync()
-    res = batch.download(n)
-    batch.free()
-    trk.free()
     d = data
-    # background: the 10th percentile of the ROI (the rings reach the ROI border, its dark minima do not)
-    bg0 = torch.kthvalue(d.flatten(1), max(1, size * size // 10), dim=1).values.clamp_min(1.0)
-    t0 = torch.zeros((n, 5), device="cuda")
-    t0[:, 0] = torch.from_numpy(res["x"]).cuda()
-    t0[:, 1] = torch.from_numpy(res["y"]).cuda()
-    t0[:, 2] = torch.from_numpy(res["z"]).cuda().clamp(0, 199)
-    t0[:, 3] = 1.0
-    gsum = model.forward(t0).sum((1, 2))
-    t0[:, 3] = ((d.sum((1, 2)) - bg0 * size * size) / gsum).clamp_min(100.0)
-    t0[:, 4] = bg0
+    t0 = synth.mle_start(model, lut, d)
     start_rms = np.sqrt(np.mean((t0[:, :3].cpu().numpy() - truth[:, :3]) ** 2, axis=0))
 
     model.fit(d, t0, iters)  # warm-up

@@ -67,6 +67,49 @@ def model_params(n, rng, size=48, planes=200):
                      rng.uniform(5, 50, n)], axis=1).astype(np.float32)
 
 
+# QTrkSettings of the differentiation workload's tracker (48x48 ROIs): QI 5 iterations, 32 radial
+# steps, 48 Z-LUT radial steps from 0 to 24 px (the model LUT's bins)
+MLE_SETTINGS = dict(width=48, height=48, qi_minradius=1.0, qi_roi_coverage=1.0, qi_radial_coverage=1.4,
+                    qi_angular_coverage=1.5, qi_angstep_factor=1.0, qi_iterations=5, zlut_minradius=0.0,
+                    zlut_roi_coverage=1.0, zlut_radial_coverage=2.0, zlut_angular_coverage=0.7)
+
+
+def mle_start(model, lut, data):
+    """The LUT-model fit's start for photon-count ROIs data [n][size][size] (a torch tensor on the
+    GPU): x, y, z from the tracker's COM + QI + Z-LUT localization (MLE_SETTINGS, the model LUT's
+    normalised profiles), bg from the ROI's 10th percentile (the rings reach the border, their dark
+    minima do not), I from the photons above bg over the model's sum at that position.
+    Returns theta0 [n][5] (torch, GPU)."""
+    import torch
+
+    n, size = data.shape[0], data.shape[1]
+    planes = lut.shape[0]
+    trk = api.QTrkInstance(api.default_settings(**MLE_SETTINGS))
+    try:
+        trk.set_radial_zlut(normalize_profiles(lut)[None])
+        trk.set_localization_mode(MODE_QI_Z)
+        batch = api.DeviceBatch(trk, n, api.QTrkFloat)
+        try:
+            batch.upload(data.cpu().numpy(), api.make_jobs(n))
+            batch.run(n)
+            batch.sync()
+            res = batch.download(n)
+        finally:
+            batch.free()
+    finally:
+        trk.free()
+    bg0 = torch.kthvalue(data.flatten(1), max(1, size * size // 10), dim=1).values.clamp_min(1.0)
+    t0 = torch.zeros((n, 5), device=data.device)
+    t0[:, 0] = torch.from_numpy(res["x"]).to(data.device)
+    t0[:, 1] = torch.from_numpy(res["y"]).to(data.device)
+    t0[:, 2] = torch.from_numpy(res["z"]).to(data.device).clamp(0, planes - 1)
+    t0[:, 3] = 1.0
+    gsum = model.forward(t0).sum((1, 2))
+    t0[:, 3] = ((data.sum((1, 2)) - bg0 * size * size) / gsum).clamp_min(100.0)
+    t0[:, 4] = bg0
+    return t0
+
+
 def scaled_model_lut(planes, steps, size):
     """model_lut's family for a size x size ROI: widths and ring periods scaled by size/48 and the
     family's z range spread over `planes` planes; bins at i * (size/2) / steps."""

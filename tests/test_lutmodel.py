@@ -230,25 +230,33 @@ def test_fit_rejects_more_than_8_sub_samples(lut, gpu):
 
 @pytest.mark.gpu
 def test_fit_precision_matches_crlb(lut, gpu):
-    """On Poisson data the errors divided by the per-ROI CRLB have unit spread (within 10% on 4,096
-    ROIs; the benchmark reports the figure on 65,536) and no significant bias."""
+    """The differentiation workload's precision bar: 16,384 Poisson ROIs of 48x48 (x, y +-3 px, z
+    20-180, I 2,000-20,000, bg 5-50), 10 iterations from the tracker's COM + QI + Z-LUT estimate.
+    The errors divided by each ROI's CRLB have a spread within 5% of 1 for all of x, y, z, I and bg
+    (the sample standard deviation's own standard error is 1/sqrt(2n) = 0.0055 here), and no bias
+    beyond 0.05.  Fits whose start fell outside the likelihood's capture range (an error above 6
+    CRLB in a parameter) are counted, at most 0.2% of the ROIs, and left out of the spread."""
     import torch
     rng = np.random.default_rng(16)
-    n = 4096
+    n = 16384
     th = synth.model_params(n, rng, W)
     m = api.LUTModel(lut, 0.0, MAXR, W, H)
     try:
-        torch.manual_seed(16)
-        data = torch.poisson(m.forward(torch.from_numpy(th)))
-        t0 = th.copy()
-        t0[:, :2] += rng.uniform(-0.3, 0.3, (n, 2)).astype(np.float32)
-        t0[:, 2] += rng.uniform(-1, 1, n).astype(np.float32)
-        fit, _, crlb, _ = m.fit(data, torch.from_numpy(t0), iterations=10)
-        e = (fit.cpu().numpy() - th) / crlb.cpu().numpy()
-        sd = e.std(axis=0)
-        print("std of error / CRLB (x, y, z, I, bg):", np.round(sd, 3), " mean:", np.round(e.mean(axis=0), 3))
-        assert np.all(np.abs(sd - 1) < 0.1), sd
-        assert np.all(np.abs(e.mean(axis=0)) < 0.1)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(16)
+        data = torch.poisson(m.forward(torch.from_numpy(th)), generator=gen)
+        t0 = synth.mle_start(m, lut, data)
+        fit, _, crlb, _ = m.fit(data, t0, iterations=10)
+        fit, crlb = fit.cpu().numpy(), crlb.cpu().numpy()
+        ok = np.all(np.isfinite(crlb), axis=1) & np.all(crlb > 0, axis=1)
+        e = (fit[ok] - th[ok]) / crlb[ok]
+        inl = np.all(np.abs(e) <= 6, axis=1)
+        sd, mu = e[inl].std(axis=0), e[inl].mean(axis=0)
+        print(f"{int((~ok).sum())} without a CRLB, {int((~inl).sum())} beyond 6 CRLB of {n}; "
+              "std of error / CRLB (x, y, z, I, bg):", np.round(sd, 4), " mean:", np.round(mu, 4), flush=True)
+        assert ok.mean() >= 0.999 and inl.mean() >= 0.998
+        assert np.all(np.abs(sd - 1) < 0.05), sd
+        assert np.all(np.abs(mu) < 0.05), mu
     finally:
         m.free()
 
