@@ -13,6 +13,7 @@
 #include <mutex>
 #include <set>
 #include <string>
+#include <vector>
 
 namespace qtrk {
 static thread_local std::string g_last_error;
@@ -272,6 +273,9 @@ struct QTrkBatch {
 	QueuedHIPTracker* trk = nullptr;
 	int maxROIs = 0, pdt = 0;
 	int prec = kMatchFP32;  // QTrkBatchSetMatchPrecision
+	// a host copy of the uploaded jobs, whose largest bead count sizes the grouped match's grid
+	// (cleared once QTrkBatchDeviceJobs hands out the device buffer: the jobs may then change there)
+	std::vector<LocalizationJob> h_jobs;
 	DeviceBuffer rois, jobs, results, best, prof, split;
 	bool diag = false;
 	hipStream_t stream = nullptr, stream2 = nullptr;  // repeats alternate between the two
@@ -323,12 +327,17 @@ QTRK_API int QTrkBatchUpload(QTrkBatch* b, const void* rois, const LocalizationJ
 		const size_t rb = (size_t)b->trk->cfg.width * b->trk->cfg.height * PDT_BytesPerPixel((QTRK_PixelDataType)b->pdt);
 		HIP_CHECK(hipMemcpy(b->rois.p, rois, rb * n, hipMemcpyHostToDevice));
 		HIP_CHECK(hipMemcpy(b->jobs.p, jobs, sizeof(DevJob) * n, hipMemcpyHostToDevice));
+		b->h_jobs.assign(jobs, jobs + n);
 	});
 	return ok ? 0 : -1;
 }
 
 QTRK_API void* QTrkBatchDeviceROIs(QTrkBatch* b) { return b->rois.p; }
-QTRK_API void* QTrkBatchDeviceJobs(QTrkBatch* b) { return b->jobs.p; }
+QTRK_API void* QTrkBatchDeviceJobs(QTrkBatch* b)
+{
+	b->h_jobs.clear();
+	return b->jobs.p;
+}
 QTRK_API void* QTrkBatchDeviceResults(QTrkBatch* b) { return b->results.p; }
 
 QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
@@ -354,10 +363,16 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			b->split.ensure(QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, nb, tab->beads));
 			QueuedHIPTracker::SplitScratch sc = QueuedHIPTracker::SplitScratch::Carve(b->split.p, b->maxROIs, nb, tab->beads);
 			if (b->diag) sc.prof = b->prof.as<float>();
+			// the most ROIs of one bead, from the uploaded jobs (0 when no ROI has a bead in the
+			// LUT); unknown for jobs written on the GPU
+			const int mpb = n <= (int)b->h_jobs.size()
+			                    ? QueuedHIPTracker::MaxPerBead((const DevJob*)b->h_jobs.data(), n, tab->beads)
+			                    : -1;
 			HIP_CHECK(hipEventRecord(b->e0, b->stream));
 			for (int r = 0; r < repeats; r++)
 				b->trk->LocalizeSplit(b->rois.p, b->pdt, b->jobs.as<DevJob>(), b->results.as<DevResult>(), n, b->prec,
-				                      tab, sc, b->diag ? b->best.as<int>() : nullptr, b->stream);
+				                      tab, sc, b->diag ? b->best.as<int>() : nullptr, b->stream, nullptr, 0,
+				                      mpb == 0 ? 1 : mpb);
 			HIP_CHECK(hipEventRecord(b->e1, b->stream));
 			return;
 		}

@@ -473,8 +473,12 @@ void QueuedHIPTracker::Launch(Slot* s)
 		const int nb = cfg.zlut_radialsteps;
 		s->d_split.ensure(SplitScratch::Bytes(batchSize, nb, tab->beads));
 		const SplitScratch sc = SplitScratch::Carve(s->d_split.p, batchSize, nb, tab->beads);
-		LocalizeSplit(P.src, P.pdt, P.jobs, P.results, s->n, matchPrec, tab, sc, nullptr, s->stream, P.roi_offset,
-		              P.pitch);
+		const int mpb = MaxPerBead(s->h_jobs.as<DevJob>(), s->n, tab->beads);
+		if (mpb > 0)
+			LocalizeSplit(P.src, P.pdt, P.jobs, P.results, s->n, matchPrec, tab, sc, nullptr, s->stream, P.roi_offset,
+			              P.pitch, mpb);
+		else
+			HIP_CHECK(launch_localize(P, s->stream));  // no ROI has a bead in the LUT: nothing to match
 	} else {
 		HIP_CHECK(launch_localize(P, s->stream));
 	}
@@ -1117,7 +1121,7 @@ QueuedHIPTracker::SplitScratch QueuedHIPTracker::SplitScratch::Carve(void* base,
 
 void QueuedHIPTracker::LocalizeSplit(const void* rois, int pdt, const DevJob* jobs, DevResult* results, int n,
                                      int prec, const LutMatchTables* tab, const SplitScratch& s, int* best_out,
-                                     hipStream_t stream, const int64_t* roi_offset, int pitch)
+                                     hipStream_t stream, const int64_t* roi_offset, int pitch, int max_per_bead)
 {
 	if (n <= 0) return;
 	LocalizeParams P;
@@ -1140,8 +1144,21 @@ void QueuedHIPTracker::LocalizeSplit(const void* rois, int pdt, const DevJob* jo
 	p.zstride = (int)(sizeof(DevResult) / sizeof(float));
 	p.best = best_out;
 	p.cand = nullptr;
-	// workgroups for the average bead; beads with more profiles loop over their tiles
-	HIP_CHECK(launch_lutmatch(p, prec, std::max(1, (n + beads - 1) / beads), stream, 0));
+	if (max_per_bead > 0)
+		HIP_CHECK(launch_lutmatch(p, prec, max_per_bead, stream, 0));
+	else  // workgroups for the average bead; beads with more profiles loop over their tiles
+		HIP_CHECK(launch_lutmatch(p, prec, std::max(1, (n + beads - 1) / beads), stream, 0, true));
+}
+
+int QueuedHIPTracker::MaxPerBead(const DevJob* jobs, int n, int beads)
+{
+	std::vector<int> c((size_t)std::max(beads, 0), 0);
+	int m = 0;
+	for (int i = 0; i < n; i++) {
+		const int b = jobs[i].zlutIndex;
+		if (b >= 0 && b < beads) m = std::max(m, ++c[b]);
+	}
+	return m;
 }
 
 // ------------------------------------------------------------- config

@@ -117,10 +117,14 @@ public:
 	// `stream`; allocates nothing.  Needs LT_LocalizeZ without LT_ZLUTAlign, LT_LocalizeZWeighted
 	// and LT_FourierLUT (SplitMatchUsable).
 	// roi_offset/pitch: ROI i at rois + roi_offset[i] with rows pitch bytes apart (roi_offset null:
-	// contiguous ROIs, pitch 0: width * bytes per pixel).
+	// contiguous ROIs, pitch 0: width * bytes per pixel).  max_per_bead: the most ROIs of one bead
+	// among the jobs when the caller knows it (> 0), which sizes the match's grid; otherwise (<= 0)
+	// the match's workgroups loop over their bead's tiles.
 	void LocalizeSplit(const void* rois, int pdt, const DevJob* jobs, DevResult* results, int n, int prec,
 	                   const LutMatchTables* tab, const SplitScratch& s, int* best_out, hipStream_t stream,
-	                   const int64_t* roi_offset = nullptr, int pitch = 0);
+	                   const int64_t* roi_offset = nullptr, int pitch = 0, int max_per_bead = 0);
+	// The most jobs of one bead index in [0, beads) among n host jobs (0 when none).
+	static int MaxPerBead(const DevJob* jobs, int n, int beads);
 	// Empty when the current mode and LUT can take the split localization, else the reason.
 	std::string SplitMatchUsable() const;
 	bool HasCalibration() const { return has_off || has_gain; }

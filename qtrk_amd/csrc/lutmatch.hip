@@ -397,9 +397,11 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 	}
 }
 
-// Workgroup (bead, chunk) takes the bead's profile tiles chunk, chunk + chunks, ... (one tile when the
-// grid was sized for the largest bead).
-template <int PREC, int FT, int KSMAX>
+// Workgroup (bead, chunk) takes the bead's profile tile `chunk` (the grid sized for the largest
+// bead) or, with LOOP, the tiles chunk, chunk + chunks, ... (a grid sized without knowing the
+// largest bead).  The loop is a separate instantiation: around the tile it keeps 34 more VGPRs
+// live (143 against 109), which costs a wavefront per SIMD.
+template <int PREC, int FT, int KSMAX, bool LOOP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void lutmatch_mfma_kernel(LutMatchParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -407,6 +409,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
 	xcd_map(blockIdx.x, P.chunks, bead, chunk);
 	if (bead >= P.beads) return;
 	const int count = P.start[bead + 1] - P.start[bead];
+	if (!LOOP) {
+		if (chunk * FT * 32 < count) mfma_chunk<PREC, FT, KSMAX>(P, bead, chunk, smem);
+		return;
+	}
 	for (; chunk * FT * 32 < count; chunk += P.chunks) {
 		mfma_chunk<PREC, FT, KSMAX>(P, bead, chunk, smem);
 		__syncthreads();
@@ -512,7 +518,7 @@ __device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, i
 	}
 }
 
-template <bool WEIGHTED>
+template <bool WEIGHTED, bool LOOP>
 __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -520,6 +526,10 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 	xcd_map(blockIdx.x, P.chunks, bead, chunk);
 	if (bead >= P.beads) return;
 	const int count = P.start[bead + 1] - P.start[bead];
+	if (!LOOP) {
+		if (chunk * kMatchExactTile < count) exact_chunk<WEIGHTED>(P, bead, chunk, smem);
+		return;
+	}
 	for (; chunk * kMatchExactTile < count; chunk += P.chunks) {
 		exact_chunk<WEIGHTED>(P, bead, chunk, smem);
 		__syncthreads();
@@ -635,26 +645,43 @@ hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* fra
 	return hipGetLastError();
 }
 
-template <int PREC, int FT, int KSMAX>
+template <int PREC, int FT, int KSMAX, bool LOOP>
 static hipError_t launch_mfma(const LutMatchParams& q, int grid, hipStream_t stream)
 {
 	const size_t lds = (size_t)mfma_lds(FT, q.res, q.ksteps).total;
-	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_mfma_kernel<PREC, FT, KSMAX>,
+	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_mfma_kernel<PREC, FT, KSMAX, LOOP>,
 	                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 	if (e != hipSuccess) return e;
-	lutmatch_mfma_kernel<PREC, FT, KSMAX><<<grid, 256, lds, stream>>>(q);
+	lutmatch_mfma_kernel<PREC, FT, KSMAX, LOOP><<<grid, 256, lds, stream>>>(q);
 	return hipGetLastError();
 }
 
-template <int PREC>
-static hipError_t launch_mfma_prec(const LutMatchParams& q, int FT, int grid, hipStream_t stream)
+template <int PREC, bool LOOP>
+static hipError_t launch_mfma_loop(const LutMatchParams& q, int FT, int grid, hipStream_t stream)
 {
 	if (q.ksteps <= 8)
-		return FT == 1 ? launch_mfma<PREC, 1, 8>(q, grid, stream) : launch_mfma<PREC, 2, 8>(q, grid, stream);
-	return FT == 1 ? launch_mfma<PREC, 1, 16>(q, grid, stream) : launch_mfma<PREC, 2, 16>(q, grid, stream);
+		return FT == 1 ? launch_mfma<PREC, 1, 8, LOOP>(q, grid, stream) : launch_mfma<PREC, 2, 8, LOOP>(q, grid, stream);
+	return FT == 1 ? launch_mfma<PREC, 1, 16, LOOP>(q, grid, stream) : launch_mfma<PREC, 2, 16, LOOP>(q, grid, stream);
 }
 
-hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream, int frame_tile)
+template <int PREC>
+static hipError_t launch_mfma_prec(const LutMatchParams& q, int FT, int grid, hipStream_t stream, bool loop)
+{
+	return loop ? launch_mfma_loop<PREC, true>(q, FT, grid, stream) : launch_mfma_loop<PREC, false>(q, FT, grid, stream);
+}
+
+template <bool WEIGHTED, bool LOOP>
+static hipError_t launch_exact(const LutMatchParams& q, int grid, size_t lds, hipStream_t stream)
+{
+	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<WEIGHTED, LOOP>,
+	                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+	if (e != hipSuccess) return e;
+	lutmatch_exact_kernel<WEIGHTED, LOOP><<<grid, 256, lds, stream>>>(q);
+	return hipGetLastError();
+}
+
+hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream, int frame_tile,
+                           bool loop_tiles)
 {
 	if (p.beads <= 0 || max_per_bead <= 0) return hipSuccess;
 	LutMatchParams q = p;
@@ -664,26 +691,17 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 		const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
 		const size_t lds = (size_t)exact_lds(p.res, p.planes).total;
 		if (lds > 160 * 1024) return hipErrorInvalidValue;
-		if (p.w) {
-			hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<true>,
-			                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-			if (e != hipSuccess) return e;
-			lutmatch_exact_kernel<true><<<grid, 256, lds, stream>>>(q);
-		} else {
-			hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<false>,
-			                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-			if (e != hipSuccess) return e;
-			lutmatch_exact_kernel<false><<<grid, 256, lds, stream>>>(q);
-		}
-		return hipGetLastError();
+		if (p.w)
+			return loop_tiles ? launch_exact<true, true>(q, grid, lds, stream) : launch_exact<true, false>(q, grid, lds, stream);
+		return loop_tiles ? launch_exact<false, true>(q, grid, lds, stream) : launch_exact<false, false>(q, grid, lds, stream);
 	}
 	if (p.res > kMatchMaxRes) return hipErrorInvalidValue;
 	if (frame_tile != 0 && frame_tile != 32 && frame_tile != 64) return hipErrorInvalidValue;
 	const int FT = frame_tile ? frame_tile / 32 : 1;
 	q.chunks = (max_per_bead + FT * 32 - 1) / (FT * 32);
 	const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
-	if (prec == kMatchBF16) return launch_mfma_prec<kMatchBF16>(q, FT, grid, stream);
-	if (prec == kMatchFP16) return launch_mfma_prec<kMatchFP16>(q, FT, grid, stream);
+	if (prec == kMatchBF16) return launch_mfma_prec<kMatchBF16>(q, FT, grid, stream, loop_tiles);
+	if (prec == kMatchFP16) return launch_mfma_prec<kMatchFP16>(q, FT, grid, stream, loop_tiles);
 	return hipErrorInvalidValue;
 }
 

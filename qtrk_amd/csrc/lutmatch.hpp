@@ -68,10 +68,10 @@ hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* fra
 // LDS of 64 holds more workgroups per CU and was faster on every shape measured,
 // although a bead's LUT tiles are then read once per 32 profiles).
 hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream,
-                           int frame_tile = 0);
-// The launch's grid has ceil(max_per_bead / tile) workgroups per bead; a workgroup also takes the
-// bead's later tiles (every ceil(max_per_bead / tile)-th), so a smaller max_per_bead than the
-// largest bead's count is correct, only slower.
+                           int frame_tile = 0, bool loop_tiles = false);
+// The launch's grid has ceil(max_per_bead / tile) workgroups per bead.  max_per_bead must bound
+// every bead's count unless loop_tiles, where a workgroup also takes the bead's later tiles (every
+// ceil(max_per_bead / tile)-th): then any max_per_bead is correct, a smaller one only slower.
 // Dynamic LDS bytes of that launch.
 size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile = 0);
 
