@@ -170,6 +170,11 @@ def precision_profiles(beads, frames, planes, seed):
     return luts, prof.reshape(beads, frames, 128), z.reshape(beads, frames)
 
 
+# Bytes of one precision-workload match counted without the exact rescoring's FP32 rows: BF16 LUT
+# 131.1 MB + BF16 profiles 16.8 MB + top-8 output 4.2 MB
+MATCH_BYTES_NO_RESCORE = 1024 * 500 * 128 * 2 + 65536 * 128 * 2 + 65536 * 8 * 8
+
+
 def precision_bench(a):
     """Grouped Z-LUT match, 1,024 beads x 500 planes x 128 radial steps, 64 profiles per bead:
     the reference arithmetic over every plane (fp32) against the BF16 and FP16 matrix-core paths."""
@@ -235,7 +240,11 @@ def precision_bench(a):
         "roofline": {"bound": "hbm", "achieved": out["bf16"]["hbm_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(out["bf16"]["hbm_gbs"] / HBM_PEAK_GBS, 4),
                      "traffic": _profile_value(MATCH_TRAFFIC_PROFILE, "traffic_bytes_per_launch"),
-                     "kernel_ms": out["bf16"]["kernel_ms"]},
+                     "kernel_ms": out["bf16"]["kernel_ms"],
+                     # against the bytes without the exact rescoring's FP32 rows
+                     "frac_without_rescore_bytes": round(MATCH_BYTES_NO_RESCORE / (out["bf16"]["kernel_ms"] * 1e-3) / 1e9
+                                                 / HBM_PEAK_GBS, 4),
+                     "mfma_frac_of_dense_bf16_peak": out["bf16"]["mfma_frac_of_dense_peak"]},
         "cpu_baseline": {"value": round(nb * frames / t_cpu, 1), "unit": "profiles/s", "cores": 1, "kind": "port",
                          "sample": f"{nb * frames} profiles through the oracle's LUTProfileCompare + FP64 fit"},
     }

@@ -14,7 +14,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INCLUDE = os.path.join(ROOT, "include")
 REF = "/root/reference"
 
-# Byte offsets of the packed structs (cputrack/qtrk_c_api.h:32-140, SURVEY Appendix B layout)
+# Byte offsets of the packed structs (cputrack/qtrk_c_api.h:32-140)
 LAYOUT = {
     "LocalizationJob": (24, {"frame": 0, "timestamp": 4, "zlutIndex": 8, "initialPos": 12}),
     "LocalizationResult": (52, {"job": 0, "pos": 24, "firstGuess": 36, "error": 44, "imageMean": 48}),
