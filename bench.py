@@ -748,6 +748,25 @@ def localize_bench(a, name):
                                          "achieved": round(valu / kern_s / 1e9, 1),
                                          "peak": VALU_PEAK_WAVE_INSTR / 1e9, "unit": "G wave-instr/s",
                                          "frac": round(valu / kern_s / VALU_PEAK_WAVE_INSTR, 4)}
+    if realtime:
+        # the mixed-precision batched form on the same batch: profile kernel + grouped BF16 match
+        # (QTrkBatchSetMatchPrecision), consecutive batches as above, against the FP32 results
+        b.set_match_precision(api.MATCH_BF16)
+        b.run(n, a.warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(n, a.steps)
+        b.sync()
+        mwall = time.perf_counter() - t0
+        mres = b.download(n)
+        b.set_match_precision(api.MATCH_FP32)
+        same = int(np.sum((mres["x"] == res["x"]) & (mres["y"] == res["y"]) & (mres["z"] == res["z"])))
+        out["mixed_precision"] = {"form": "profile kernel + grouped BF16 matrix-core Z-LUT match, FP32 top-8 "
+                                          "rescoring, FP64 fit",
+                                  "value": round(n * a.steps / mwall, 1), "unit": "localizations/s",
+                                  "ms_per_step": round(mwall / a.steps * 1e3, 4),
+                                  "speedup_vs_fp32": round(wall / mwall, 4),
+                                  "results_identical_to_fp32": same, "rois": n}
     b.free()
     if realtime and n == 4096 and not a.no_frames:
         # the same batch from host memory through the drop-in C ABI (PCIe included): reported

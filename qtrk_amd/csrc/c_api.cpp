@@ -276,7 +276,7 @@ struct QTrkBatch {
 	// a host copy of the uploaded jobs, whose largest bead count sizes the grouped match's grid
 	// (cleared once QTrkBatchDeviceJobs hands out the device buffer: the jobs may then change there)
 	std::vector<LocalizationJob> h_jobs;
-	DeviceBuffer rois, jobs, results, best, prof, split;
+	DeviceBuffer rois, jobs, results, best, prof, split, split2;
 	bool diag = false;
 	hipStream_t stream = nullptr, stream2 = nullptr;  // repeats alternate between the two
 	hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
@@ -354,25 +354,37 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			P.prof_out = b->prof.as<float>();
 		}
 		if (b->prec != kMatchFP32) {
-			// profile kernel + grouped match; the repeats run one after another on the batch
-			// stream, as they share the scratch
+			// profile kernel + grouped match; repeats alternate between the two streams as below,
+			// each stream with its own scratch (diagnostics: one stream, their buffers)
 			const std::string why = b->trk->SplitMatchUsable();
 			if (!why.empty()) fail("the grouped Z-LUT match cannot run: " + why);
 			const LutMatchTables* tab = b->trk->PrepareMatchTables(b->prec);
 			const int nb = b->trk->cfg.zlut_radialsteps;
-			b->split.ensure(QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, nb, tab->beads));
-			QueuedHIPTracker::SplitScratch sc = QueuedHIPTracker::SplitScratch::Carve(b->split.p, b->maxROIs, nb, tab->beads);
-			if (b->diag) sc.prof = b->prof.as<float>();
+			const size_t bytes = QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, nb, tab->beads);
+			const bool two = repeats > 1 && !b->diag;
+			b->split.ensure(bytes);
+			if (two) b->split2.ensure(bytes);
+			QueuedHIPTracker::SplitScratch sc[2] = {
+				QueuedHIPTracker::SplitScratch::Carve(b->split.p, b->maxROIs, nb, tab->beads),
+				QueuedHIPTracker::SplitScratch::Carve(two ? b->split2.p : b->split.p, b->maxROIs, nb, tab->beads) };
+			if (b->diag) sc[0].prof = b->prof.as<float>();
 			// the most ROIs of one bead, from the uploaded jobs (0 when no ROI has a bead in the
 			// LUT); unknown for jobs written on the GPU
 			const int mpb = n <= (int)b->h_jobs.size()
 			                    ? QueuedHIPTracker::MaxPerBead((const DevJob*)b->h_jobs.data(), n, tab->beads)
 			                    : -1;
 			HIP_CHECK(hipEventRecord(b->e0, b->stream));
-			for (int r = 0; r < repeats; r++)
+			if (two) HIP_CHECK(hipStreamWaitEvent(b->stream2, b->e0, 0));
+			for (int r = 0; r < repeats; r++) {
+				const int k = two ? (r & 1) : 0;
 				b->trk->LocalizeSplit(b->rois.p, b->pdt, b->jobs.as<DevJob>(), b->results.as<DevResult>(), n, b->prec,
-				                      tab, sc, b->diag ? b->best.as<int>() : nullptr, b->stream, nullptr, 0,
-				                      mpb == 0 ? 1 : mpb);
+				                      tab, sc[k], b->diag ? b->best.as<int>() : nullptr, k ? b->stream2 : b->stream,
+				                      nullptr, 0, mpb == 0 ? 1 : mpb);
+			}
+			if (two) {
+				HIP_CHECK(hipEventRecord(b->e2, b->stream2));
+				HIP_CHECK(hipStreamWaitEvent(b->stream, b->e2, 0));
+			}
 			HIP_CHECK(hipEventRecord(b->e1, b->stream));
 			return;
 		}

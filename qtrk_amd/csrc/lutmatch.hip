@@ -764,10 +764,61 @@ __global__ void group_scatter_kernel(const int* bead_idx, int stride, int n, int
 	if (b >= 0 && b < beads) rowidx[atomicAdd(&cursor[b], 1)] = i;
 }
 
+// The same grouping in one workgroup, the counts in LDS: one launch instead of four, for batches of
+// up to kGroupSmallRows rows and kGroupSmallBeads beads.
+constexpr int kGroupSmallRows = 65536, kGroupSmallBeads = 16384;
+
+__global__ __launch_bounds__(1024) void group_small_kernel(const int* bead_idx, int stride, int n, int beads,
+                                                           int* start, int* rowidx)
+{
+	extern __shared__ int cnt[];  // [beads], then [1024] partial sums
+	int* part = cnt + beads;
+	const int tid = threadIdx.x;
+	for (int b = tid; b < beads; b += 1024) cnt[b] = 0;
+	__syncthreads();
+	for (int i = tid; i < n; i += 1024) {
+		const int b = bead_idx[(size_t)i * stride];
+		if (b >= 0 && b < beads) atomicAdd(&cnt[b], 1);
+	}
+	__syncthreads();
+	const int per = (beads + 1023) / 1024, b0 = tid * per;
+	int sum = 0;
+	for (int b = b0; b < b0 + per && b < beads; b++) sum += cnt[b];
+	part[tid] = sum;
+	__syncthreads();
+	for (int off = 1; off < 1024; off <<= 1) {
+		const int v = tid >= off ? part[tid - off] : 0;
+		__syncthreads();
+		part[tid] += v;
+		__syncthreads();
+	}
+	int run = tid ? part[tid - 1] : 0;
+	for (int b = b0; b < b0 + per && b < beads; b++) {
+		const int c = cnt[b];
+		start[b] = run;
+		cnt[b] = run;
+		run += c;
+	}
+	if (tid == 1023) start[beads] = part[1023];
+	__syncthreads();
+	for (int i = tid; i < n; i += 1024) {
+		const int b = bead_idx[(size_t)i * stride];
+		if (b >= 0 && b < beads) rowidx[atomicAdd(&cnt[b], 1)] = i;
+	}
+}
+
 hipError_t launch_group_by_bead(const int* bead_idx, int stride, int n, int beads, int* counts, int* start,
                                 int* rowidx, hipStream_t stream)
 {
 	if (beads <= 0) return hipSuccess;
+	if (n <= kGroupSmallRows && beads <= kGroupSmallBeads) {
+		const size_t lds = ((size_t)beads + 1024) * sizeof(int);
+		hipError_t e = hipFuncSetAttribute((const void*)group_small_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+		                                   (int)lds);
+		if (e != hipSuccess) return e;
+		group_small_kernel<<<1, 1024, lds, stream>>>(bead_idx, stride, n, beads, start, rowidx);
+		return hipGetLastError();
+	}
 	group_zero_kernel<<<grid_for((size_t)beads), 256, 0, stream>>>(counts, beads);
 	if (n > 0) group_count_kernel<<<(n + 255) / 256, 256, 0, stream>>>(bead_idx, stride, n, beads, counts);
 	group_scan_kernel<<<1, 1024, 0, stream>>>(counts, beads, start);
