@@ -772,7 +772,7 @@ def localize_bench(a, name):
         # the same batch from host memory through the drop-in C ABI (PCIe included): reported
         # beside the device-resident value, never as it
         frames, pos, jobs = _camera_frames(w)
-        rates, _ = frame_path_rates(trk, frames, pos, jobs, 5, 1)
+        rates, _ = frame_path_rates(trk, frames, pos, jobs, 10, 3)
         out["pcie_inclusive"] = dict({"path": "QTrkScheduleFrame: 16 host frames of 1600x1600 u16, 256 ROIs each",
                                       "unit": "localizations/s"}, **rates)
     trk.free()
