@@ -35,6 +35,13 @@ QTRK_API void QTrkGetConfigValues(QueuedTracker* qtrk, char* dst, int maxStrLen)
 QTRK_API int QTrkSetImageZLUT(QueuedTracker* qtrk, float* src, float* radial_zlut, int* dims);
 QTRK_API void QTrkGetImageZLUT(QueuedTracker* qtrk, float* dst);
 QTRK_API void QTrkGetImageZLUTSize(QueuedTracker* qtrk, int* dims);
+/* The image LUT's first and second differences across planes that QTrkFinalizeLUT computes, as the
+ * reference's FinalizeLUT does into its image_lut_dz / image_lut_dz2 (cputrack/QueuedCPUTracker.cpp:
+ * 661-696), which it exposes through no interface: dz = (U - L) / 2 and dz2 = U - 2 C + L of the
+ * planes below, at and above, the first and last planes copied from their neighbours; each
+ * [beads][planes][height][width] (either may be NULL).  1 when present, 0 when not (no image LUT
+ * of at least 3 planes at QTrkFinalizeLUT, or set again since), -1 on error. */
+QTRK_API int QTrkGetImageZLUTDerivatives(QueuedTracker* qtrk, float* dz, float* dz2);
 
 /* Name, compute units and clock of the GPU the library runs on. */
 QTRK_API int QTrkGetDeviceInfo(char* dst, int maxStrLen);

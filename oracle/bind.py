@@ -79,6 +79,7 @@ class Oracle:
             "oq_find_beads": (C.c_int, [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, fl, fl, P, C.c_int]),
             "oq_fourier_radial_profile": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int]),
             "oq_finalize_lut": (None, [P, C.c_int, C.c_int]),
+            "oq_image_lut_diff": (None, [P, C.c_int, C.c_int, C.c_int, P, P]),
             "oq_bspline_weights": (None, [P, fl]),
             "oq_generate_image_from_lut": (None, [P, C.c_int, C.c_int, P, C.c_int, C.c_int, fl, fl, fl, fl, fl,
                                                   C.c_int, C.c_int]),
@@ -378,6 +379,14 @@ class Oracle:
         _, planes, ih, iw = image_lut.shape
         self.L.oq_build_image_lut(C.byref(cfg), _p(images), pdt, len(images), int(plane),
                                   None if kp is None else _p(kp), None, None, _p(image_lut), planes, ih, iw)
+
+    def image_lut_diff(self, lut):
+        """FinalizeLUT's (dz, dz2) of an image LUT [beads][planes][h][w] (float32)."""
+        lut = np.ascontiguousarray(lut, np.float32)
+        dz, dz2 = np.zeros_like(lut), np.zeros_like(lut)
+        b, p = lut.shape[:2]
+        self.L.oq_image_lut_diff(_p(lut), b, p, int(np.prod(lut.shape[2:])), _p(dz), _p(dz2))
+        return dz, dz2
 
     def finalize_lut(self, zlut):
         self.L.oq_finalize_lut(_p(zlut), zlut.shape[0] * zlut.shape[1], zlut.shape[2])

@@ -1838,3 +1838,32 @@ void oq_com_qi_img(const OConfig* cfg, const fl* img, fl bgcorr, int window, fl*
 	free(wnd);
 	oq_worker_free(&wk);
 }
+
+/* ------------------------------------------------- image LUT differences */
+
+/* QueuedCPUTracker::FinalizeLUT's image LUT differences (cputrack/QueuedCPUTracker.cpp:661-696):
+ * for 0 < z < planes-1, dz = 0.5f * (U - L) / h and dz2 = (U - 2*C + L) / (h*h) with h = 1.0f,
+ * then the first and last planes copied from planes 1 and planes-2.  lut, dz, dz2 are
+ * [beads][planes][npx]; planes >= 3. */
+void oq_image_lut_diff(const float* lut, int beads, int planes, int npx, float* dz, float* dz2)
+{
+	for (int i = 0; i < beads; i++) {
+		const size_t b = (size_t)i * planes * npx;
+		for (int z = 1; z < planes - 1; z++) {
+			const float* img = lut + b + (size_t)npx * z;
+			const float* imgL = lut + b + (size_t)npx * (z - 1);
+			const float* imgU = lut + b + (size_t)npx * (z + 1);
+			for (int k = 0; k < npx; k++) {
+				const float h = 1.0f;
+				dz[b + (size_t)npx * z + k] = 0.5f * (imgU[k] - imgL[k]) / h;
+				dz2[b + (size_t)npx * z + k] = (imgU[k] - 2 * img[k] + imgL[k]) / (h * h);
+			}
+		}
+		for (int k = 0; k < npx; k++) {
+			dz[b + k] = dz[b + (size_t)npx + k];
+			dz[b + (size_t)npx * (planes - 1) + k] = dz[b + (size_t)npx * (planes - 2) + k];
+			dz2[b + k] = dz2[b + (size_t)npx + k];
+			dz2[b + (size_t)npx * (planes - 1) + k] = dz2[b + (size_t)npx * (planes - 2) + k];
+		}
+	}
+}

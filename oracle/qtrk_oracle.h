@@ -160,6 +160,8 @@ int oq_find_beads(const fl* img, int w, int h, int sx, int sy, int roi, fl img_d
                   int max_out);
 void oq_fourier_radial_profile(fl* img, int w, int h, fl* dst, int nbins, int nang);
 void oq_finalize_lut(float* zlut, int rows, int res);
+/* FinalizeLUT's image LUT first and second plane differences (cputrack/QueuedCPUTracker.cpp:661-696) */
+void oq_image_lut_diff(const float* lut, int beads, int planes, int npx, float* dz, float* dz2);
 
 /* Image model and noise-free image generators (cputrack/utils.cpp:325-402, CubicBSpline.h:15-24). */
 void oq_bspline_weights(fl* w, fl t);

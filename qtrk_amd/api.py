@@ -156,6 +156,7 @@ SIGNATURES = {
     "QTrkSetImageZLUT": (C.c_int, [_P, _FP, _FP, _IP]),
     "QTrkGetImageZLUT": (None, [_P, _FP]),
     "QTrkGetImageZLUTSize": (None, [_P, _IP]),
+    "QTrkGetImageZLUTDerivatives": (C.c_int, [_P, _P, _P]),
     "QTrkFindBeads": (_P, [C.POINTER(ImageData), C.c_int, C.c_int, C.c_int, C.c_float, C.c_float, _IP,
                            C.POINTER(ImageData)]),
     "ComputeRadialProfile": (None, [_FP, C.c_int, C.c_int, C.c_float, C.c_float, Vector2f,
@@ -361,6 +362,15 @@ class QTrkInstance:
         if out.size:
             self._call("QTrkGetImageZLUT", _fptr(out))
         return out
+
+    def get_image_zlut_derivatives(self):
+        """(dz, dz2) of the image LUT after FinalizeLUT ([beads][planes][h][w] each), or None."""
+        dims = np.zeros(4, np.int32)
+        lib().QTrkGetImageZLUTSize(self.h, dims.ctypes.data_as(_IP))
+        dz = np.zeros(tuple(int(d) for d in dims), np.float32)
+        dz2 = np.zeros_like(dz)
+        r = self._call("QTrkGetImageZLUTDerivatives", _ptr(dz), _ptr(dz2))
+        return (dz, dz2) if r == 1 else None
 
     def get_radial_zlut_size(self):
         c, p, r = C.c_int(), C.c_int(), C.c_int()

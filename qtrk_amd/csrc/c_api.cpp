@@ -232,6 +232,16 @@ QTRK_API int QTrkSetImageZLUT(QueuedTracker* q, float* src, float* radial_zlut, 
 }
 QTRK_API void QTrkGetImageZLUT(QueuedTracker* q, float* dst) { GUARD(q->GetImageZLUT(dst)); }
 QTRK_API void QTrkGetImageZLUTSize(QueuedTracker* q, int* dims) { GUARD(q->GetImageZLUTSize(dims)); }
+QTRK_API int QTrkGetImageZLUTDerivatives(QueuedTracker* q, float* dz, float* dz2)
+{
+	int r = -1;
+	GUARD({
+		auto* t = dynamic_cast<QueuedHIPTracker*>(q);
+		if (!t) fail("not a MI355X tracker");
+		r = t->GetImageZLUTDerivatives(dz, dz2) ? 1 : 0;
+	});
+	return r;
+}
 
 QTRK_API void QTrkSetConfigValue(QueuedTracker* q, const char* name, const char* value)
 {

@@ -1026,6 +1026,7 @@ bool QueuedHIPTracker::SetImageZLUT(float* src, float* radial_zlut, int* dims)
 		WaitAllLocked(lk);
 		const size_t n = (size_t)dims[0] * dims[1] * dims[2] * dims[3];
 		d_image_lut.release();
+		ilut_diff = false;
 		for (int i = 0; i < 4; i++) ilut_dims[i] = dims[i];
 		if (n) {
 			d_image_lut.ensure(n * 4);
@@ -1037,14 +1038,37 @@ bool QueuedHIPTracker::SetImageZLUT(float* src, float* radial_zlut, int* dims)
 	return true;
 }
 
-// Normalizes every plane profile (cputrack/QueuedCPUTracker.cpp:648-659).  The reference
-// also computes the image LUT's first and second differences across planes there
-// (:661-696); no interface reads them back, so they are not computed.
+bool QueuedHIPTracker::GetImageZLUTDerivatives(float* dz, float* dz2)
+{
+	std::unique_lock<std::mutex> lk(mtx);
+	HIP_CHECK(hipSetDevice(dev));
+	WaitAllLocked(lk);
+	if (!ilut_diff) return false;
+	const size_t n = (size_t)ilut_dims[0] * ilut_dims[1] * ilut_dims[2] * ilut_dims[3];
+	if (dz) HIP_CHECK(hipMemcpy(dz, d_image_lut_dz.p, n * 4, hipMemcpyDeviceToHost));
+	if (dz2) HIP_CHECK(hipMemcpy(dz2, d_image_lut_dz2.p, n * 4, hipMemcpyDeviceToHost));
+	return true;
+}
+
+// Normalizes every plane profile (cputrack/QueuedCPUTracker.cpp:648-659) and computes the image
+// LUT's first and second differences across planes (:661-696; read back with
+// GetImageZLUTDerivatives).
 void QueuedHIPTracker::FinalizeLUT()
 {
 	std::unique_lock<std::mutex> lk(mtx);
 	HIP_CHECK(hipSetDevice(dev));
 	WaitAllLocked(lk);
+	const size_t npx = (size_t)ilut_dims[2] * ilut_dims[3];
+	ilut_diff = false;
+	if (npx > 0 && ilut_dims[0] > 0 && ilut_dims[1] >= 3 && d_image_lut.p) {
+		const size_t n = (size_t)ilut_dims[0] * ilut_dims[1] * npx;
+		d_image_lut_dz.ensure(n * 4);
+		d_image_lut_dz2.ensure(n * 4);
+		HIP_CHECK(launch_image_lut_diff(d_image_lut.as<float>(), d_image_lut_dz.as<float>(),
+		                                d_image_lut_dz2.as<float>(), ilut_dims[0], ilut_dims[1], (int)npx, aux_stream()));
+		HIP_CHECK(hipStreamSynchronize(aux_stream()));
+		ilut_diff = true;
+	}
 	if (zcount <= 0) return;
 	HIP_CHECK(launch_normalize_rows(d_zlut.as<float>(), zcount * zplanes, cfg.zlut_radialsteps, aux_stream()));
 	HIP_CHECK(hipStreamSynchronize(aux_stream()));

@@ -61,6 +61,10 @@ public:
 	void GetImageZLUTSize(int* dims) override;
 	void GetImageZLUT(float* dst) override;
 	bool SetImageZLUT(float* src, float* radial_zlut, int* dims) override;
+	// The image LUT's first and second differences across planes that FinalizeLUT computed
+	// ([beads][planes][height][width] each; either pointer may be null); false when there are
+	// none (no image LUT of at least 3 planes when FinalizeLUT ran, or it was set again since).
+	bool GetImageZLUTDerivatives(float* dz, float* dz2);
 	int GetResultCount() override;
 	int FetchResults(LocalizationResult* results, int maxResults) override;
 	int GetQueueLength(int* maxQueueLen = 0) override;
@@ -196,6 +200,9 @@ private:
 	DeviceBuffer d_build_rois, d_build_pos;
 	// the image LUT, [beads][planes][height][width] as ilut_dims (QueuedTracker.h's order)
 	DeviceBuffer d_image_lut;
+	// its first and second differences across planes, made by FinalizeLUT
+	DeviceBuffer d_image_lut_dz, d_image_lut_dz2;
+	bool ilut_diff = false;
 	int ilut_dims[4] = { 0, 0, 0, 0 };
 	PinnedBuffer h_build_rois;
 

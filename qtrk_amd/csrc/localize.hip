@@ -1743,6 +1743,38 @@ __global__ void normalize_rows_kernel(float* data, int rows, int len)
 	for (int i = 0; i < len; i++) p[i] = (float)((double)p[i] * inv);
 }
 
+// The image LUT's first and second differences across planes (cputrack/QueuedCPUTracker.cpp:
+// 661-696): plane z of bead b, pixel k, for 0 < z < planes - 1: dz = 0.5 (U - L) / 1 and
+// dz2 = (U - 2 C + L) / (1 * 1) with L, C, U the planes z - 1, z, z + 1; the first and last planes
+// take those of their neighbour.  One thread per (bead, pixel), planes in order.
+__global__ void image_lut_diff_kernel(const float* lut, float* dz, float* dz2, int beads, int planes, int npx)
+{
+	const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (g >= (size_t)beads * npx) return;
+	const size_t b = g / npx, k = g - b * npx;
+	const size_t base = b * planes * (size_t)npx + k;
+	for (int z = 1; z < planes - 1; z++) {
+		const float L = lut[base + (size_t)(z - 1) * npx], C = lut[base + (size_t)z * npx],
+		            U = lut[base + (size_t)(z + 1) * npx];
+		dz[base + (size_t)z * npx] = 0.5f * (U - L) / 1.0f;
+		dz2[base + (size_t)z * npx] = (U - 2 * C + L) / (1.0f * 1.0f);
+	}
+	dz[base] = dz[base + npx];
+	dz2[base] = dz2[base + npx];
+	dz[base + (size_t)(planes - 1) * npx] = dz[base + (size_t)(planes - 2) * npx];
+	dz2[base + (size_t)(planes - 1) * npx] = dz2[base + (size_t)(planes - 2) * npx];
+}
+
+hipError_t launch_image_lut_diff(const float* lut, float* dz, float* dz2, int beads, int planes, int npx,
+                                 hipStream_t stream)
+{
+	const size_t n = (size_t)beads * npx;
+	if (n == 0 || planes < 3) return hipSuccess;
+	hipLaunchKernelGGL(image_lut_diff_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, lut, dz, dz2,
+	                   beads, planes, npx);
+	return hipGetLastError();
+}
+
 hipError_t launch_normalize_rows(float* data, int rows, int len, hipStream_t stream)
 {
 	if (rows <= 0 || len <= 0) return hipSuccess;

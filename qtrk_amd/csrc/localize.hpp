@@ -125,5 +125,8 @@ size_t localize_lds_bytes(const LocalizeParams& p);
 hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream);
 // Normalizes `rows` profiles of `len` values in place (FinalizeLUT).
 hipError_t launch_normalize_rows(float* data, int rows, int len, hipStream_t stream);
+// The image LUT's plane differences of FinalizeLUT ([beads][planes][npx] each; planes >= 3).
+hipError_t launch_image_lut_diff(const float* lut, float* dz, float* dz2, int beads, int planes, int npx,
+                                 hipStream_t stream);
 
 }  // namespace qtrk

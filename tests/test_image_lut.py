@@ -107,3 +107,30 @@ def test_image_lut_set_get_and_checks(gpu):
     assert trk.set_image_zlut((0, 0, 0, 0), None)
     assert trk.get_image_zlut().size == 0
     trk.free()
+
+
+@pytest.mark.gpu
+def test_image_lut_derivatives_after_finalize(gpu, oracle):
+    """QTrkFinalizeLUT computes the image LUT's first and second differences across planes, as the
+    reference's FinalizeLUT does (cputrack/QueuedCPUTracker.cpp:661-696): equal to the oracle's bit
+    for bit; none before FinalizeLUT, none after the image LUT is set again, none for fewer than 3
+    planes."""
+    w = synth.make_workload("plain64", n=4)
+    rng = np.random.default_rng(9)
+    lut = rng.normal(0, 100, (3, 7, 9, 11)).astype(np.float32)
+    trk = api.QTrkInstance(w["settings"])
+    try:
+        trk.set_image_zlut(lut)
+        assert trk.get_image_zlut_derivatives() is None
+        trk.finalize_lut()
+        got = trk.get_image_zlut_derivatives()
+        assert got is not None
+        ref = oracle.image_lut_diff(lut)
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+        trk.set_image_zlut(lut)
+        assert trk.get_image_zlut_derivatives() is None
+        trk.set_image_zlut(lut[:, :2])
+        trk.finalize_lut()
+        assert trk.get_image_zlut_derivatives() is None
+    finally:
+        trk.free()

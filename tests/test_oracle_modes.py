@@ -286,3 +286,20 @@ def test_fourier_lut_localizes_z_independent_of_position(oracle):
     print("Fourier-LUT z", r["z"])
     assert abs(r["z"][0] - 20.3) < 0.05 and abs(r["z"][2] - 41.7) < 0.05
     assert abs(r["z"][1] - r["z"][0]) < 2e-3 and abs(r["z"][3] - r["z"][2]) < 2e-3
+
+
+def test_image_lut_differences_closed_form(oracle):
+    """FinalizeLUT's image LUT differences (cputrack/QueuedCPUTracker.cpp:661-696) on planes that are
+    quadratic in z, v = a + b z + c z^2 per pixel: the central difference is b + 2 c z and the second
+    difference 2 c (exact here: small integers), the first and last planes equal their neighbours'."""
+    rng = np.random.default_rng(7)
+    beads, planes, h, w = 2, 6, 3, 4
+    a, b, c = (rng.integers(-20, 20, (beads, 1, h, w)).astype(np.float32) for _ in range(3))
+    z = np.arange(planes, dtype=np.float32)[None, :, None, None]
+    lut = (a + b * z + c * z * z).astype(np.float32)
+    dz, dz2 = oracle.image_lut_diff(lut)
+    zi = z[:, 1:-1]
+    assert np.array_equal(dz[:, 1:-1], np.broadcast_to(b + 2 * c * zi, dz[:, 1:-1].shape))
+    assert np.array_equal(dz2[:, 1:-1], np.broadcast_to(2 * c, dz2[:, 1:-1].shape))
+    assert np.array_equal(dz[:, 0], dz[:, 1]) and np.array_equal(dz[:, -1], dz[:, -2])
+    assert np.array_equal(dz2[:, 0], dz2[:, 1]) and np.array_equal(dz2[:, -1], dz2[:, -2])
