@@ -567,7 +567,13 @@ def frame_path_rates(trk, frames, pos, jobs, steps, warmup):
 
     for _ in range(max(1, warmup)):
         one_pass()
+    # the first streamed passes of a process run at about half the steady host-to-device rate
+    # (measured: ~20 GB/s for the first 20 passes, then 40-47 GB/s), so the warm-up streams for at
+    # least one second as well as `warmup` passes
+    t0 = time.perf_counter()
     stream(max(1, warmup))
+    while time.perf_counter() - t0 < 1.0:
+        stream(4)
     t0 = time.perf_counter()
     got = stream(steps)
     wall = time.perf_counter() - t0

@@ -242,10 +242,12 @@ void QueuedHIPTracker::BuildTables()
 
 void QueuedHIPTracker::AllocSlots()
 {
+	if (!copyStream) HIP_CHECK(hipStreamCreateWithFlags(&copyStream, hipStreamNonBlocking));
 	for (int i = 0; i < numSlots; i++) {
 		Slot* s = new Slot();
 		HIP_CHECK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
 		HIP_CHECK(hipEventCreate(&s->ev_start));
+		HIP_CHECK(hipEventCreateWithFlags(&s->ev_h2d, hipEventDisableTiming));
 		HIP_CHECK(hipEventCreate(&s->ev_k0));
 		HIP_CHECK(hipEventCreate(&s->ev_k1));
 		// blocking synchronisation: a long wait on a full queue sleeps instead of spinning
@@ -266,6 +268,7 @@ void QueuedHIPTracker::FreeSlots()
 	for (Slot* s : slots) {
 		if (s->stream) hipStreamDestroy(s->stream);
 		hipEventDestroy(s->ev_start);
+		hipEventDestroy(s->ev_h2d);
 		hipEventDestroy(s->ev_k0);
 		hipEventDestroy(s->ev_k1);
 		hipEventDestroy(s->ev_done);
@@ -273,6 +276,8 @@ void QueuedHIPTracker::FreeSlots()
 	}
 	slots.clear();
 	filling = nullptr;
+	if (copyStream) hipStreamDestroy(copyStream);
+	copyStream = nullptr;
 }
 
 void QueuedHIPTracker::Warn(const std::string& w)
@@ -454,7 +459,9 @@ void QueuedHIPTracker::Launch(Slot* s)
 	const size_t rb = (size_t)cfg.width * cfg.height * bpp;
 	LocalizeParams P;
 	if (s->frame) {
-		// the row blocks are on their way already (ScheduleFrame), ev_start recorded before the first
+		// the row blocks are on their way already on the copy stream (ScheduleFrame), ev_start
+		// recorded before the first; the kernel waits for the last
+		HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_h2d, 0));
 		HIP_CHECK(hipMemcpyAsync(s->d_off.p, s->h_off.p, sizeof(int64_t) * s->n, hipMemcpyHostToDevice, s->stream));
 		HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
 		FillParams(P, s->pdt, s->n, s->d_rois.p, s->d_off.as<int64_t>(), s->frame_pitch, s->d_jobs.as<DevJob>(),
@@ -567,12 +574,15 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 			s->h_off.ensure(sizeof(int64_t) * (size_t)batchSize);
 			s->d_off.ensure(sizeof(int64_t) * (size_t)batchSize);
 		}
-		if (s->n == 0) HIP_CHECK(hipEventRecord(s->ev_start, s->stream));
+		if (s->n == 0) HIP_CHECK(hipEventRecord(s->ev_start, copyStream));
 		copier->Copy(s->h_rois.as<uint8_t>() + at, img + (size_t)pitch * y0, bytes);
-		// each block goes to the GPU as soon as it is in pinned memory; the kernel follows on
-		// the same stream when the slot launches
+		// each block goes to the GPU as soon as it is in pinned memory, in schedule order on the
+		// one copy stream; the slot's kernel waits for its last block (ev_h2d) when it launches.
+		// The slot is taken again only after that kernel, so the pinned and device blocks are
+		// not overwritten while a copy or the kernel still reads them.
 		HIP_CHECK(hipMemcpyAsync(s->d_rois.as<uint8_t>() + at, s->h_rois.as<uint8_t>() + at, bytes,
-		                         hipMemcpyHostToDevice, s->stream));
+		                         hipMemcpyHostToDevice, copyStream));
+		HIP_CHECK(hipEventRecord(s->ev_h2d, copyStream));
 		int64_t* off = s->h_off.as<int64_t>() + s->n;
 		DevJob* jobs = s->h_jobs.as<DevJob>() + s->n;
 		for (size_t k = c0; k < c1; k++) {

@@ -149,6 +149,7 @@ private:
 		DeviceBuffer d_off;
 		hipStream_t stream = nullptr;
 		hipEvent_t ev_start = nullptr, ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
+		hipEvent_t ev_h2d = nullptr;  // after the slot's last frame block on the copy stream
 		DeviceBuffer d_split;  // LocalizeSplit's scratch, when match_precision is not fp32
 	};
 	// the grouped-match tables for a launch on `stream` (made or remade there when stale); the
@@ -215,6 +216,9 @@ private:
 	std::unique_ptr<CopyPool> copier;  // made on the first frame
 
 	int matchPrec = kMatchFP32;  // "match_precision": how queued batches match the Z-LUT
+	// frame blocks (ScheduleFrame) go to the GPU one after another on this stream, so a slot's
+	// kernel starts as soon as its own blocks are over rather than when every slot's copies are
+	hipStream_t copyStream = nullptr;
 	uint64_t lutVersion = 0;
 	std::unique_ptr<LutMatchTables> matchTab;
 	uint64_t matchTabVersion = 0;
