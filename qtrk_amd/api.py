@@ -219,6 +219,7 @@ SIGNATURES = {
     "QTrkLUTModelBackward": (C.c_int, [_P, _P, _P, C.c_int, _P, _P, _P]),
     "QTrkLUTModelFitMLE": (C.c_int, [_P, _P, _P, _P, C.c_int, C.c_int, _P, _P, _P, _P, _P]),
     "QTrkLUTModelSync": (C.c_int, [_P, _P]),
+    "QTrkLUTModelSetFitLanes": (C.c_int, [_P, C.c_int]),
     "QTrkPipelineCreate": (_P, []),
     "QTrkPipelineFree": (None, [_P]),
     "QTrkPipelineAddTracker": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _FP, C.c_int, C.c_int, C.c_int,
@@ -746,6 +747,11 @@ class LUTModel:
                                     _dptr(fisher), _dptr(crlb), _dptr(nll), self._stream()):
             check()
         return th, fisher, crlb, nll
+
+    def set_fit_lanes(self, lanes):
+        """Lanes per ROI of fit(): 64, 128, 256 (default), 512 (models of <= 6x6 sub-samples), 1024 (<= 3x3)."""
+        if lib().QTrkLUTModelSetFitLanes(self.h, int(lanes)):
+            check()
 
     def render(self, theta, lut_index=None):
         """mu as a differentiable function of theta (torch autograd through the adjoint kernel)."""

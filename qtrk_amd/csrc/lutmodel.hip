@@ -15,13 +15,14 @@
 
 namespace qtrk {
 
-constexpr int kModelBlock = 256;  // lanes per ROI of the forward/backward passes; the fit takes 64-256
+constexpr int kModelBlock = 256;  // lanes per ROI of the forward/backward passes; the fit takes 64-1024
 
 struct ModelLds {
 	int zi, dzi, row, data, red, th, total;
 };
 __host__ __device__ inline int m16(int x) { return (x + 15) & ~15; }
-__host__ __device__ inline ModelLds model_lds(const LutModelParams& P, bool fit)
+// `waves`: wavefronts per workgroup whose partial sums block_sum_d keeps (at least 4)
+__host__ __device__ inline ModelLds model_lds(const LutModelParams& P, bool fit, int waves = 4)
 {
 	ModelLds L;
 	int o = 0;
@@ -29,12 +30,15 @@ __host__ __device__ inline ModelLds model_lds(const LutModelParams& P, bool fit)
 	L.dzi = o;   o += m16(P.steps * 4);
 	L.row = o;   o += fit ? m16(P.steps * 16) : 0;
 	L.data = o;  o += fit ? m16(P.width * P.height * 4) : 0;
-	L.red = o;   o += m16(5 * 24 * 8);  // 4 wavefront partials + the totals
+	L.red = o;   o += m16(((waves > 4 ? waves : 4) + 1) * 24 * 8);  // the totals + one row per wavefront
 	L.th = o;    o += m16(8 * 4);
 	L.total = o;
 	return L;
 }
-size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit) { return (size_t)model_lds(p, fit).total; }
+size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit, int block)
+{
+	return (size_t)model_lds(p, fit, block / 64).total;
+}
 
 // zi[r] and dzi[r] for depth z: the cubic B-spline blend of planes iz-1..iz+2
 // and its derivative d/dz (or the linear blend of planes iz, iz+1 and its slope).
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_forward_kernel(LutModelP
 }
 
 // Sum of K per-lane float partials over the workgroup in double; the result is
-// in red[0..K) for every lane after the call.
+// in red[0..K) for every lane after the call (red[24 + 24 w ..] hold wavefront w's sums).
 template <int K>
 __device__ __forceinline__ void block_sum_d(const float (&v)[K], double* red)
 {
@@ -280,12 +284,12 @@ __device__ __forceinline__ void block_sum_d(const float (&v)[K], double* red)
 	__syncthreads();  // red may still be read from the previous call
 	if (lane == 0)
 #pragma unroll
-		for (int k = 0; k < K; k++) red[wid * 24 + k] = d[k];
+		for (int k = 0; k < K; k++) red[24 + wid * 24 + k] = d[k];
 	__syncthreads();
 	if (threadIdx.x < K) {
 		double s = 0;
-		for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[w * 24 + threadIdx.x];
-		red[96 + threadIdx.x] = s;
+		for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += red[24 + w * 24 + threadIdx.x];
+		red[threadIdx.x] = s;
 	}
 	__syncthreads();
 }
@@ -320,7 +324,7 @@ __global__ __launch_bounds__(kModelBlock) void lutmodel_backward_kernel(LutModel
 		acc[4] += gm;
 	}
 	block_sum_d<kModelParams>(acc, red);
-	if (threadIdx.x < kModelParams) P.grad[(size_t)roi * kModelParams + threadIdx.x] = (float)red[96 + threadIdx.x];
+	if (threadIdx.x < kModelParams) P.grad[(size_t)roi * kModelParams + threadIdx.x] = (float)red[threadIdx.x];
 }
 
 // ------------------------------------------------------------ maximum likelihood
@@ -378,13 +382,15 @@ __host__ __device__ constexpr int tri5(int i, int j) { return i * 5 - i * (i - 1
 // turn non-positive are halved instead; z stays within the LUT.  After the
 // last update one more pass evaluates F, its FP64 inverse (the CRLB) and the
 // negative log-likelihood at the fitted parameters.
-template <int OVS>
-__global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
+// MAXB: the largest workgroup the instance is launched with (256, 512 or 1024; a workgroup of 1024
+// lanes puts 4 wavefronts on each SIMD, which leaves 128 VGPRs per lane).
+template <int OVS, int MAXB>
+__global__ __launch_bounds__(MAXB) void lutmodel_fit_kernel(LutModelParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
 	const int roi = blockIdx.x;
 	const int nt = blockDim.x;
-	const ModelLds L = model_lds(P, true);
+	const ModelLds L = model_lds(P, true, nt >> 6);
 	float* zi = (float*)(smem + L.zi);
 	float* dzi = (float*)(smem + L.dzi);
 	float4* row = (float4*)(smem + L.row);
@@ -404,7 +410,7 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;
 	// pixel p = y * width + x stepped by the block size without a division per pixel
-	// (nt <= 256, so one wrap per step needs dxp < width: dxp = nt mod width)
+	// (dxp = nt mod width < width, so x wraps at most once per step)
 	const int y1 = (int)threadIdx.x / P.width, x1 = (int)threadIdx.x - y1 * P.width;
 	const int dyp = nt / P.width, dxp = nt - dyp * P.width;
 	__syncthreads();
@@ -428,7 +434,7 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		}
 		block_sum_d<4>(v, red);
 		if (threadIdx.x == 0) {
-			const double sg = red[96], sgg = red[97], sgd = red[98], sd = red[99], N = (double)npx;
+			const double sg = red[0], sgg = red[1], sgd = red[2], sd = red[3], N = (double)npx;
 			const double det = sgg * N - sg * sg;
 			double I = det > 0 ? (N * sgd - sg * sd) / det : 0.0;
 			double bg = det > 0 ? (sgg * sd - sg * sgd) / det : sd / N;
@@ -471,7 +477,7 @@ __global__ __launch_bounds__(256) void lutmodel_fit_kernel(LutModelParams P)
 		}
 		block_sum_d<21>(v, red);
 		if (threadIdx.x == 0) {
-			const double* s = red + 96;
+			const double* s = red;
 			double F[5][5], g[5];
 			for (int i = 0; i < 5; i++) {
 				g[i] = -s[i];
@@ -591,31 +597,55 @@ hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream)
 	}
 }
 
-template <int OVS>
+template <int OVS, int MAXB>
 static hipError_t launch_fit(const LutModelParams& p, size_t lds, hipStream_t stream, int block)
 {
 	static_assert(OVS >= 1 && OVS <= kMaxFitOvs, "sub-sample count");
-	hipError_t e = set_lds((const void*)lutmodel_fit_kernel<OVS>, lds);
+	hipError_t e = set_lds((const void*)lutmodel_fit_kernel<OVS, MAXB>, lds);
 	if (e != hipSuccess) return e;
-	lutmodel_fit_kernel<OVS><<<p.n, block, lds, stream>>>(p);
+	lutmodel_fit_kernel<OVS, MAXB><<<p.n, block, lds, stream>>>(p);
 	return hipGetLastError();
+}
+
+template <int MAXB>
+static hipError_t launch_fit_ovs(const LutModelParams& p, size_t lds, hipStream_t stream, int block)
+{
+	switch (p.ovs) {
+	case 1: return launch_fit<1, MAXB>(p, lds, stream, block);
+	case 2: return launch_fit<2, MAXB>(p, lds, stream, block);
+	case 3: return launch_fit<3, MAXB>(p, lds, stream, block);
+	case 4: return launch_fit<4, MAXB>(p, lds, stream, block);
+	case 5: return launch_fit<5, MAXB>(p, lds, stream, block);
+	default: return launch_fit<6, MAXB>(p, lds, stream, block);
+	}
+}
+
+bool lutmodel_fit_block_ok(int block, int ovs)
+{
+	if (block == 64 || block == 128 || block == 256) return true;
+	// 8 wavefronts per workgroup leave 256 VGPRs per lane, 16 leave 128: the 1024-lane instances
+	// of more than 3 x 3 sub-samples would spill
+	return (block == 512 && ovs <= kMaxWideFitOvs) || (block == 1024 && ovs <= kMax1024FitOvs);
 }
 
 hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int block)
 {
 	if (p.n <= 0) return hipSuccess;
-	if (block != 64 && block != 128 && block != 256) return hipErrorInvalidValue;
 	if (p.ovs < 1 || p.ovs > kMaxFitOvs || p.steps < 2) return hipErrorInvalidValue;
-	const size_t lds = lutmodel_lds_bytes(p, true);
+	if (!lutmodel_fit_block_ok(block, p.ovs)) return hipErrorInvalidValue;
+	const size_t lds = lutmodel_lds_bytes(p, true, block);
+	if (block == 1024) {
+		switch (p.ovs) {
+		case 1: return launch_fit<1, 1024>(p, lds, stream, block);
+		case 2: return launch_fit<2, 1024>(p, lds, stream, block);
+		default: return launch_fit<3, 1024>(p, lds, stream, block);
+		}
+	}
+	if (block == 512) return launch_fit_ovs<512>(p, lds, stream, block);
 	switch (p.ovs) {
-	case 1: return launch_fit<1>(p, lds, stream, block);
-	case 2: return launch_fit<2>(p, lds, stream, block);
-	case 3: return launch_fit<3>(p, lds, stream, block);
-	case 4: return launch_fit<4>(p, lds, stream, block);
-	case 5: return launch_fit<5>(p, lds, stream, block);
-	case 6: return launch_fit<6>(p, lds, stream, block);
-	case 7: return launch_fit<7>(p, lds, stream, block);
-	default: return launch_fit<8>(p, lds, stream, block);
+	case 7: return launch_fit<7, 256>(p, lds, stream, block);
+	case 8: return launch_fit<8, 256>(p, lds, stream, block);
+	default: return launch_fit_ovs<256>(p, lds, stream, block);
 	}
 }
 

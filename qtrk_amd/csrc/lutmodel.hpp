@@ -23,6 +23,8 @@ namespace qtrk {
 
 constexpr int kModelParams = 5;  // x, y, z, I, bg
 constexpr int kMaxFitOvs = 8;    // sub-samples per pixel side the fit supports (forward/backward: any)
+constexpr int kMaxWideFitOvs = 6;  // ... with 512 lanes per ROI
+constexpr int kMax1024FitOvs = 3;  // ... with 1024 lanes per ROI
 
 struct LutModelParams {
 	const float* luts;       // [count][planes][steps]
@@ -53,7 +55,10 @@ struct LutModelParams {
 hipError_t launch_lutmodel_forward(const LutModelParams& p, hipStream_t stream);
 hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream);
 // block: lanes per ROI (64, 128 or 256)
+// block: lanes per ROI, 64, 128, 256, 512 (up to kMaxWideFitOvs sub-samples per side) or 1024 (up to
+// kMax1024FitOvs)
 hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int block = 256);
-size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit);
+bool lutmodel_fit_block_ok(int block, int ovs);
+size_t lutmodel_lds_bytes(const LutModelParams& p, bool fit, int block = 256);
 
 }  // namespace qtrk

@@ -5,6 +5,7 @@
 #include "lutmodel.hpp"
 #include "qtrk_amd.h"
 
+#include <algorithm>
 #include <memory>
 
 using namespace qtrk;
@@ -14,6 +15,7 @@ struct QTrkLUTModel {
 	LutModelParams base = {};
 	DeviceBuffer luts;
 	hipStream_t own = nullptr;  // uploads only
+	int fit_lanes = 256;        // lanes per ROI of FitMLE (QTrkLUTModelSetFitLanes)
 
 	// NULL is the device's default (null) stream, as in every HIP call
 	static hipStream_t stream(void* s) { return (hipStream_t)s; }
@@ -30,7 +32,7 @@ struct QTrkLUTModel {
 	void check(int n) const
 	{
 		if (n < 0) fail("n must be >= 0");
-		if (lutmodel_lds_bytes(base, true) > 160 * 1024) fail("ROI too large for the fit (LDS)");
+		if (lutmodel_lds_bytes(base, true, fit_lanes) > 160 * 1024) fail("ROI too large for the fit (LDS)");
 	}
 };
 
@@ -120,7 +122,19 @@ QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float*
 		p.fisher = fisher;
 		p.crlb = crlb;
 		p.nll = nll;
-		HIP_CHECK(launch_lutmodel_fit(p, m->stream(stream)));
+		HIP_CHECK(launch_lutmodel_fit(p, m->stream(stream), m->fit_lanes));
+	}) ? 0 : -1;
+}
+
+QTRK_API int QTrkLUTModelSetFitLanes(QTrkLUTModel* m, int lanes)
+{
+	return GUARD({
+		if (!m) fail("null argument");
+		if (!lutmodel_fit_block_ok(lanes, std::min(m->base.ovs, kMaxFitOvs)))
+			fail("lanes per ROI must be 64, 128, 256, 512 (at most " + std::to_string(kMaxWideFitOvs) +
+			     " sub-samples per pixel side) or 1024 (at most " + std::to_string(kMax1024FitOvs) + ")");
+		if (lutmodel_lds_bytes(m->base, true, lanes) > 160 * 1024) fail("ROI too large for the fit (LDS)");
+		m->fit_lanes = lanes;
 	}) ? 0 : -1;
 }
 

@@ -274,7 +274,8 @@ void Pipeline::tune(PipelineTracker& t)
 	if (t.mle) {
 		best = 1e30f;
 		int pb = 256;
-		for (int v : { 64, 128, 256 }) {
+		for (int v : { 64, 128, 256, 512, 1024 }) {
+			if (!lutmodel_fit_block_ok(v, t.model.ovs) || lutmodel_lds_bytes(t.model, true, v) > 160 * 1024) continue;
 			t.mle_block = v;
 			const float ms = time_on(s0, e0, e1, 1, [&] { enqueue_mle(t, s0); });
 			os << " mle[block " << v << "] " << ms << " ms;";
@@ -283,8 +284,9 @@ void Pipeline::tune(PipelineTracker& t)
 		t.mle_block = pb;
 	}
 	HIP_CHECK(hipStreamSynchronize(s0));
+	t.chain_ms = time_on(s0, e0, e1, 1, [&] { enqueue_chain(t, s0); });
 	std::ostringstream ch;
-	ch << "chosen: lds " << (t.lds_pix == 2 ? "f32" : t.lds_pix == 1 ? "u16" : "u8") << ", match tile "
+	ch << "chain " << t.chain_ms << " ms; chosen: lds " << (t.lds_pix == 2 ? "f32" : t.lds_pix == 1 ? "u16" : "u8") << ", match tile "
 	   << t.frame_tile << ", mle block " << t.mle_block << ";" << os.str();
 	t.tuning = ch.str();
 }
@@ -310,9 +312,19 @@ void Pipeline::Finalize(bool autotune)
 			t->lds_pix = 2;
 			t->frame_tile = 0;
 			t->mle_block = 256;
+			t->chain_ms = 0;
 			t->tuning = "defaults";
 		}
 	}
+	// The graph's kernels are submitted one node after another in capture order, some microseconds
+	// each, so a branch captured last starts that much later.  Capture stage by stage (every
+	// tracker's localization, then every match, then every MLE), the trackers with the longest
+	// chain first: each branch starts within the first few submissions and the longest (the
+	// critical path of a batch) first of all.
+	std::vector<PipelineTracker*> order;
+	for (auto& t : ts) order.push_back(t.get());
+	std::stable_sort(order.begin(), order.end(),
+	                 [](const PipelineTracker* a, const PipelineTracker* b) { return a->chain_ms > b->chain_ms; });
 	// one untimed pass sets every kernel's LDS attribute before capture
 	for (auto& t : ts) enqueue_chain(*t, s0);
 	HIP_CHECK(hipStreamSynchronize(s0));
@@ -320,9 +332,11 @@ void Pipeline::Finalize(bool autotune)
 	HIP_CHECK(hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed));
 	try {
 		HIP_CHECK(hipEventRecord(e_fork, s0));
-		for (auto& t : ts) {
-			HIP_CHECK(hipStreamWaitEvent(t->stream, e_fork, 0));
-			enqueue_chain(*t, t->stream);
+		for (PipelineTracker* t : order) HIP_CHECK(hipStreamWaitEvent(t->stream, e_fork, 0));
+		for (PipelineTracker* t : order) enqueue_localize(*t, t->stream);
+		for (PipelineTracker* t : order) enqueue_match(*t, t->stream);
+		for (PipelineTracker* t : order) enqueue_mle(*t, t->stream);
+		for (PipelineTracker* t : order) {
 			HIP_CHECK(hipEventRecord(t->done, t->stream));
 			HIP_CHECK(hipStreamWaitEvent(s0, t->done, 0));
 		}

@@ -47,6 +47,7 @@ struct PipelineTracker {
 	LutModelParams model = {};
 	// launch shapes (autotuned)
 	int lds_pix = 2, frame_tile = 0, mle_block = 256;
+	float chain_ms = 0;  // the tuned chain's time alone on the GPU (0 when not autotuned)
 	hipStream_t stream = nullptr;
 	hipEvent_t done = nullptr;
 	std::string tuning;

@@ -216,6 +216,63 @@ def test_fit_noise_free_other_sub_sampling(lut, gpu, ovs):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("size,ovs", [(48, 4), (100, 3)])
+def test_fit_lanes_per_roi_agree(lut, gpu, size, ovs):
+    """The fit with 64 to 1024 lanes per ROI (QTrkLUTModelSetFitLanes): the lanes only split the pixel
+    sums differently, so on Poisson data every lane count lands on the 256-lane fit to float rounding,
+    from a given start and from the least-squares start (I <= 0)."""
+    import torch
+    rng = np.random.default_rng(77 + size)
+    n = 48
+    th = synth.model_params(n, rng, size)
+    m = api.LUTModel(lut, 0.0, MAXR * size / W, size, size, ovs=ovs)
+    try:
+        mu = m.forward(torch.from_numpy(th)).cpu().numpy()
+        data = torch.from_numpy(rng.poisson(mu).astype(np.float32))
+        t0 = th.copy()
+        t0[:, :2] += rng.uniform(-0.3, 0.3, (n, 2)).astype(np.float32)
+        t0[:, 2] += rng.uniform(-1.0, 1.0, n).astype(np.float32)
+        t0[::3, 3] = -1.0
+        out = {}
+        wide = (512, 1024) if ovs <= 3 else (512,)
+        for lanes in (64, 256) + wide:
+            m.set_fit_lanes(lanes)
+            fit, _, crlb, nll = m.fit(data, torch.from_numpy(t0), iterations=8)
+            out[lanes] = (fit.cpu().numpy(), crlb.cpu().numpy(), nll.cpu().numpy())
+        ref = out[256]
+        for lanes, (fit, crlb, nll) in out.items():
+            d = np.abs(fit - ref[0])
+            print(f"{size}px lanes {lanes}: max |dx| {d[:, 0].max():.2e} |dz| {d[:, 2].max():.2e} "
+                  f"rel dI {np.max(d[:, 3] / ref[0][:, 3]):.2e}")
+            assert np.max(d[:, :2]) < 1e-3
+            assert np.max(d[:, 2]) < 1e-3
+            assert np.max(d[:, 3] / ref[0][:, 3]) < 1e-3
+            np.testing.assert_allclose(crlb, ref[1], rtol=1e-3)
+            np.testing.assert_allclose(nll, ref[2], rtol=1e-5)
+        # the widest fit still recovers noise-free parameters
+        m.set_fit_lanes(wide[-1])
+        clean = m.forward(torch.from_numpy(th))
+        fit, _, _, _ = m.fit(clean, torch.from_numpy(t0), iterations=10)
+        fit = fit.cpu().numpy()
+        assert np.max(np.abs(fit[:, :2] - th[:, :2])) < 1e-3
+        assert np.max(np.abs(fit[:, 2] - th[:, 2])) < 1e-2
+    finally:
+        m.free()
+
+
+@pytest.mark.gpu
+def test_fit_lanes_rejects_unsupported(lut, gpu):
+    for ovs, bad, good in ((7, 512, 256), (4, 1024, 512), (4, 300, 128)):
+        m = api.LUTModel(lut, 0.0, MAXR, W, H, ovs=ovs)
+        try:
+            with pytest.raises(api.QTrkError, match="lanes per ROI"):
+                m.set_fit_lanes(bad)
+            m.set_fit_lanes(good)
+        finally:
+            m.free()
+
+
+@pytest.mark.gpu
 def test_fit_rejects_more_than_8_sub_samples(lut, gpu):
     import torch
     th = synth.model_params(4, np.random.default_rng(3), W)
