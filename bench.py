@@ -618,6 +618,46 @@ def frame_path_rates(trk, frames, pos, jobs, steps, warmup):
             "host_bytes_per_pass": int(frames.nbytes)}, res
 
 
+def c_abi_speed_test(w, mode, passes):
+    """The reference's speed-test clock through the C ABI from a C program (qtrk_amd/speed_test.c):
+    every ROI of the workload scheduled from host memory by one call (mode "frame1": QTrkScheduleFrame
+    with the ROI as a one-ROI frame, cudatrack-test/test.cu:343-347; "roi": QTrkScheduleLocalization),
+    then Flush and wait until the result count is N.  Returns (dict, results ordered by frame, bead)."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    from qtrk_amd import api
+    from qtrk_amd.build import SPEED_BIN
+
+    if not os.path.exists(SPEED_BIN):
+        raise RuntimeError(f"{SPEED_BIN} is not built (qtrk_amd/build.py build_speed_test)")
+    n, h, wd = w["rois"].shape
+    d = tempfile.mkdtemp(prefix="qtrk_speed_")
+    try:
+        luts = np.ascontiguousarray(w["luts"], np.float32)
+        with open(os.path.join(d, "params.txt"), "w") as f:
+            f.write(f"{n} {wd} {h} {luts.shape[0]} {luts.shape[1]} {luts.shape[2]} {int(w['mode'])}\n")
+        with open(os.path.join(d, "settings.bin"), "wb") as f:
+            f.write(bytes(w["settings"]))
+        luts.tofile(os.path.join(d, "lut.bin"))
+        np.ascontiguousarray(w["rois"], np.uint16).tofile(os.path.join(d, "rois.bin"))
+        np.ascontiguousarray(w["jobs"]).tofile(os.path.join(d, "jobs.bin"))
+        r = subprocess.run([SPEED_BIN, d, mode, str(passes)], capture_output=True, text=True, timeout=300)
+        if r.returncode != 0:
+            raise RuntimeError(f"qtrk_speed_test failed ({r.returncode}): {r.stderr[-2000:]}")
+        out = json.loads(r.stdout.strip().splitlines()[-1])
+        res = np.fromfile(os.path.join(d, "results.bin"), api.RESULT_DTYPE)
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    rates = out["locs_per_s"]
+    return {"call": {"frame1": "QTrkScheduleFrame, one ROI per call (the reference's SpeedTest)",
+                     "roi": "QTrkScheduleLocalization per ROI"}[mode],
+            "locs_per_s_median": round(float(np.median(rates)), 1),
+            "locs_per_s_min_max": [round(min(rates), 1), round(max(rates), 1)], "passes": passes,
+            "schedule_call_us_mean": out["schedule_call_us_mean"]}, res[np.lexsort((res["zlutIndex"], res["frame"]))]
+
+
 def queue_bench(a):
     """The realtime100 ROIs through the drop-in queue API from host memory: 16 camera frames of
     1,600 x 1,600 u16 per step, each holding a 16 x 16 grid of the batch's 100 x 100 ROIs (one per
@@ -640,6 +680,13 @@ def queue_bench(a):
     trk.free()
     res = res[np.lexsort((res["zlutIndex"], res["frame"]))]
     same = bool(np.array_equal(res["x"], ref["x"]) and np.array_equal(res["z"], ref["z"]))
+    # one ROI per call from a C client, the reference's own speed-test pattern
+    per_roi = {}
+    for mode in ("frame1", "roi"):
+        st, sres = c_abi_speed_test(w, mode, max(3, a.steps // 4))
+        st["results_equal_device_batch"] = bool(np.array_equal(sres["x"], ref["x"]) and np.array_equal(sres["z"], ref["z"]))
+        per_roi[mode] = st
+    rates["one_roi_per_call_c_client"] = per_roi
     value = rates["stream_locs_per_s"]
     out = {"metric": "localizations/sec through QTrkScheduleFrame from host frames (PCIe included)",
            "value": value, "unit": "localizations/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -809,6 +856,11 @@ def localize_bench(a, name):
         out["pcie_inclusive"] = dict({"path": "QTrkScheduleFrame: 16 host frames of 1600x1600 u16, 256 ROIs each",
                                       "unit": "localizations/s"}, **rates)
     trk.free()
+    if realtime and n == 4096 and not a.no_frames:
+        # the reference's speed-test pattern from a C client: one QTrkScheduleFrame call per ROI
+        st, sres = c_abi_speed_test(w, "frame1", 5)
+        st["results_equal_device_batch"] = bool(np.array_equal(sres["x"], res["x"]) and np.array_equal(sres["z"], res["z"]))
+        out["pcie_inclusive"]["one_roi_per_call_c_client"] = st
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w)
     print(json.dumps(out), flush=True)

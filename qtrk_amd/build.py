@@ -72,6 +72,8 @@ def build_library(verbose=False, jobs=None):
 
 CHECK_SRC = os.path.join(ROOT, "tests", "cxx_surface_check.cpp")
 CHECK_BIN = os.path.join(ROOT, "tests", "_bin", "cxx_surface_check")
+SPEED_SRC = os.path.join(HERE, "speed_test.c")
+SPEED_BIN = os.path.join(HERE, "_bin", "qtrk_speed_test")
 
 
 def build_checks(verbose=False):
@@ -84,6 +86,17 @@ def build_checks(verbose=False):
     return CHECK_BIN
 
 
+def build_speed_test(verbose=False):
+    """qtrk_speed_test (speed_test.c): the reference's speed-test clock through the C ABI, a plain C
+    client of the built library; run by bench.py."""
+    os.makedirs(os.path.dirname(SPEED_BIN), exist_ok=True)
+    cc = os.environ.get("CC", "gcc")
+    _run([cc, "-std=c11", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"), SPEED_SRC, "-L" + HERE, "-lqtrk",
+          "-Wl,-rpath,$ORIGIN/..", "-o", SPEED_BIN], verbose)
+    return SPEED_BIN
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
     print(build_checks(verbose=True))
+    print(build_speed_test(verbose=True))

@@ -16,6 +16,10 @@ namespace qtrk {
 // alone is larger
 static constexpr size_t kFrameBlockAlign = 256;
 static constexpr size_t kFrameSlotMaxBytes = (size_t)64 << 20;
+// a slot's row blocks go to the GPU once this many bytes are waiting (or at launch): a camera
+// frame's block goes at once, while one-ROI frames (the reference's speed test schedules every ROI
+// as one) share a copy instead of paying a copy call and an event each
+static constexpr size_t kFrameSendBytes = (size_t)1 << 20;
 
 void DeviceBuffer::ensure(size_t n)
 {
@@ -460,7 +464,8 @@ void QueuedHIPTracker::Launch(Slot* s)
 	LocalizeParams P;
 	if (s->frame) {
 		// the row blocks are on their way already on the copy stream (ScheduleFrame), ev_start
-		// recorded before the first; the kernel waits for the last
+		// recorded before the first; the rest go now, and the kernel waits for the last
+		SendFrameBlocks(s);
 		HIP_CHECK(hipStreamWaitEvent(s->stream, s->ev_h2d, 0));
 		HIP_CHECK(hipMemcpyAsync(s->d_off.p, s->h_off.p, sizeof(int64_t) * s->n, hipMemcpyHostToDevice, s->stream));
 		HIP_CHECK(hipMemcpyAsync(s->d_jobs.p, s->h_jobs.p, sizeof(DevJob) * s->n, hipMemcpyHostToDevice, s->stream));
@@ -494,6 +499,19 @@ void QueuedHIPTracker::Launch(Slot* s)
 	HIP_CHECK(hipEventRecord(s->ev_done, s->stream));
 	s->seq = ++seqCounter;
 	s->state = Slot::Launched;
+}
+
+// The slot's row blocks not yet sent go to the GPU in one copy, in schedule order on the one copy
+// stream; ev_h2d follows it, and the slot's kernel waits for ev_h2d when it launches.  The slot is
+// taken again only after that kernel, so the pinned and device blocks are not overwritten while a
+// copy or the kernel still reads them.
+void QueuedHIPTracker::SendFrameBlocks(Slot* s)
+{
+	if (s->frame_sent >= s->frame_bytes) return;
+	HIP_CHECK(hipMemcpyAsync(s->d_rois.as<uint8_t>() + s->frame_sent, s->h_rois.as<uint8_t>() + s->frame_sent,
+	                         s->frame_bytes - s->frame_sent, hipMemcpyHostToDevice, copyStream));
+	HIP_CHECK(hipEventRecord(s->ev_h2d, copyStream));
+	s->frame_sent = s->frame_bytes;
 }
 
 // QueuedTracker::ScheduleFrame (cputrack/QueuedTracker.cpp:91-111) with the
@@ -548,6 +566,7 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 			filling->pdt = (int)pdt;
 			filling->frame_pitch = pitch;
 			filling->frame_bytes = 0;
+			filling->frame_sent = 0;
 		}
 		Slot* s = filling;
 		const size_t c1 = std::min(sel.size(), c0 + (size_t)(batchSize - s->n));
@@ -576,13 +595,8 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 		}
 		if (s->n == 0) HIP_CHECK(hipEventRecord(s->ev_start, copyStream));
 		copier->Copy(s->h_rois.as<uint8_t>() + at, img + (size_t)pitch * y0, bytes);
-		// each block goes to the GPU as soon as it is in pinned memory, in schedule order on the
-		// one copy stream; the slot's kernel waits for its last block (ev_h2d) when it launches.
-		// The slot is taken again only after that kernel, so the pinned and device blocks are
-		// not overwritten while a copy or the kernel still reads them.
-		HIP_CHECK(hipMemcpyAsync(s->d_rois.as<uint8_t>() + at, s->h_rois.as<uint8_t>() + at, bytes,
-		                         hipMemcpyHostToDevice, copyStream));
-		HIP_CHECK(hipEventRecord(s->ev_h2d, copyStream));
+		s->frame_bytes = at + bytes;
+		if (s->frame_bytes - s->frame_sent >= kFrameSendBytes) SendFrameBlocks(s);
 		int64_t* off = s->h_off.as<int64_t>() + s->n;
 		DevJob* jobs = s->h_jobs.as<DevJob>() + s->n;
 		for (size_t k = c0; k < c1; k++) {
@@ -593,7 +607,6 @@ int QueuedHIPTracker::ScheduleFrame(void* imgptr, int pitch, int width, int heig
 			memcpy(&jobs[k - c0], &job, sizeof(DevJob));
 		}
 		s->n += (int)(c1 - c0);
-		s->frame_bytes = at + bytes;
 		if (s->n >= batchSize) {
 			Launch(s);
 			filling = nullptr;

@@ -145,6 +145,7 @@ private:
 		bool frame = false;
 		int frame_pitch = 0;
 		size_t frame_bytes = 0;
+		size_t frame_sent = 0;  // bytes of the row blocks already sent to the GPU
 		PinnedBuffer h_off;
 		DeviceBuffer d_off;
 		hipStream_t stream = nullptr;
@@ -158,6 +159,7 @@ private:
 
 	void BuildTables();
 	void AllocSlots();
+	void SendFrameBlocks(Slot* s);
 	void FreeSlots();
 	Slot* TakeFreeSlot();
 	void WaitEventLocked(std::unique_lock<std::mutex>& lk, hipEvent_t ev);

@@ -555,6 +555,33 @@ def test_schedule_frame_packs_frames_into_batches(gpu, oracle):
     _assert_parity(r, ref)
 
 
+def test_schedule_frame_one_roi_frames(gpu, oracle):
+    """The reference's speed-test pattern (cudatrack-test/test.cu:343-347): every ROI is a frame of
+    its own, scheduled by one call.  Small row blocks go to the GPU together once 1 MiB is waiting
+    or when the slot launches: with batch_size 256, the first slot's 256 blocks of 20 KB go in
+    several copies and the second slot's 44 only at Flush.  Every result matches the oracle."""
+    w = synth.make_workload("realtime100", n=300)
+    src = w["rois"]
+    luts = np.concatenate([w["luts"]] * 2)
+    trk = api.QTrkInstance(w["settings"])
+    trk.set_config_value("batch_size", "256")
+    trk.set_radial_zlut(luts)
+    trk.set_localization_mode(w["mode"])
+    calls = []
+    for k in range(len(src)):
+        job = api.make_jobs(1, frame=np.array([k // 16]), zlut_index=np.array([k]))[0]
+        frame = np.ascontiguousarray(src[k])
+        assert trk.schedule_frame(frame, [(0, 0)], job) == 1
+        calls.append((frame, [(0, 0)], job))
+    trk.wait_for(len(src))
+    r = _by_index(trk.fetch_results(len(src)))
+    trk.free()
+    assert sorted(r["zlutIndex"]) == list(range(len(src)))
+    ref = _oracle_frames(oracle, w, luts, calls)
+    _report("one-ROI frames", r, ref)
+    _assert_parity(r, ref)
+
+
 def test_frame_copy_threads_config(gpu, oracle):
     """frame_copy_threads sets how many threads copy a frame into pinned memory: results do not
     depend on it (a 1.5 MB row block, copied by 1 and by 8 threads) and match the oracle's
