@@ -67,7 +67,10 @@ public:
 	void Copy(void* dst, const void* src, size_t bytes)
 	{
 		if (bytes < kMinSplit || helpers_ == 0) {
-			std::memcpy(dst, src, bytes);
+			if (streaming_)
+				copy_streaming(dst, src, bytes);
+			else
+				std::memcpy(dst, src, bytes);
 			return;
 		}
 		std::lock_guard<std::mutex> serial(call_);

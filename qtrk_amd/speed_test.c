@@ -14,7 +14,7 @@
  * <dir> holds params.txt ("n width height luts planes steps locmode"), settings.bin (QTrkSettings),
  * lut.bin (float [luts][planes][steps]), rois.bin (uint16 [n][height][width]) and jobs.bin
  * (LocalizationJob [n]).  Writes results.bin (LocalizationResult [n], the last pass, in fetch
- * order) and prints one JSON line: per-pass loc/s and the mean schedule-call time.
+ * order) and prints one JSON line: per pass, loc/s and the mean time of a schedule call.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdio.h>
@@ -96,7 +96,7 @@ int main(int argc, char** argv)
 	const size_t roi_px = (size_t)w * h;
 
 	printf("{\"mode\": \"%s\", \"rois\": %d, \"locs_per_s\": [", argv[2], n);
-	double sched_sum = 0;
+	double* sched = (double*)malloc(sizeof(double) * (size_t)passes);
 	int got = 0;
 	for (int p = 0; p < passes; p++) {
 		QTrkClearResults(trk);
@@ -111,11 +111,14 @@ int main(int argc, char** argv)
 		QTrkFlush(trk);
 		while (QTrkGetResultCount(trk) < n) sleep_us(20);
 		const double t2 = now_s();
-		sched_sum += t1 - t0;
+		sched[p] = t1 - t0;
 		got = QTrkFetchResults(trk, res, n);
 		printf("%s%.1f", p ? ", " : "", n / (t2 - t0));
 	}
-	printf("], \"schedule_call_us_mean\": %.3f, \"fetched\": %d}\n", 1e6 * sched_sum / ((double)passes * n), got);
+	printf("], \"schedule_call_us\": [");
+	for (int p = 0; p < passes; p++) printf("%s%.3f", p ? ", " : "", 1e6 * sched[p] / n);
+	printf("], \"fetched\": %d}\n", got);
+	free(sched);
 	{
 		char path[4096];
 		snprintf(path, sizeof path, "%s/results.bin", dir);
