@@ -239,7 +239,7 @@ QTRK_API int QTrkLUTModelFitMLE(QTrkLUTModel* m, const float* data, const float*
                                 int n, int iterations, float* theta, double* fisher, float* crlb, float* nll,
                                 void* stream);
 /* Lanes per ROI of QTrkLUTModelFitMLE: 64, 128, 256 (the default), 512 for models of at most 6x6
- * sub-samples per pixel, 1024 for at most 3x3.  More lanes shorten the fit of few large ROIs (one workgroup
+ * sub-samples per pixel, 1024 for at most 4x4.  More lanes shorten the fit of few large ROIs (one workgroup
  * per ROI; e.g. 32 ROIs of 128x128 px fill 32 of the GPU's 256 CUs); the per-lane partial sums, and
  * so the last bits of the result, depend on the lane count. */
 QTRK_API int QTrkLUTModelSetFitLanes(QTrkLUTModel* m, int lanes);

@@ -749,7 +749,7 @@ class LUTModel:
         return th, fisher, crlb, nll
 
     def set_fit_lanes(self, lanes):
-        """Lanes per ROI of fit(): 64, 128, 256 (default), 512 (models of <= 6x6 sub-samples), 1024 (<= 3x3)."""
+        """Lanes per ROI of fit(): 64, 128, 256 (default), 512 (models of <= 6x6 sub-samples), 1024 (<= 4x4)."""
         if lib().QTrkLUTModelSetFitLanes(self.h, int(lanes)):
             check()
 

@@ -623,8 +623,9 @@ static hipError_t launch_fit_ovs(const LutModelParams& p, size_t lds, hipStream_
 bool lutmodel_fit_block_ok(int block, int ovs)
 {
 	if (block == 64 || block == 128 || block == 256) return true;
-	// 8 wavefronts per workgroup leave 256 VGPRs per lane, 16 leave 128: the 1024-lane instances
-	// of more than 3 x 3 sub-samples would spill
+	// 8 wavefronts per workgroup leave 256 VGPRs per lane, 16 leave 128: the 1024-lane instance of
+	// 4 x 4 sub-samples spills 23 VGPRs around the FP64 solve, outside the pixel loop; larger ones
+	// would spill inside it
 	return (block == 512 && ovs <= kMaxWideFitOvs) || (block == 1024 && ovs <= kMax1024FitOvs);
 }
 
@@ -638,7 +639,8 @@ hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int 
 		switch (p.ovs) {
 		case 1: return launch_fit<1, 1024>(p, lds, stream, block);
 		case 2: return launch_fit<2, 1024>(p, lds, stream, block);
-		default: return launch_fit<3, 1024>(p, lds, stream, block);
+		case 3: return launch_fit<3, 1024>(p, lds, stream, block);
+		default: return launch_fit<4, 1024>(p, lds, stream, block);
 		}
 	}
 	if (block == 512) return launch_fit_ovs<512>(p, lds, stream, block);

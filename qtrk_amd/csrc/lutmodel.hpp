@@ -24,7 +24,7 @@ namespace qtrk {
 constexpr int kModelParams = 5;  // x, y, z, I, bg
 constexpr int kMaxFitOvs = 8;    // sub-samples per pixel side the fit supports (forward/backward: any)
 constexpr int kMaxWideFitOvs = 6;  // ... with 512 lanes per ROI
-constexpr int kMax1024FitOvs = 3;  // ... with 1024 lanes per ROI
+constexpr int kMax1024FitOvs = 4;  // ... with 1024 lanes per ROI
 
 struct LutModelParams {
 	const float* luts;       // [count][planes][steps]

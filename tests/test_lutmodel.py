@@ -234,7 +234,7 @@ def test_fit_lanes_per_roi_agree(lut, gpu, size, ovs):
         t0[:, 2] += rng.uniform(-1.0, 1.0, n).astype(np.float32)
         t0[::3, 3] = -1.0
         out = {}
-        wide = (512, 1024) if ovs <= 3 else (512,)
+        wide = (512, 1024)
         for lanes in (64, 256) + wide:
             m.set_fit_lanes(lanes)
             fit, _, crlb, nll = m.fit(data, torch.from_numpy(t0), iterations=8)
@@ -262,7 +262,7 @@ def test_fit_lanes_per_roi_agree(lut, gpu, size, ovs):
 
 @pytest.mark.gpu
 def test_fit_lanes_rejects_unsupported(lut, gpu):
-    for ovs, bad, good in ((7, 512, 256), (4, 1024, 512), (4, 300, 128)):
+    for ovs, bad, good in ((7, 512, 256), (5, 1024, 512), (4, 300, 1024)):
         m = api.LUTModel(lut, 0.0, MAXR, W, H, ovs=ovs)
         try:
             with pytest.raises(api.QTrkError, match="lanes per ROI"):
