@@ -167,6 +167,8 @@ QueuedHIPTracker::~QueuedHIPTracker()
 		for (Slot* s : slots)
 			if (s->state == Slot::Launched) hipEventSynchronize(s->ev_done);
 		slots_cv.wait(lk, [&] { return waiters == 0; });
+		// a filling frame slot's blocks may still be on their way to the GPU
+		if (copyStream) hipStreamSynchronize(copyStream);
 	}
 	FreeSlots();
 }
