@@ -142,10 +142,12 @@ __device__ __forceinline__ float make_key(float s, unsigned p, unsigned imask)
 __host__ __device__ inline int prof_stride(int R) { return ((R + 3) & ~3) + 4; }
 
 struct MfmaLds {
-	int prof, w, w2, row, bq, keys, cand, ex, total;
+	int prof, w, w2, row, cm, bq, keys, cand, ex, mst, total;
 };
+// bytes of tile-mean rows staged in LDS at a time to compute the tile-mean terms
+constexpr int kMeanStage = 8192;
 __host__ __device__ inline int a16(int x) { return (x + 15) & ~15; }
-__host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS)
+__host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS, int ptiles)
 {
 	MfmaLds L;
 	const int F = FT * 32;
@@ -154,6 +156,7 @@ __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS)
 	L.w = o;     o += a16(KS * 16 * 4);
 	L.w2 = o;    o += a16(KS * 16 * 4);
 	L.row = o;   o += a16(F * 4);
+	L.cm = o;    o += a16(ptiles * F * 4);  // p . M_t for every (tile, profile)
 	// the B fragments live until the tile loop ends; the selection and
 	// rescoring scratch reuses their space afterwards
 	L.bq = o;
@@ -161,7 +164,10 @@ __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS)
 	L.keys = o;  o += a16(4 * F * 8 * 4);
 	L.cand = o;  o += a16(F * 8 * 4);
 	L.ex = o;    o += a16(F * 8 * 8);
-	L.total = ebq > o ? ebq : o;
+	// the tile-mean rows are staged before the tile loop, past the B fragments (the selection
+	// scratch above is used after the loop only)
+	L.mst = ebq;
+	L.total = ebq + kMeanStage > o ? ebq + kMeanStage : o;
 	return L;
 }
 
@@ -189,6 +195,7 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 {
 	typedef typename MfmaOp<PREC>::V V;
 	typedef typename MfmaOp<PREC>::E E;
+	constexpr int F = FT * 32;  // profiles per workgroup
 	const int s0 = P.start[bead] + chunk * FT * 32;
 	const int rem = P.start[bead + 1] - s0;
 	const int nf = rem < FT * 32 ? rem : FT * 32;
@@ -197,11 +204,13 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 	const int R = P.res, RS = prof_stride(R), KS = P.ksteps, NP = P.planes;
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	const int n = lane & 31, h = lane >> 5;
-	const MfmaLds L = mfma_lds(FT, R, KS);
+	const MfmaLds L = mfma_lds(FT, R, KS, P.ptiles);
 	float* prof_s = (float*)(smem + L.prof);
 	float* w_s = (float*)(smem + L.w);
 	float* w2_s = (float*)(smem + L.w2);
 	int* row_s = (int*)(smem + L.row);
+	float* cm_s = (float*)(smem + L.cm);
+	float4* mst_s = (float4*)(smem + L.mst);
 	uint4* bq_s = (uint4*)(smem + L.bq);
 	float* keys_s = (float*)(smem + L.keys);
 	int* cand_s = (int*)(smem + L.cand);
@@ -253,13 +262,53 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 		}
 		bq_s[i] = __builtin_bit_cast(uint4, v);
 	}
+	// the first tile's operands load while the tile-mean terms are computed
+	const int wu = __builtin_amdgcn_readfirstlane(wid);
+	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
+	uint4 a[KSMAX];
+	if (wu < P.ptiles) {
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++)
+			if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
+	}
+	// the FP32 tile-mean term p.M_t of every (tile, profile) pair, once per workgroup: a float FMA
+	// chain over the radii in order (two pairs per thread at a time), the mean rows staged in LDS
+	{
+		const int mq = P.mres >> 2;
+		const int tchunk = kMeanStage / (P.mres * 4) > 0 ? kMeanStage / (P.mres * 4) : 1;
+		const float4* mb = (const float4*)(P.mean + (size_t)bead * P.ptiles * P.mres);
+		for (int tc = 0; tc < P.ptiles; tc += tchunk) {
+			const int ntc = P.ptiles - tc < tchunk ? P.ptiles - tc : tchunk, total = ntc * F;
+			__syncthreads();  // the previous chunk's rows are read
+			for (int e = tid; e < ntc * mq; e += 256) mst_s[e] = mb[(size_t)tc * mq + e];
+			__syncthreads();
+			for (int i0 = tid; i0 < total; i0 += 512) {
+				const int i1 = i0 + 256 < total ? i0 + 256 : i0;
+				const int t0 = i0 / F, t1 = i1 / F;
+				const float4* p0 = (const float4*)(prof_s + (i0 - t0 * F) * RS);
+				const float4* p1 = (const float4*)(prof_s + (i1 - t1 * F) * RS);
+				const float4* m0 = mst_s + t0 * mq;
+				const float4* m1 = mst_s + t1 * mq;
+				float c0 = 0.0f, c1 = 0.0f;
+				for (int q = 0; q < mq; q++) {
+					const float4 a0 = p0[q], b0 = m0[q], a1 = p1[q], b1 = m1[q];
+					c0 = __builtin_fmaf(a0.x, b0.x, c0);
+					c1 = __builtin_fmaf(a1.x, b1.x, c1);
+					c0 = __builtin_fmaf(a0.y, b0.y, c0);
+					c1 = __builtin_fmaf(a1.y, b1.y, c1);
+					c0 = __builtin_fmaf(a0.z, b0.z, c0);
+					c1 = __builtin_fmaf(a1.z, b1.z, c1);
+					c0 = __builtin_fmaf(a0.w, b0.w, c0);
+					c1 = __builtin_fmaf(a1.w, b1.w, c1);
+				}
+				cm_s[tc * F + i0] = c0;
+				cm_s[tc * F + i1] = c1;
+			}
+		}
+	}
 	__syncthreads();
 
-	const int wu = __builtin_amdgcn_readfirstlane(wid);  // wave-uniform: tile-mean rows load through the scalar cache
-	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
 	const float* nb = P.norm + (size_t)bead * P.ptiles * 32 + 4 * h;
-	const int mq = P.mres >> 2;
-	const float4* prow = (const float4*)(prof_s + (lane < FT * 32 ? lane : 0) * RS);
 	const unsigned imask = (1u << P.idx_bits) - 1u;
 	float top[FT][8];
 #pragma unroll
@@ -267,23 +316,7 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 #pragma unroll
 		for (int i = 0; i < 8; i++) top[ft][i] = -FLT_MAX;
 
-	uint4 a[KSMAX];
-	if (wu < P.ptiles) {
-#pragma unroll
-		for (int ks = 0; ks < KSMAX; ks++)
-			if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
-	}
 	for (int t = wu; t < P.ptiles; t += 4) {
-		// FP32 tile-mean term p.M_t for profile `lane`
-		const float* mt = P.mean + ((size_t)bead * P.ptiles + t) * P.mres;
-		float c = 0.0f;
-		for (int q = 0; q < mq; q++) {
-			const float4 pv = prow[q];
-			c = __builtin_fmaf(pv.x, mt[4 * q], c);
-			c = __builtin_fmaf(pv.y, mt[4 * q + 1], c);
-			c = __builtin_fmaf(pv.z, mt[4 * q + 2], c);
-			c = __builtin_fmaf(pv.w, mt[4 * q + 3], c);
-		}
 		// |L|^2_w of this lane's rows: 8*(i/4) + 4*h + i%4 of the tile
 		float nv[16];
 #pragma unroll
@@ -311,11 +344,10 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 			for (int ks = 0; ks < KSMAX; ks++)
 				if (ks < KS) a[ks] = fb[((size_t)(t + 4) * KS + ks) * 64];
 		}
-		// lane (n, h) needs p.M_t of profiles n and 32 + n: its own value and lane n^32's
-		const float co = __shfl_xor(c, 32);
+		// lane (n, h) takes p.M_t of profile ft * 32 + n
 		float cm[FT];
-		cm[0] = 2.0f * (h ? co : c);
-		if (FT > 1) cm[FT - 1] = 2.0f * (h ? c : co);
+#pragma unroll
+		for (int ft = 0; ft < FT; ft++) cm[ft] = 2.0f * cm_s[t * F + ft * 32 + n];
 #pragma unroll
 		for (int ft = 0; ft < FT; ft++) {
 			float g0[8], g1[8];
@@ -648,7 +680,7 @@ hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* fra
 template <int PREC, int FT, int KSMAX, bool LOOP>
 static hipError_t launch_mfma(const LutMatchParams& q, int grid, hipStream_t stream)
 {
-	const size_t lds = (size_t)mfma_lds(FT, q.res, q.ksteps).total;
+	const size_t lds = (size_t)mfma_lds(FT, q.res, q.ksteps, q.ptiles).total;
 	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_mfma_kernel<PREC, FT, KSMAX, LOOP>,
 	                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 	if (e != hipSuccess) return e;
@@ -709,7 +741,7 @@ size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per
 {
 	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes).total;
 	const int FT = frame_tile ? frame_tile / 32 : 1;
-	return (size_t)mfma_lds(FT, res, ksteps).total;
+	return (size_t)mfma_lds(FT, res, ksteps, (planes + 31) / 32).total;
 }
 
 // ------------------------------------------------------- grouping by bead
