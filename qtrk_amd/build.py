@@ -19,7 +19,7 @@ LIB = os.path.join(HERE, "libqtrk.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["localize.hip", "beadfinder.hip", "lutmatch.hip", "lutmodel.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp",
-           "lutmodel_host.cpp", "pipeline_kernels.hip", "pipeline.cpp", "tracker_common.cpp", "result_manager.cpp", "noalloc.hip"]
+           "lutmodel_host.cpp", "pipeline.cpp", "tracker_common.cpp", "result_manager.cpp", "noalloc.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -405,7 +405,19 @@ __global__ __launch_bounds__(MAXB) void lutmodel_fit_kernel(LutModelParams P)
 		for (int p = threadIdx.x; p < npx; p += nt) dat[p] = (float)((const uint8_t*)P.data)[d0 + p];
 	else
 		for (int p = threadIdx.x; p < npx; p += nt) dat[p] = ((const float*)P.data)[d0 + p];
-	if (threadIdx.x < kModelParams) th[threadIdx.x] = P.theta[(size_t)roi * kModelParams + threadIdx.x];
+	if (threadIdx.x < kModelParams) {
+		float v;
+		if (P.start_xy) {
+			// (x, y, z, -1, -1) from the localization and the matched z
+			const int k = threadIdx.x;
+			const float lim = k == 0 ? (float)(P.width - 1) : k == 1 ? (float)(P.height - 1) : (float)(P.planes - 1);
+			const float s = k < 2 ? P.start_xy[(size_t)roi * P.start_stride + k] : k == 2 ? P.start_z[roi] : -1.0f;
+			v = k > 2 ? -1.0f : s == s ? fminf(fmaxf(s, 0.0f), lim) : 0.5f * lim;
+		} else {
+			v = P.theta[(size_t)roi * kModelParams + threadIdx.x];
+		}
+		th[threadIdx.x] = v;
+	}
 	const float* lut = P.luts + (size_t)lut_of(P, roi) * P.planes * P.steps;
 	const float radialcov = P.steps / (P.maxr - P.minr);
 	const float step = 1.0f / P.ovs;

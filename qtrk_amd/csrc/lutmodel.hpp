@@ -50,11 +50,16 @@ struct LutModelParams {
 	double* fisher;          // [n][25] Fisher matrix at the fitted parameters, or null
 	float* crlb;             // [n][5] sqrt(diag(F^-1)), or null
 	float* nll;              // [n] negative log-likelihood (without the data-only log-factorial term), or null
+	// fit start from localizations instead of theta (when start_xy is set): x and y at
+	// start_xy[i * start_stride] and [+1], z at start_z[i], each clamped into the ROI and the LUT
+	// (NaN: the middle), I = bg = -1 (the least-squares start)
+	const float* start_xy;
+	int start_stride;        // in floats
+	const float* start_z;
 };
 
 hipError_t launch_lutmodel_forward(const LutModelParams& p, hipStream_t stream);
 hipError_t launch_lutmodel_backward(const LutModelParams& p, hipStream_t stream);
-// block: lanes per ROI (64, 128 or 256)
 // block: lanes per ROI, 64, 128, 256, 512 (up to kMaxWideFitOvs sub-samples per side) or 1024 (up to
 // kMax1024FitOvs)
 hipError_t launch_lutmodel_fit(const LutModelParams& p, hipStream_t stream, int block = 256);

@@ -115,7 +115,6 @@ int Pipeline::Add(QueuedHIPTracker* trk, int n, int pdt, int precision, const fl
 		m.ovs = 4;
 		m.spline = 1;
 		if (lutmodel_lds_bytes(m, true) > 160 * 1024) fail("ROI too large for the MLE (LDS)");
-		t->theta0.ensure((size_t)n * kModelParams * sizeof(float));
 		t->theta.ensure((size_t)n * kModelParams * sizeof(float));
 		t->crlb.ensure((size_t)n * kModelParams * sizeof(float));
 	}
@@ -198,11 +197,13 @@ void Pipeline::enqueue_match(PipelineTracker& t, hipStream_t s)
 void Pipeline::enqueue_mle(PipelineTracker& t, hipStream_t s)
 {
 	if (!t.mle) return;
-	HIP_CHECK(launch_assemble_start(t.results.as<DevResult>(), t.z.as<float>(), t.theta0.as<float>(), t.n, t.width,
-	                                t.height, t.model.planes, s));
 	LutModelParams m = t.model;
 	m.n = t.n;
-	m.theta = t.theta0.as<float>();
+	// the start: the localization's x/y and the matched z, read by the fit kernel itself
+	m.theta = nullptr;
+	m.start_xy = (const float*)((const char*)t.results.p + offsetof(DevResult, x));
+	m.start_stride = (int)(sizeof(DevResult) / sizeof(float));
+	m.start_z = t.z.as<float>();
 	m.lut_index = m.count > 1 ? (const int*)((const char*)t.jobs.p + offsetof(DevJob, zlutIndex)) : nullptr;
 	m.lut_index_stride = (int)(sizeof(DevJob) / sizeof(int));
 	m.data = t.rois.p;

@@ -26,11 +26,6 @@
 
 namespace qtrk {
 
-// theta0[i] = (x, y, z, -1, -1) from the localization result and the matched z
-// (x, y clamped into the ROI, z into the model LUT; NaN -> the middle), so that
-// the fit starts with its least-squares I/bg estimate.
-hipError_t launch_assemble_start(const DevResult* res, const float* z, float* theta0, int n, int width, int height,
-                                 int planes, hipStream_t stream);
 
 struct PipelineTracker {
 	QueuedHIPTracker* trk = nullptr;
@@ -38,7 +33,7 @@ struct PipelineTracker {
 	int precision = kMatchBF16, iters = 5;
 	uint64_t lut_version = 0;  // the tracker's LUTVersion() the match tables were made from
 	bool mle = false;
-	DeviceBuffer rois, jobs, results, prof, z, best, cand, theta0, theta, crlb;
+	DeviceBuffer rois, jobs, results, prof, z, best, cand, theta, crlb;
 	DeviceBuffer start, rowidx;
 	int max_per_bead = 0;
 	bool jobs_set = false;
