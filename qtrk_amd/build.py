@@ -19,7 +19,8 @@ LIB = os.path.join(HERE, "libqtrk.so")
 OBJDIR = os.path.join(HERE, "_build")
 
 SOURCES = ["localize.hip", "beadfinder.hip", "lutmatch.hip", "lutmodel.hip", "hip_tracker.cpp", "c_api.cpp", "lutmatch_host.cpp",
-           "lutmodel_host.cpp", "pipeline.cpp", "tracker_common.cpp", "result_manager.cpp", "noalloc.hip"]
+           "lutmodel_host.cpp", "pipeline.cpp", "tracker_common.cpp", "result_manager.cpp", "noalloc.hip",
+           "c_api_idle.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

@@ -207,11 +207,12 @@ QTRK_API int QTrkGetQueueLength(QueuedTracker* q, int* maxQueueLen)
 	GUARD(r = q->GetQueueLength(maxQueueLen));
 	return r;
 }
-QTRK_API bool QTrkIsIdle(QueuedTracker* q)
+// QTrkIsIdle itself is defined in c_api_idle.cpp, with an int result
+extern "C" __attribute__((visibility("hidden"))) int qtrk_is_idle(void* qtrk)
 {
 	bool r = false;
-	GUARD(r = q->IsIdle());
-	return r;
+	GUARD(r = ((QueuedTracker*)qtrk)->IsIdle());
+	return r ? 1 : 0;
 }
 static void copy_str(const std::string& s, char* dst, int n)
 {

@@ -214,6 +214,26 @@ def test_queue_api_matches_batch_and_is_lifo(gpu, oracle):
     trk.free()
 
 
+def test_is_idle_fills_the_return_register(gpu):
+    """QTrkIsIdle read as a 4-byte integer, as C#'s default bool marshalling reads it
+    (QTrkDotNet/QTrkDLL.cs): exactly 1 for an idle tracker, 0 or 1 while busy."""
+    import ctypes as C
+    w = synth.make_workload("realtime100", n=256)
+    trk = api.QTrkInstance(w["settings"])
+    try:
+        trk.set_radial_zlut(w["luts"])
+        trk.set_localization_mode(w["mode"])
+        as_int = C.CFUNCTYPE(C.c_int, C.c_void_p)(C.cast(api.lib().QTrkIsIdle, C.c_void_p).value)
+        assert as_int(trk.h) == 1
+        for i in range(len(w["jobs"])):
+            trk.schedule_localization(w["rois"][i], w["jobs"][i])
+        assert as_int(trk.h) in (0, 1)
+        trk.wait_for(len(w["jobs"]))
+        assert as_int(trk.h) == 1
+    finally:
+        trk.free()
+
+
 def test_small_batches_many_streams_block_and_complete(gpu, oracle):
     """Batches of 8 on 2 streams: ScheduleLocalization blocks while 16 ROIs are queued."""
     w = synth.make_workload("plain64", n=200)
