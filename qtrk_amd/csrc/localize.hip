@@ -923,7 +923,8 @@ __device__ __forceinline__ void power_spectrum_phase(const LocalizeParams& P, co
 
 // EXT compiles in the optional modes (LT_XCor1D, LT_ZLUTAlign, LT_LocalizeZWeighted,
 // LT_FourierLUT); the variant without them is the one the common modes run.
-template <typename SPix, bool EXT>
+// CW, CH: the ROI size as compile-time constants (0: from P), see launch_variant
+template <typename SPix, bool EXT, int CW = 0, int CH = 0>
 __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 {
 	extern __shared__ __align__(16) unsigned char lds[];
@@ -944,7 +945,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
 	const int roi = blockIdx.x;
-	const int W = P.width, H = P.height, npx = W * H;
+	const int W = CW ? CW : P.width, H = CH ? CH : P.height, npx = W * H;
 	const bool build = P.build != 0;
 	const DevJob job = P.jobs ? P.jobs[roi] : DevJob{};
 	const int bead = build ? roi : job.zlutIndex;
@@ -1692,16 +1693,40 @@ __global__ void __launch_bounds__(kBlock) gauss_wave_kernel(LocalizeParams P)
 
 // Launches above the default 64 KB of dynamic LDS raise the kernel's limit first (on the
 // current device, every time: the setting is per device).
-template <typename SPix, bool EXT>
-static hipError_t launch_variant(const LocalizeParams& p, size_t lds, hipStream_t stream)
+template <typename SPix, bool EXT, int CW, int CH>
+static hipError_t launch_sized(const LocalizeParams& p, size_t lds, hipStream_t stream)
 {
 	if (lds > 64 * 1024) {
-		const hipError_t e = hipFuncSetAttribute((const void*)localize_kernel<SPix, EXT>,
+		const hipError_t e = hipFuncSetAttribute((const void*)localize_kernel<SPix, EXT, CW, CH>,
 		                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 		if (e != hipSuccess) return e;
 	}
-	hipLaunchKernelGGL((localize_kernel<SPix, EXT>), dim3(p.n), dim3(kBlock), lds, stream, p);
+	hipLaunchKernelGGL((localize_kernel<SPix, EXT, CW, CH>), dim3(p.n), dim3(kBlock), lds, stream, p);
 	return hipGetLastError();
+}
+
+// Square ROIs of common sizes, staged as float, run an instance compiled with the size as a
+// constant: a bilinear sample's second-row offset (W and W + 1 floats) then folds into the LDS
+// instruction's immediate offsets instead of costing an address addition per sample, and the loops
+// over rows and columns have constant bounds (realtime100, 100x100: 0.747 -> 0.728 ms per launch).
+template <typename SPix, bool EXT>
+static hipError_t launch_variant(const LocalizeParams& p, size_t lds, hipStream_t stream)
+{
+	if constexpr (sizeof(SPix) == sizeof(float) && !EXT) {
+		if (p.width == p.height) {
+			switch (p.width) {
+			case 32: return launch_sized<SPix, EXT, 32, 32>(p, lds, stream);
+			case 48: return launch_sized<SPix, EXT, 48, 48>(p, lds, stream);
+			case 64: return launch_sized<SPix, EXT, 64, 64>(p, lds, stream);
+			case 80: return launch_sized<SPix, EXT, 80, 80>(p, lds, stream);
+			case 96: return launch_sized<SPix, EXT, 96, 96>(p, lds, stream);
+			case 100: return launch_sized<SPix, EXT, 100, 100>(p, lds, stream);
+			case 128: return launch_sized<SPix, EXT, 128, 128>(p, lds, stream);
+			default: break;
+			}
+		}
+	}
+	return launch_sized<SPix, EXT, 0, 0>(p, lds, stream);
 }
 
 hipError_t launch_localize(const LocalizeParams& p, hipStream_t stream)
