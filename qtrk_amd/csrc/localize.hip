@@ -639,6 +639,19 @@ __device__ __forceinline__ float rotated_peak(const float2* G, int N, int shift,
 	return lane == 0 ? peak_fit<float>(bi, N, kQIW, at) : 0.0f;
 }
 
+// One term of a LUT score (cputrack/cpu_tracker.cpp:829-836): d is the float difference in double;
+// s - (w d)^2 with radial weights w, else s - d^2 as one fused multiply-add, which rounds as the
+// separate product and sum do because d^2 (a 24-bit significand squared) is exact in double.
+__device__ __forceinline__ double zcmp_step(double s, double d, const float* w, int r)
+{
+	if (w) {
+		d *= (double)w[r];
+		d = -d * d;
+		return s + d;
+	}
+	return __builtin_fma(-d, d, s);
+}
+
 // Best plane (first maximum) of the LUT scores over the workgroup and the FP64
 // 7-point fit around it (cputrack/cpu_tracker.cpp:851-861); valid in thread 0.
 __device__ __forceinline__ float scores_peak(const double* scores, int planes, double* red, float* st, int tid,
@@ -1426,10 +1439,8 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 								}
 #pragma unroll
 								for (int j = 0; j < 4; j++) {
-									double d = (double)(pv[j] - lv[j]);
-									if (P.zcmp) d *= (double)P.zcmp[r0 + j];
-									d = -d * d;
-									s += d;
+									const double d = (double)(pv[j] - lv[j]);
+									s = zcmp_step(s, d, P.zcmp, r0 + j);
 								}
 							}
 							scores[k0 + tid] = s;
@@ -1441,12 +1452,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					for (int k = tid; k < P.zplanes; k += kBlock) {
 						const float* Lk = L0 + (size_t)k * nb;
 						double s = 0.0;
-						for (int r = 0; r < nb; r++) {
-							double d = (double)(zprof[r] - Lk[r]);
-							if (P.zcmp) d *= (double)P.zcmp[r];
-							d = -d * d;
-							s += d;
-						}
+						for (int r = 0; r < nb; r++) s = zcmp_step(s, (double)(zprof[r] - Lk[r]), P.zcmp, r);
 						scores[k] = s;
 						if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k] = (float)s;
 					}

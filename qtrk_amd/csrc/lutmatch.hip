@@ -3,13 +3,17 @@
 //
 // Matrix-core path, one workgroup (4 wavefronts) per bead and 32 (or 64) profiles:
 //   * the bead's profiles are staged in LDS (float) and turned into MFMA
-//     B operands once: B[k=r][n=profile] = w_r^2 * prof[r], rounded to BF16/FP16;
-//   * the LUT was split once (launch_lutmatch_prepare) into FP32 tile means and
+//     B operands once: B[k=r][n=profile] = w_r^2 * prof[r], rounded to BF16/FP16
+//     (its rounding remainder is a second set of B operands);
+//   * the LUT was split once (launch_lutmatch_prepare) into tile means and
 //     residual A-operand fragments, [bead][32-plane tile][16-radius step][lane]
 //     x 8 values, so each wavefront's operand load is one contiguous, fully
 //     coalesced 1 KB read;
+//   * the tile-mean term p.M_t of every (tile, profile) pair comes from the
+//     matrix cores too: the means' high and low parts against B and its
+//     remainder (hi.hi + lo.hi + hi.lo + lo.lo, FP32 accumulation);
 //   * v_mfma_f32_32x32x16_{bf16,f16} accumulates p.D_w for a 32-plane x
-//     32-profile tile in FP32; the epilogue adds the FP32 tile-mean term
+//     32-profile tile in FP32; the epilogue adds the tile-mean term
 //     2 p.M_t, subtracts |L|^2_w and folds the 16 values each lane holds into a
 //     per-profile top-8 kept in registers (sorting networks on keys that carry
 //     the plane index in their low bits);
@@ -61,12 +65,19 @@ __device__ __forceinline__ void xcd_map(int id, int chunks, int& bead, int& chun
 // w are in LDS; the LUT row L is read from global memory 32 radii at a time
 // (eight 16-byte loads in flight) when R is a multiple of 4, which keeps the
 // row 16-byte aligned.
+// Without weights the negated square of the float difference is exact in double
+// (24-bit significand squared, far inside the exponent range), so one fused
+// multiply-add rounds exactly as the separate product and sum do.
 __device__ __forceinline__ void score_step(double& s, float p, float l, const float* w, int r)
 {
 	double d = (double)(p - l);
-	if (w) d *= (double)w[r];
-	d = -d * d;
-	s += d;
+	if (w) {
+		d *= (double)w[r];
+		d = -d * d;
+		s += d;
+	} else {
+		s = __builtin_fma(-d, d, s);
+	}
 }
 __device__ __forceinline__ double exact_score(const float* pr, const float* L, const float* w, int R)
 {
@@ -142,10 +153,8 @@ __device__ __forceinline__ float make_key(float s, unsigned p, unsigned imask)
 __host__ __device__ inline int prof_stride(int R) { return ((R + 3) & ~3) + 4; }
 
 struct MfmaLds {
-	int prof, w, w2, row, cm, bq, keys, cand, ex, mst, total;
+	int prof, w, w2, row, cm, bq, keys, cand, ex, blo, total;
 };
-// bytes of tile-mean rows staged in LDS at a time to compute the tile-mean terms
-constexpr int kMeanStage = 8192;
 __host__ __device__ inline int a16(int x) { return (x + 15) & ~15; }
 __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS, int ptiles)
 {
@@ -164,10 +173,11 @@ __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS, int ptiles)
 	L.keys = o;  o += a16(4 * F * 8 * 4);
 	L.cand = o;  o += a16(F * 8 * 4);
 	L.ex = o;    o += a16(F * 8 * 8);
-	// the tile-mean rows are staged before the tile loop, past the B fragments (the selection
-	// scratch above is used after the loop only)
-	L.mst = ebq;
-	L.total = ebq + kMeanStage > o ? ebq + kMeanStage : o;
+	// the B fragments' rounding remainders, used before the tile loop only, lie past the B
+	// fragments (the selection scratch above is used after the loop only)
+	L.blo = ebq;
+	const int eblo = ebq + a16(FT * KS * 64 * 16);
+	L.total = eblo > o ? eblo : o;
 	return L;
 }
 
@@ -210,12 +220,23 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 	float* w2_s = (float*)(smem + L.w2);
 	int* row_s = (int*)(smem + L.row);
 	float* cm_s = (float*)(smem + L.cm);
-	float4* mst_s = (float4*)(smem + L.mst);
 	uint4* bq_s = (uint4*)(smem + L.bq);
+	uint4* blo_s = (uint4*)(smem + L.blo);
 	float* keys_s = (float*)(smem + L.keys);
 	int* cand_s = (int*)(smem + L.cand);
 	double* ex_s = (double*)(smem + L.ex);
 
+	// this wavefront's mean tile (mean tile m, profile tile ft for job m * FT + ft) loads while the
+	// profiles are staged
+	const int wu = __builtin_amdgcn_readfirstlane(wid);
+	const int MJ = P.mtiles * FT;
+	uint4 am[KSMAX];
+	if (wu < MJ) {
+		const uint4* mb = P.mfrag + ((size_t)bead * P.mtiles + wu / FT) * KS * 64 + lane;
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++)
+			if (ks < KS) am[ks] = mb[(size_t)ks * 64];
+	}
 	for (int i = tid; i < FT * 32; i += 256) row_s[i] = i < nf ? (P.rowidx ? P.rowidx[s0 + i] : s0 + i) : -1;
 	for (int k = tid; k < KS * 16; k += 256) {
 		const float wk = k < R ? (P.w ? P.w[k] : 1.0f) : 0.0f;
@@ -249,63 +270,68 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 
 	// B operands in MFMA order, shared by the four wavefronts: fragment (ft, ks)
 	// lane l holds profile column l%32 of tile ft, radii 16*ks + 8*(l/32) + [0, 8),
-	// weighted by w^2 and rounded to E
+	// weighted by w^2 and rounded to E; the remainders (exact in float) rounded to E
 	for (int i = tid; i < FT * KS * 64; i += 256) {
 		const int l = i & 63, fk = i >> 6, ks = fk % KS, ft = fk / KS;
 		const float* pr = prof_s + (ft * 32 + (l & 31)) * RS;
 		const int k0 = ks * 16 + (l >> 5) * 8;
-		V v;
+		V v, vl;
 #pragma unroll
 		for (int j = 0; j < 8; j++) {
 			const int k = k0 + j;
-			v[j] = (E)(k < R ? pr[k] * w2_s[k] : 0.0f);
+			const float x = k < R ? pr[k] * w2_s[k] : 0.0f;
+			v[j] = (E)x;
+			vl[j] = (E)(x - (float)v[j]);
 		}
 		bq_s[i] = __builtin_bit_cast(uint4, v);
+		blo_s[i] = __builtin_bit_cast(uint4, vl);
 	}
-	// the first tile's operands load while the tile-mean terms are computed
-	const int wu = __builtin_amdgcn_readfirstlane(wid);
+	// the first tile's operands load while the tile-mean terms are computed (after them with
+	// 16 k-steps, where both operand sets would not fit in the registers together)
 	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
 	uint4 a[KSMAX];
-	if (wu < P.ptiles) {
+	auto load_first_tile = [&]() {
+		if (wu < P.ptiles) {
 #pragma unroll
-		for (int ks = 0; ks < KSMAX; ks++)
-			if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
-	}
-	// the FP32 tile-mean term p.M_t of every (tile, profile) pair, once per workgroup: a float FMA
-	// chain over the radii in order (two pairs per thread at a time), the mean rows staged in LDS
-	{
-		const int mq = P.mres >> 2;
-		const int tchunk = kMeanStage / (P.mres * 4) > 0 ? kMeanStage / (P.mres * 4) : 1;
-		const float4* mb = (const float4*)(P.mean + (size_t)bead * P.ptiles * P.mres);
-		for (int tc = 0; tc < P.ptiles; tc += tchunk) {
-			const int ntc = P.ptiles - tc < tchunk ? P.ptiles - tc : tchunk, total = ntc * F;
-			__syncthreads();  // the previous chunk's rows are read
-			for (int e = tid; e < ntc * mq; e += 256) mst_s[e] = mb[(size_t)tc * mq + e];
-			__syncthreads();
-			for (int i0 = tid; i0 < total; i0 += 512) {
-				const int i1 = i0 + 256 < total ? i0 + 256 : i0;
-				const int t0 = i0 / F, t1 = i1 / F;
-				const float4* p0 = (const float4*)(prof_s + (i0 - t0 * F) * RS);
-				const float4* p1 = (const float4*)(prof_s + (i1 - t1 * F) * RS);
-				const float4* m0 = mst_s + t0 * mq;
-				const float4* m1 = mst_s + t1 * mq;
-				float c0 = 0.0f, c1 = 0.0f;
-				for (int q = 0; q < mq; q++) {
-					const float4 a0 = p0[q], b0 = m0[q], a1 = p1[q], b1 = m1[q];
-					c0 = __builtin_fmaf(a0.x, b0.x, c0);
-					c1 = __builtin_fmaf(a1.x, b1.x, c1);
-					c0 = __builtin_fmaf(a0.y, b0.y, c0);
-					c1 = __builtin_fmaf(a1.y, b1.y, c1);
-					c0 = __builtin_fmaf(a0.z, b0.z, c0);
-					c1 = __builtin_fmaf(a1.z, b1.z, c1);
-					c0 = __builtin_fmaf(a0.w, b0.w, c0);
-					c1 = __builtin_fmaf(a1.w, b1.w, c1);
-				}
-				cm_s[tc * F + i0] = c0;
-				cm_s[tc * F + i1] = c1;
+			for (int ks = 0; ks < KSMAX; ks++)
+				if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
+		}
+	};
+	if constexpr (KSMAX <= 8) load_first_tile();
+	__syncthreads();
+	// the tile-mean term p.M_t of every (tile, profile) pair: mean tile m's rows r < 16 (high
+	// parts of M_{16m+r}) and 16 + r (low parts) against B and its remainder; lane (n, h) holds
+	// rows 8*(i/4) + 4*h + i%4 in accumulator i, so row r's high and low products are
+	// accumulators i and i + 8 of the same lane
+	for (int j = wu; j < MJ; j += 4) {
+		const int ft = j % FT, mt = j / FT;
+		if (j != wu) {
+			const uint4* mb = P.mfrag + ((size_t)bead * P.mtiles + mt) * KS * 64 + lane;
+#pragma unroll
+			for (int ks = 0; ks < KSMAX; ks++)
+				if (ks < KS) am[ks] = mb[(size_t)ks * 64];
+		}
+		f32x16 hi, lo;
+#pragma unroll
+		for (int i = 0; i < 16; i++) {
+			hi[i] = 0.0f;
+			lo[i] = 0.0f;
+		}
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++) {
+			if (ks < KS) {
+				const V av = __builtin_bit_cast(V, am[ks]);
+				hi = MfmaOp<PREC>::mma(av, __builtin_bit_cast(V, bq_s[(ft * KS + ks) * 64 + lane]), hi);
+				lo = MfmaOp<PREC>::mma(av, __builtin_bit_cast(V, blo_s[(ft * KS + ks) * 64 + lane]), lo);
 			}
 		}
+#pragma unroll
+		for (int i = 0; i < 8; i++) {
+			const int t = mt * 16 + 8 * (i >> 2) + 4 * h + (i & 3);
+			if (t < P.ptiles) cm_s[t * F + ft * 32 + n] = hi[i] + ((hi[i + 8] + lo[i]) + lo[i + 8]);
+		}
 	}
+	if constexpr (KSMAX > 8) load_first_tile();
 	__syncthreads();
 
 	const float* nb = P.norm + (size_t)bead * P.ptiles * 32 + 4 * h;
@@ -488,14 +514,19 @@ __device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, i
 #pragma unroll
 		for (int f = 0; f < kMatchExactTile; f++) acc[f] = 0.0;
 		// radius r for all 16 profiles: float difference, weight, negated square, double sum
+		// (unweighted: one exact fused multiply-add, as in score_step)
 		auto step = [&](int r, float l) {
 			const double wr = WEIGHTED ? (double)w_s[r] : 1.0;
 #pragma unroll
 			for (int f = 0; f < kMatchExactTile; f++) {
 				double d = (double)(prof_s[f * R4 + r] - l);
-				if (WEIGHTED) d *= wr;
-				d = -d * d;
-				acc[f] += d;
+				if (WEIGHTED) {
+					d *= wr;
+					d = -d * d;
+					acc[f] += d;
+				} else {
+					acc[f] = __builtin_fma(-d, d, acc[f]);
+				}
 			}
 		};
 		int r = 0;
@@ -570,14 +601,13 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 
 // ------------------------------------------------------------ preparation
 
-// Tile means: raw[b][t][r] = mean of the tile's valid rows, mean = w_r^2 * raw;
-// rows are mres long, zero past res.
-__global__ void lutmatch_mean_kernel(LutMatchParams P, float* mean, float* raw)
+// Tile means: raw[b][t][r] = mean of the tile's valid rows (mres-long rows, zero past res).
+__global__ void lutmatch_mean_kernel(LutMatchParams P, float* raw, int mres)
 {
-	const size_t total = (size_t)P.beads * P.ptiles * P.mres;
+	const size_t total = (size_t)P.beads * P.ptiles * mres;
 	for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
-		const int r = (int)(g % P.mres);
-		const size_t q = g / P.mres;
+		const int r = (int)(g % mres);
+		const size_t q = g / mres;
 		const int t = (int)(q % P.ptiles);
 		const size_t b = q / P.ptiles;
 		const int p0 = t * 32;
@@ -588,14 +618,42 @@ __global__ void lutmatch_mean_kernel(LutMatchParams P, float* mean, float* raw)
 			s /= (float)cnt;
 		}
 		raw[g] = s;
-		mean[g] = (r < P.res && P.w ? P.w[r] * P.w[r] : 1.0f) * s;
+	}
+}
+
+// Tile means as A-operand fragments: lane (row = lane % 32, radii 8*(lane/32) + [0, 8)) of each
+// (mean tile m, 16-radius step) holds, for row j < 16, M_{16m+j} rounded to E, and for row j >= 16
+// the remainder M_{16m+j-16} - (float)E(M) rounded to E; rows of tiles past ptiles are 0.
+template <typename E>
+__global__ void lutmatch_meanfrag_kernel(LutMatchParams P, uint4* mfrag, const float* raw, int mres)
+{
+	typedef E V8 __attribute__((ext_vector_type(8)));
+	const size_t total = (size_t)P.beads * P.mtiles * P.ksteps * 64;
+	for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < total; g += (size_t)gridDim.x * blockDim.x) {
+		const int lane = (int)(g & 63);
+		size_t q = g >> 6;
+		const int ks = (int)(q % P.ksteps);
+		q /= P.ksteps;
+		const int m = (int)(q % P.mtiles);
+		const size_t b = q / P.mtiles;
+		const int j = lane & 31, t = m * 16 + (j & 15);
+		const int k0 = ks * 16 + (lane >> 5) * 8;
+		V8 v;
+#pragma unroll
+		for (int i = 0; i < 8; i++) {
+			const int k = k0 + i;
+			const float x = (t < P.ptiles && k < P.res) ? raw[(b * P.ptiles + t) * mres + k] : 0.0f;
+			const E hi = (E)x;
+			v[i] = j < 16 ? hi : (E)(x - (float)hi);
+		}
+		mfrag[g] = __builtin_bit_cast(uint4, v);
 	}
 }
 
 // Residual fragments: lane (row = lane % 32, radii 8*(lane/32) + [0, 8)) of each
 // (tile, 16-radius step) holds L - M_t rounded to E; padded rows and radii are 0.
 template <typename E>
-__global__ void lutmatch_frag_kernel(LutMatchParams P, uint4* frag, const float* raw)
+__global__ void lutmatch_frag_kernel(LutMatchParams P, uint4* frag, const float* raw, int mres)
 {
 	typedef E V8 __attribute__((ext_vector_type(8)));
 	const size_t total = (size_t)P.beads * P.ptiles * P.ksteps * 64;
@@ -608,7 +666,7 @@ __global__ void lutmatch_frag_kernel(LutMatchParams P, uint4* frag, const float*
 		const size_t b = q / P.ptiles;
 		const int row = t * 32 + (lane & 31);
 		const int k0 = ks * 16 + (lane >> 5) * 8;
-		const float* mt = raw + (b * P.ptiles + t) * P.mres;
+		const float* mt = raw + (b * P.ptiles + t) * mres;
 		V8 v;
 #pragma unroll
 		for (int j = 0; j < 8; j++) {
@@ -659,18 +717,22 @@ __global__ void lutmatch_norm_kernel(LutMatchParams P, float* norm)
 
 static int grid_for(size_t n) { return (int)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536); }
 
-hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, float* norm, float* mean,
-                                   float* raw_mean, hipStream_t stream)
+hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, uint4* mfrag, float* norm,
+                                   float* raw_mean, int mres, hipStream_t stream)
 {
 	if (prec != kMatchBF16 && prec != kMatchFP16) return hipErrorInvalidValue;
-	lutmatch_mean_kernel<<<grid_for((size_t)p.beads * p.ptiles * p.mres), 256, 0, stream>>>(p, mean, raw_mean);
+	lutmatch_mean_kernel<<<grid_for((size_t)p.beads * p.ptiles * mres), 256, 0, stream>>>(p, raw_mean, mres);
 	hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return e;
 	const size_t nfrag = (size_t)p.beads * p.ptiles * p.ksteps * 64;
-	if (prec == kMatchBF16)
-		lutmatch_frag_kernel<__bf16><<<grid_for(nfrag), 256, 0, stream>>>(p, frag, raw_mean);
-	else
-		lutmatch_frag_kernel<_Float16><<<grid_for(nfrag), 256, 0, stream>>>(p, frag, raw_mean);
+	const size_t nmfrag = (size_t)p.beads * p.mtiles * p.ksteps * 64;
+	if (prec == kMatchBF16) {
+		lutmatch_frag_kernel<__bf16><<<grid_for(nfrag), 256, 0, stream>>>(p, frag, raw_mean, mres);
+		lutmatch_meanfrag_kernel<__bf16><<<grid_for(nmfrag), 256, 0, stream>>>(p, mfrag, raw_mean, mres);
+	} else {
+		lutmatch_frag_kernel<_Float16><<<grid_for(nfrag), 256, 0, stream>>>(p, frag, raw_mean, mres);
+		lutmatch_meanfrag_kernel<_Float16><<<grid_for(nmfrag), 256, 0, stream>>>(p, mfrag, raw_mean, mres);
+	}
 	e = hipGetLastError();
 	if (e != hipSuccess) return e;
 	lutmatch_norm_kernel<<<grid_for((size_t)p.beads * p.ptiles * 32), 256, 0, stream>>>(p, norm);

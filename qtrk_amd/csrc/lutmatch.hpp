@@ -10,10 +10,12 @@
 //     LUT row read once per 16 profiles;
 //   * matrix-core path: -score = |p|^2_w - 2 p.L_w + |L|^2_w, and the cross
 //     term for all (profile, plane) pairs is computed on the matrix cores.
-//     Each tile of 32 planes is stored as its FP32 mean row M_t plus BF16/FP16
-//     residuals D = L - M_t: p.L = p.M_t (FP32, per profile and tile) +
-//     p.D (MFMA GEMM, FP32 accumulation).  Neighbouring planes of a LUT are
-//     similar, so |D| is a fraction of |L| and so is the rounding error of the
+//     Each tile of 32 planes is stored as its mean row M_t plus BF16/FP16
+//     residuals D = L - M_t: p.L = p.M_t (per profile and tile, from M_t and
+//     p split into high and low 16-bit parts, three MFMA products: about
+//     16 bits more than one part) + p.D (MFMA GEMM, FP32 accumulation).
+//     Neighbouring planes of a LUT are similar, so |D| is a fraction of |L|
+//     and so is the rounding error of the
 //     approximate score -- what decides whether the right planes reach the
 //     candidate list.  The 8 planes with the highest approximate score are
 //     rescored in the reference arithmetic, the best of them is the best plane,
@@ -41,8 +43,10 @@ struct LutMatchParams {
 	const float* lut;         // [beads][planes][res] float32
 	const uint4* frag;        // [beads][ptiles][ksteps][64 lanes] x 8 16-bit residuals (MFMA A-operand order)
 	const float* norm;        // [beads][ptiles * 32]: sum_r w_r^2 L^2 (float)
-	const float* mean;        // [beads][ptiles][mres]: w_r^2 * M_t[r], the weighted tile means (0 past res)
-	int mres;                 // tile-mean row stride: res rounded up to a multiple of 8
+	const uint4* mfrag;       // [beads][mtiles][ksteps][64 lanes] x 8 16-bit tile means (MFMA A-operand order):
+	                          // rows 0-15 of mean tile m hold the high parts of M_t for t = 16m ... 16m+15,
+	                          // rows 16-31 the low parts M_t - high (0 past planes and res)
+	int mtiles;               // ceil(ptiles / 16)
 	const float* w;           // [res] radial weights, or null
 	// profiles, grouped by bead: slots start[b] .. start[b+1]-1 belong to bead b;
 	// slot s reads profile row rowidx[s] (s when rowidx is null) of prof[rows][res]
@@ -57,11 +61,12 @@ struct LutMatchParams {
 	int* cand;                // [rows][8] candidate planes (matrix-core path), or null
 };
 
-// Splits the FP32 LUT into tile means (raw_mean: [beads][ptiles][mres]
-// unweighted; mean: weighted) and residual MFMA A-operand fragments of `prec`
-// (kMatchBF16 or kMatchFP16), and computes the weighted plane norms.
-hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, float* norm, float* mean,
-                                   float* raw_mean, hipStream_t stream);
+// Splits the FP32 LUT into tile means (raw_mean: [beads][ptiles][mres], mres = res
+// rounded up to 8) and residual MFMA A-operand fragments of `prec` (kMatchBF16 or
+// kMatchFP16), writes the means as high/low fragments (mfrag) of the same
+// precision, and computes the weighted plane norms.
+hipError_t launch_lutmatch_prepare(const LutMatchParams& p, int prec, uint4* frag, uint4* mfrag, float* norm,
+                                   float* raw_mean, int mres, hipStream_t stream);
 // One grouped match of all beads' profiles (prec: kMatchFP32 / BF16 / FP16);
 // max_per_bead bounds start[b+1] - start[b].  frame_tile: profiles per
 // matrix-core workgroup, 32 or 64 (0: 32, which with half the registers and
@@ -84,8 +89,8 @@ hipError_t launch_group_by_bead(const int* bead_idx, int stride, int n, int bead
 // LUT and weights, and per precision (built on first use) the tile means,
 // residual fragments and plane norms of the matrix-core path.
 struct LutMatchTables {
-	int beads = 0, planes = 0, res = 0, ptiles = 0, ksteps = 0, mres = 0, idx_bits = 0;
-	DeviceBuffer lut, w, norm, mean, raw_mean, frag[3];
+	int beads = 0, planes = 0, res = 0, ptiles = 0, ksteps = 0, mres = 0, mtiles = 0, idx_bits = 0;
+	DeviceBuffer lut, w, norm, raw_mean, frag[3], mfrag[3];
 	bool have_w = false, prepared[3] = {};
 
 	LutMatchTables(int beads, int planes, int res);

@@ -20,6 +20,7 @@ LutMatchTables::LutMatchTables(int beads_, int planes_, int res_)
 	ptiles = (planes + 31) / 32;
 	ksteps = (res + 15) / 16;
 	mres = (res + 7) & ~7;
+	mtiles = (ptiles + 15) / 16;
 	idx_bits = 0;
 	while ((1 << idx_bits) < ptiles * 32) idx_bits++;
 	lut.ensure((size_t)beads * planes * res * sizeof(float));
@@ -42,10 +43,10 @@ void LutMatchTables::prepare(int prec, hipStream_t stream)
 	if (res > kMatchMaxRes) fail("the BF16/FP16 match supports at most 256 radial steps");
 	frag[prec].ensure((size_t)beads * ptiles * ksteps * 64 * 16);
 	norm.ensure((size_t)beads * ptiles * 32 * sizeof(float));
-	mean.ensure((size_t)beads * ptiles * mres * sizeof(float));
+	mfrag[prec].ensure((size_t)beads * mtiles * ksteps * 64 * 16);
 	raw_mean.ensure((size_t)beads * ptiles * mres * sizeof(float));
-	HIP_CHECK(launch_lutmatch_prepare(params(prec), prec, frag[prec].as<uint4>(), norm.as<float>(), mean.as<float>(),
-	                                  raw_mean.as<float>(), stream));
+	HIP_CHECK(launch_lutmatch_prepare(params(prec), prec, frag[prec].as<uint4>(), mfrag[prec].as<uint4>(),
+	                                  norm.as<float>(), raw_mean.as<float>(), mres, stream));
 	prepared[prec] = true;
 }
 
@@ -61,8 +62,8 @@ LutMatchParams LutMatchTables::params(int prec) const
 	p.lut = lut.as<float>();
 	p.frag = prec > 0 ? frag[prec].as<uint4>() : nullptr;
 	p.norm = norm.as<float>();
-	p.mean = mean.as<float>();
-	p.mres = mres;
+	p.mfrag = prec > 0 ? mfrag[prec].as<uint4>() : nullptr;
+	p.mtiles = mtiles;
 	p.w = have_w ? w.as<float>() : nullptr;
 	return p;
 }
