@@ -1293,6 +1293,28 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		const bool has_lut = !build && P.zluts && bead >= 0 && bead < P.zcount && P.zplanes > 0;
 		const float* L0 = has_lut ? P.zluts + (size_t)bead * P.zplanes * nb : nullptr;
 		const int passes = (EXT && mode_align(P) && has_lut) ? 5 : 1;
+		// The LUT scores read the bead's LUT through LDS tiles in the image's place (below);
+		// its first NA * kBlock 16-byte chunks load into registers here, so that they arrive
+		// while the radial profile is sampled.  Only in the instances for ROIs of 96 px and
+		// more, whose LDS holds a CU to three ROIs or fewer: there the 19 more VGPRs cost no
+		// wavefronts (smaller ROIs fit five per CU, and five wavefronts per SIMD need <= 96
+		// VGPRs; the variant with the optional modes has registers to spare neither).
+		const int pitchL = nb + 1;
+		const int tmax = min(kBlock, (W * H * (int)sizeof(SPix)) / (pitchL * 4));
+		const bool tiled = has_lut && !(EXT && mode_align(P)) && (nb & 3) == 0 && ((uintptr_t)L0 & 15) == 0 &&
+		                   tmax >= 32;
+		constexpr int NA = (!EXT && CW >= 96) ? kLutAhead : 0;
+		const int qpr = nb >> 2;  // 16-byte chunks per LUT row
+		const float4* L4 = (const float4*)L0;
+		float4 lahead[NA > 0 ? NA : 1];
+		if (NA > 0 && tiled) {
+			const int nqall = P.zplanes * qpr;
+#pragma unroll
+			for (int u = 0; u < NA; u++) {
+				const int c = tid + u * kBlock;
+				if (c < nqall) lahead[u] = L4[c];
+			}
+		}
 		for (int pass = 0; pass < passes; pass++) {
 			if constexpr (EXT && std::is_same<SPix, float>::value) {
 				if (fourier) power_spectrum_phase(P, L, lds, img, tid, lane, wid);
@@ -1390,39 +1412,40 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				// ---- LUT scores, best plane, FP64 peak fit (cputrack/cpu_tracker.cpp:791-868)
 				// One thread per plane sums its bins in order.  Unless QuadrantAlign samples the
 				// ROI again, the LUT goes through LDS tiles in the image's place: coalesced
-				// 16-byte loads, rows padded to nb + 1 floats so that the threads' reads of a
-				// bin fall in different banks.
-				const int pitchL = nb + 1;
-				const int tmax = min(kBlock, (W * H * (int)sizeof(SPix)) / (pitchL * 4));
-				const bool tiled = !(EXT && mode_align(P)) && (nb & 3) == 0 && ((uintptr_t)L0 & 15) == 0 &&
-				                   tmax >= 32;
+				// 16-byte loads (the first ones loaded ahead, above), rows padded to nb + 1 floats
+				// so that the threads' reads of a bin fall in different banks.
 				if (tiled) {
 					float* tile = (float*)(lds + L.img);
 					const int ntile = (P.zplanes + tmax - 1) / tmax;
 					const int tp = (P.zplanes + ntile - 1) / ntile;  // planes per tile, balanced
-					const int qpr = nb >> 2;                         // 16-byte chunks per row
+					auto put = [&](int cr, const float4& v) {   // chunk cr of the tile
+						const int kk = cr / qpr;
+						float* d = tile + kk * pitchL + ((cr - kk * qpr) << 2);
+						d[0] = v.x;
+						d[1] = v.y;
+						d[2] = v.z;
+						d[3] = v.w;
+					};
 					for (int k0 = 0; k0 < P.zplanes; k0 += tp) {
 						const int np = min(tp, P.zplanes - k0);
-						const float4* src = (const float4*)(L0 + (size_t)k0 * nb);
-						const int nq = np * qpr;
-						for (int c0 = tid; c0 < nq; c0 += 4 * kBlock) {
+						const int cb = k0 * qpr, ce = cb + np * qpr;  // the tile's chunks
+#pragma unroll
+						for (int u = 0; u < NA; u++) {
+							const int c = tid + u * kBlock;
+							if (c >= cb && c < ce) put(c - cb, lahead[u]);
+						}
+						// chunks past those loaded ahead, 4 loads in flight
+						for (int c0 = max(cb, NA * kBlock) + tid; c0 < ce; c0 += 4 * kBlock) {
 							float4 v[4];
 #pragma unroll
 							for (int u = 0; u < 4; u++) {
 								const int c = c0 + u * kBlock;
-								v[u] = src[c < nq ? c : nq - 1];
+								v[u] = L4[c < ce ? c : ce - 1];
 							}
 #pragma unroll
 							for (int u = 0; u < 4; u++) {
 								const int c = c0 + u * kBlock;
-								if (c < nq) {
-									const int kk = c / qpr;
-									float* d = tile + kk * pitchL + ((c - kk * qpr) << 2);
-									d[0] = v[u].x;
-									d[1] = v[u].y;
-									d[2] = v[u].z;
-									d[3] = v[u].w;
-								}
+								if (c < ce) put(c - cb, v[u]);
 							}
 						}
 						__syncthreads();

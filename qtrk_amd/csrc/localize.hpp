@@ -27,6 +27,10 @@ static_assert(sizeof(DevResult) == 52, "result layout");
 
 constexpr int kMaxFFTStages = 8;
 constexpr int kBlock = 256;  // threads per ROI workgroup (4 wavefronts)
+// 16-byte LUT chunks per thread loaded into registers before the radial profile is sampled: a
+// 200-plane x 64-step LUT whole (52 VGPRs; the workgroup's LDS already limits a CU to three ROIs,
+// i.e. three wavefronts per SIMD, which leaves that many registers unused)
+constexpr int kLutAhead = 13;
 
 enum : int {
 	kModeXCor1D = 1, kModeQI = 2, kModeGauss2D = 4, kModeZLUTAlign = 8, kModeLocalizeZ = 16,

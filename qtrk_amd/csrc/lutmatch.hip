@@ -226,8 +226,10 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 	int* cand_s = (int*)(smem + L.cand);
 	double* ex_s = (double*)(smem + L.ex);
 
-	// this wavefront's mean tile (mean tile m, profile tile ft for job m * FT + ft) loads while the
-	// profiles are staged
+	// this wavefront's mean tile (mean tile m, profile tile ft for job m * FT + ft) and, with up to
+	// 8 k-steps, its first LUT tile load while the profiles are staged (with 16 k-steps both
+	// operand sets would not fit in the registers together: the first tile loads after the
+	// tile-mean terms)
 	const int wu = __builtin_amdgcn_readfirstlane(wid);
 	const int MJ = P.mtiles * FT;
 	uint4 am[KSMAX];
@@ -237,6 +239,16 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 		for (int ks = 0; ks < KSMAX; ks++)
 			if (ks < KS) am[ks] = mb[(size_t)ks * 64];
 	}
+	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
+	uint4 a[KSMAX];
+	auto load_first_tile = [&]() {
+		if (wu < P.ptiles) {
+#pragma unroll
+			for (int ks = 0; ks < KSMAX; ks++)
+				if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
+		}
+	};
+	if constexpr (KSMAX <= 8) load_first_tile();
 	for (int i = tid; i < FT * 32; i += 256) row_s[i] = i < nf ? (P.rowidx ? P.rowidx[s0 + i] : s0 + i) : -1;
 	for (int k = tid; k < KS * 16; k += 256) {
 		const float wk = k < R ? (P.w ? P.w[k] : 1.0f) : 0.0f;
@@ -286,31 +298,13 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 		bq_s[i] = __builtin_bit_cast(uint4, v);
 		blo_s[i] = __builtin_bit_cast(uint4, vl);
 	}
-	// the first tile's operands load while the tile-mean terms are computed (after them with
-	// 16 k-steps, where both operand sets would not fit in the registers together)
-	const uint4* fb = P.frag + (size_t)bead * P.ptiles * KS * 64 + lane;
-	uint4 a[KSMAX];
-	auto load_first_tile = [&]() {
-		if (wu < P.ptiles) {
-#pragma unroll
-			for (int ks = 0; ks < KSMAX; ks++)
-				if (ks < KS) a[ks] = fb[((size_t)wu * KS + ks) * 64];
-		}
-	};
-	if constexpr (KSMAX <= 8) load_first_tile();
 	__syncthreads();
 	// the tile-mean term p.M_t of every (tile, profile) pair: mean tile m's rows r < 16 (high
 	// parts of M_{16m+r}) and 16 + r (low parts) against B and its remainder; lane (n, h) holds
 	// rows 8*(i/4) + 4*h + i%4 in accumulator i, so row r's high and low products are
 	// accumulators i and i + 8 of the same lane
-	for (int j = wu; j < MJ; j += 4) {
+	auto mean_job = [&](int j) {
 		const int ft = j % FT, mt = j / FT;
-		if (j != wu) {
-			const uint4* mb = P.mfrag + ((size_t)bead * P.mtiles + mt) * KS * 64 + lane;
-#pragma unroll
-			for (int ks = 0; ks < KSMAX; ks++)
-				if (ks < KS) am[ks] = mb[(size_t)ks * 64];
-		}
 		f32x16 hi, lo;
 #pragma unroll
 		for (int i = 0; i < 16; i++) {
@@ -330,6 +324,14 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 			const int t = mt * 16 + 8 * (i >> 2) + 4 * h + (i & 3);
 			if (t < P.ptiles) cm_s[t * F + ft * 32 + n] = hi[i] + ((hi[i + 8] + lo[i]) + lo[i + 8]);
 		}
+	};
+	if (wu < MJ) mean_job(wu);
+	for (int j = wu + 4; j < MJ; j += 4) {  // more than 4 jobs: the operands load here
+		const uint4* mb = P.mfrag + ((size_t)bead * P.mtiles + j / FT) * KS * 64 + lane;
+#pragma unroll
+		for (int ks = 0; ks < KSMAX; ks++)
+			if (ks < KS) am[ks] = mb[(size_t)ks * 64];
+		mean_job(j);
 	}
 	if constexpr (KSMAX > 8) load_first_tile();
 	__syncthreads();
