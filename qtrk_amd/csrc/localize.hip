@@ -49,6 +49,7 @@ struct LdsLayout {
 	int fbuf, fcol, ftmp, ftwx, ftwy, fpx, fpy;     // 2D power-spectrum phase
 	int scores, zw;                               // LUT phase (zw: adjusted weights)
 	int gcol;                                     // 2D Gaussian MLE: per-column and per-row terms
+	int lutfull;                                  // 1: the whole LUT fits over the image and phases
 	int total;
 };
 
@@ -94,10 +95,9 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 	LdsLayout L{};
 	int o = 0;
 	const int N = P.qi_fft.n > 0 ? P.qi_fft.n : 0;
+	// the image, then the phases' scratch (one region, reused phase by phase), then the radial
+	// profile and the reduction scratch, which live through the LUT phase
 	L.img = o;    o += align16(P.width * P.height * lds_pix_bytes(P));
-	L.zprof = o;  o += align16(P.z_nb * 4);
-	L.red = o;    o += align16(64 * 8);
-	L.st = o;     o += align16(16 * 4);
 	const int u = o;
 	// QI phase: directions, quadrant profiles, 4 FFT buffers, twiddles, leaf order
 	int q = u;
@@ -163,6 +163,23 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 	o = o > a ? o : a;
 	o = o > f ? o : f;
 	o = o > g ? o : g;
+	// The LUT phase scores every plane in one pass when the bead's whole LUT (rows padded to
+	// z_nb + 1 floats) fits over the image and the phase scratch, both dead by then, at one
+	// plane per thread -- and when any growth of the region this takes leaves as many
+	// workgroups per CU; otherwise it goes through the image's space in tiles.
+	const int tail = align16(P.z_nb * 4) + align16(64 * 8) + align16(16 * 4);
+	const int need = P.zplanes * (P.z_nb + 1) * 4;
+	L.lutfull = 0;
+	if (!P.build && P.zluts && P.zplanes > 0 && P.zplanes <= kBlock && (P.z_nb & 3) == 0) {
+		const int grown = align16(o > need ? o : need);
+		if (kLdsPerCU / (grown + tail) >= kLdsPerCU / (o + tail)) {
+			L.lutfull = 1;
+			o = grown;
+		}
+	}
+	L.zprof = o;  o += align16(P.z_nb * 4);
+	L.red = o;    o += align16(64 * 8);
+	L.st = o;     o += align16(16 * 4);
 	L.total = o;
 	return L;
 }
@@ -572,6 +589,105 @@ __device__ __forceinline__ void normalize_profile(float* p, int n)
 	}
 	const double inv = (double)1.0f / sqrt(s2 / (double)n);
 	for (int i = 0; i < n; i++) p[i] = (float)((double)p[i] * inv);
+}
+
+// Whether a double sum of the floats v (n of them, lg >= log2 n) is exact in any order: every
+// partial sum is a multiple of the smallest ulp among them and below n times twice the largest,
+// which fits a double's 53 bits when the exponents span at most 29 - lg (one bit of margin
+// here).  emin/emax: biased exponents of the nonzero values (denormals count as 1); fin: all
+// finite.  Then a tree sum equals the reference's sequential one bit for bit.
+__device__ __forceinline__ bool exact_any_order(int emin, int emax, bool fin, int lg)
+{
+	return fin && (emax < emin || emax - emin <= 28 - lg);
+}
+__device__ __forceinline__ void exp_range(float v, int& emin, int& emax, bool& fin)
+{
+	const int e = (int)((__float_as_uint(v) >> 23) & 255);
+	fin = fin && e != 255;
+	if (v != 0.0f) {
+		emin = min(emin, e > 0 ? e : 1);
+		emax = max(emax, e);
+	}
+}
+__device__ __forceinline__ double wave_sum(double v)
+{
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+	return v;
+}
+__device__ __forceinline__ int wave_min(int v)
+{
+	for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off));
+	return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+	for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+	return v;
+}
+
+// NormalizeRadialProfile by one wavefront, bit-identical to normalize_profile: the mean's double
+// sum as a tree when exact_any_order holds (else lane 0 adds in index order), the deviations
+// and their float squares per lane, the squares' double sum likewise, the scaling per lane.
+__device__ __forceinline__ void normalize_profile_wave(float* p, int n, int lane)
+{
+	int lg = 0;
+	while ((1 << lg) < n) lg++;
+	int emin = 255, emax = 0;
+	bool fin = true;
+	double part = 0.0;
+	for (int i = lane; i < n; i += 64) {
+		const float v = p[i];
+		exp_range(v, emin, emax, fin);
+		part += (double)v;
+	}
+	emin = wave_min(emin);
+	emax = wave_max(emax);
+	fin = __all(fin);
+	double sum;
+	if (exact_any_order(emin, emax, fin, lg)) {
+		sum = wave_sum(part);
+	} else {
+		sum = 0.0;
+		if (lane == 0)
+			for (int i = 0; i < n; i++) sum += (double)p[i];
+		sum = __shfl(sum, 0);
+	}
+	const float pm = (float)(sum / (double)n);
+	emin = 255;
+	emax = 0;
+	fin = true;
+	part = 0.0;
+	for (int i = lane; i < n; i += 64) {
+		const float d = p[i] - pm;
+		const float q = d * d;
+		p[i] = d;
+		exp_range(q, emin, emax, fin);
+		part += (double)q;
+	}
+	emin = wave_min(emin);
+	emax = wave_max(emax);
+	fin = __all(fin);
+	double s2;
+	if (exact_any_order(emin, emax, fin, lg)) {
+		s2 = wave_sum(part);
+	} else {
+		wave_sync();
+		s2 = 0.0;
+		if (lane == 0) {
+			constexpr int K = 8;
+			for (int i0 = 0; i0 < n; i0 += K) {
+				float v[K];
+#pragma unroll
+				for (int j = 0; j < K; j++) v[j] = p[min(i0 + j, n - 1)];
+#pragma unroll
+				for (int j = 0; j < K; j++)
+					if (i0 + j < n) s2 += (double)(float)(v[j] * v[j]);
+			}
+		}
+		s2 = __shfl(s2, 0);
+	}
+	const double inv = (double)1.0f / sqrt(s2 / (double)n);
+	for (int i = lane; i < n; i += 64) p[i] = (float)((double)p[i] * inv);
 }
 
 // QI profile concatenations (cputrack/cpu_tracker.cpp:327-343):
@@ -1301,8 +1417,9 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		// VGPRs; the variant with the optional modes has registers to spare neither).
 		const int pitchL = nb + 1;
 		const int tmax = min(kBlock, (W * H * (int)sizeof(SPix)) / (pitchL * 4));
+		const bool lutfull = !EXT && L.lutfull;  // all planes in one tile (lds_layout)
 		const bool tiled = has_lut && !(EXT && mode_align(P)) && (nb & 3) == 0 && ((uintptr_t)L0 & 15) == 0 &&
-		                   tmax >= 32;
+		                   (tmax >= 32 || lutfull);
 		constexpr int NA = (!EXT && CW >= 96) ? kLutAhead : 0;
 		const int qpr = nb >> 2;  // 16-byte chunks per LUT row
 		const float4* L4 = (const float4*)L0;
@@ -1389,7 +1506,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			}
 			__syncthreads();
 			if (!fourier) {
-				if ((P.mode & kModeNormalizeProfile) && tid == 0) normalize_profile(zprof, nb);
+				if ((P.mode & kModeNormalizeProfile) && wid == 0) normalize_profile_wave(zprof, nb, lane);
 				__syncthreads();
 				boundary = (cx + P.z_maxr >= (float)W) || (cx - P.z_maxr < 0.0f) || (cy + P.z_maxr >= (float)H) ||
 				           (cy - P.z_maxr < 0.0f);
@@ -1411,12 +1528,13 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			} else if (has_lut) {
 				// ---- LUT scores, best plane, FP64 peak fit (cputrack/cpu_tracker.cpp:791-868)
 				// One thread per plane sums its bins in order.  Unless QuadrantAlign samples the
-				// ROI again, the LUT goes through LDS tiles in the image's place: coalesced
-				// 16-byte loads (the first ones loaded ahead, above), rows padded to nb + 1 floats
-				// so that the threads' reads of a bin fall in different banks.
+				// ROI again, the LUT goes through LDS tiles in the image's place (one tile over
+				// the image and the phase scratch when it fits: lutfull): coalesced 16-byte loads
+				// (the first ones loaded ahead, above), rows padded to nb + 1 floats so that the
+				// threads' reads of a bin fall in different banks.
 				if (tiled) {
 					float* tile = (float*)(lds + L.img);
-					const int ntile = (P.zplanes + tmax - 1) / tmax;
+					const int ntile = lutfull ? 1 : (P.zplanes + tmax - 1) / tmax;
 					const int tp = (P.zplanes + ntile - 1) / ntile;  // planes per tile, balanced
 					auto put = [&](int cr, const float4& v) {   // chunk cr of the tile
 						const int kk = cr / qpr;
@@ -1449,10 +1567,10 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 							}
 						}
 						__syncthreads();
+						double s = 0.0;
 						if (tid < np) {
 							// bins read 4 at a time ahead of the serial sum (nb is a multiple of 4)
 							const float* row = tile + tid * pitchL;
-							double s = 0.0;
 							for (int r0 = 0; r0 < nb; r0 += 4) {
 								float lv[4], pv[4];
 #pragma unroll
@@ -1466,6 +1584,9 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 									s = zcmp_step(s, d, P.zcmp, r0 + j);
 								}
 							}
+						}
+						if (lutfull) __syncthreads();  // the LUT is read: the scores go over it
+						if (tid < np) {
 							scores[k0 + tid] = s;
 							if (P.cmp_out) P.cmp_out[(size_t)bead * P.zplanes + k0 + tid] = (float)s;
 						}

@@ -27,6 +27,7 @@ static_assert(sizeof(DevResult) == 52, "result layout");
 
 constexpr int kMaxFFTStages = 8;
 constexpr int kBlock = 256;  // threads per ROI workgroup (4 wavefronts)
+constexpr int kLdsPerCU = 160 * 1024;  // LDS bytes of a gfx950 CU
 // 16-byte LUT chunks per thread loaded into registers before the radial profile is sampled: a
 // 200-plane x 64-step LUT whole (52 VGPRs; the workgroup's LDS already limits a CU to three ROIs,
 // i.e. three wavefronts per SIMD, which leaves that many registers unused)
