@@ -207,7 +207,8 @@ def precision_bench(a):
             win = best[..., None] + np.arange(-3, 4)
             rows = np.concatenate([cand, np.clip(win, 0, planes - 1)], axis=-1).reshape(beads, -1)
             distinct = sum(len(np.unique(rr[rr >= 0])) for rr in rows)
-            bytes_ = (beads * ptiles * ksteps * 64 * 16 + beads * ptiles * 128 * 4 + beads * ptiles * 32 * 4
+            mtiles = (ptiles + 15) // 16  # tile-mean fragments: 16 tiles' high and low parts per 32 rows
+            bytes_ = (beads * ptiles * ksteps * 64 * 16 + beads * mtiles * ksteps * 64 * 16 + beads * ptiles * 32 * 4
                       + prof.nbytes + n * (4 + 4 + 32) + distinct * 128 * 4)
             r["mfma_tflops"] = round(2.0 * n * planes * 128 / kern_s / 1e12, 2)
             r["mfma_frac_of_dense_peak"] = round(2.0 * n * planes * 128 / kern_s / 2.5e15, 5)
