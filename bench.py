@@ -706,10 +706,11 @@ def mixed_bench(a):
     done two ways, on the default 200 x 64 LUTs and on 500 x 128 LUTs (the precision workload's size):
       * fused FP32: one localize_kernel per batch, each ROI's LUT scores in the reference
         arithmetic (FP64 sums) inside its workgroup -- the headline path;
-      * split BF16 / FP16: the device batch with QTrkBatchSetMatchPrecision -- the kernel in
-        profile-only form, the ROIs grouped by bead on the GPU, then each bead's 16 profiles matched
-        together on the matrix cores, the 8 best planes and the fit window rescored in the reference
-        arithmetic, FP64 fit.
+      * split FP32: the kernel in profile-only form, the ROIs grouped by bead on the GPU, then each
+        bead's 16 profiles scored against every plane together in the reference arithmetic
+        (QTRK_MATCH_FP32_GROUPED; results identical to the fused form's);
+      * split BF16 / FP16: the same grouping, each bead's profiles matched together on the matrix
+        cores, the 8 best planes and the fit window rescored in the reference arithmetic, FP64 fit.
     Per form: loc/s over consecutive batches, and the error against the truth (bias = mean,
     sigma = standard deviation; x, y in px, z in planes) with the count of results identical to
     the fused form's."""
@@ -725,8 +726,8 @@ def mixed_bench(a):
         b = api.DeviceBatch(trk, n, api.QTrkU16)
         b.upload(w["rois"], w["jobs"])
         forms, fused = {}, None
-        for name, prec in (("fused_fp32", api.MATCH_FP32), ("split_bf16", api.MATCH_BF16),
-                           ("split_fp16", api.MATCH_FP16)):
+        for name, prec in (("fused_fp32", api.MATCH_FP32), ("split_fp32", api.MATCH_FP32_GROUPED),
+                           ("split_bf16", api.MATCH_BF16), ("split_fp16", api.MATCH_FP16)):
             b.set_match_precision(prec)
             b.run(n, a.warmup)
             b.sync()
@@ -832,24 +833,30 @@ def localize_bench(a, name):
                                          "peak": VALU_PEAK_WAVE_INSTR / 1e9, "unit": "G wave-instr/s",
                                          "frac": round(valu / kern_s / VALU_PEAK_WAVE_INSTR, 4)}
     if realtime:
-        # the mixed-precision batched form on the same batch: profile kernel + grouped BF16 match
-        # (QTrkBatchSetMatchPrecision), consecutive batches as above, against the FP32 results
-        b.set_match_precision(api.MATCH_BF16)
-        b.run(n, a.warmup)
-        b.sync()
-        t0 = time.perf_counter()
-        b.run(n, a.steps)
-        b.sync()
-        mwall = time.perf_counter() - t0
-        mres = b.download(n)
-        b.set_match_precision(api.MATCH_FP32)
-        same = int(np.sum((mres["x"] == res["x"]) & (mres["y"] == res["y"]) & (mres["z"] == res["z"])))
-        out["mixed_precision"] = {"form": "profile kernel + grouped BF16 matrix-core Z-LUT match, FP32 top-8 "
-                                          "rescoring, FP64 fit",
-                                  "value": round(n * a.steps / mwall, 1), "unit": "localizations/s",
-                                  "ms_per_step": round(mwall / a.steps * 1e3, 4),
-                                  "speedup_vs_fp32": round(wall / mwall, 4),
-                                  "results_identical_to_fp32": same, "rois": n}
+        # the batched forms on the same batch (QTrkBatchSetMatchPrecision), consecutive batches as
+        # above, against the per-ROI FP32 results: the profile kernel, the ROIs grouped by bead on
+        # the GPU, then each bead's profiles matched together -- in the reference arithmetic
+        # (FP32 grouped: the plain-FP32 results, bit for bit) or on the matrix cores in BF16 with
+        # the FP32 rescoring of the top 8 (mixed precision)
+        def batched(prec):
+            b.set_match_precision(prec)
+            b.run(n, a.warmup)
+            b.sync()
+            t0 = time.perf_counter()
+            b.run(n, a.steps)
+            b.sync()
+            bw = time.perf_counter() - t0
+            r = b.download(n)
+            b.set_match_precision(api.MATCH_FP32)
+            same = int(np.sum((r["x"] == res["x"]) & (r["y"] == res["y"]) & (r["z"] == res["z"])))
+            return {"value": round(n * a.steps / bw, 1), "unit": "localizations/s",
+                    "ms_per_step": round(bw / a.steps * 1e3, 4), "speedup_vs_fp32": round(wall / bw, 4),
+                    "results_identical_to_fp32": same, "rois": n}
+
+        out["fp32_grouped"] = dict({"form": "profile kernel + grouped Z-LUT match in the reference arithmetic "
+                                            "(QTRK_MATCH_FP32_GROUPED)"}, **batched(api.MATCH_FP32_GROUPED))
+        out["mixed_precision"] = dict({"form": "profile kernel + grouped BF16 matrix-core Z-LUT match, FP32 top-8 "
+                                               "rescoring, FP64 fit"}, **batched(api.MATCH_BF16))
     b.free()
     if realtime and n == 4096 and not a.no_frames:
         # the same batch from host memory through the drop-in C ABI (PCIe included): reported

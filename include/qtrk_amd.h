@@ -78,7 +78,10 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats);
  * localization in profile-only form, group the ROIs by bead (job.zlutIndex) on the GPU and
  * match each bead's profiles together as QTrkLUTMatch does (matrix-core scores, the 8 best
  * planes and the fit window rescored in the reference arithmetic, FP64 fit): results equal the
- * FP32 form's whenever the exact best plane is among those 8.  Not with LT_ZLUTAlign,
+ * FP32 form's whenever the exact best plane is among those 8.  QTRK_MATCH_FP32_GROUPED (3)
+ * groups the same way and scores every plane in the reference arithmetic (QTrkLUTMatch's
+ * QTRK_MATCH_FP32): results identical to QTRK_MATCH_FP32's, each bead's LUT rows read once
+ * per 16 of its ROIs instead of once per ROI.  Not with LT_ZLUTAlign,
  * LT_LocalizeZWeighted or LT_FourierLUT, and the compare profile
  * (QTrkEnableRadialZLUTCompareProfile) is not recorded.  The first call converts the
  * tracker's LUT (again after the LUT or its weights change).  0 on success. */
@@ -174,6 +177,8 @@ QTRK_API void QTrkGenerateImagesFromLUT(float* out, int n, int width, int height
 #define QTRK_MATCH_FP32 0
 #define QTRK_MATCH_BF16 1
 #define QTRK_MATCH_FP16 2
+/* device batches only (QTrkBatchSetMatchPrecision, QTrkLocalizeDeviceNoAlloc) */
+#define QTRK_MATCH_FP32_GROUPED 3
 
 typedef struct QTrkLUTMatch QTrkLUTMatch;
 

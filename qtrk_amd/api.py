@@ -236,6 +236,7 @@ SIGNATURES = {
 }
 
 MATCH_FP32, MATCH_BF16, MATCH_FP16 = 0, 1, 2
+MATCH_FP32_GROUPED = 3  # device batches: profile kernel + grouped match in the reference arithmetic
 NOALLOC_SCRATCH_MB = 32  # QTRK_NOALLOC_SCRATCH_MB (include/qtrk_amd.h)
 
 _lib = None

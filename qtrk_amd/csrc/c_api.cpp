@@ -369,7 +369,8 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			// each stream with its own scratch (diagnostics: one stream, their buffers)
 			const std::string why = b->trk->SplitMatchUsable();
 			if (!why.empty()) fail("the grouped Z-LUT match cannot run: " + why);
-			const LutMatchTables* tab = b->trk->PrepareMatchTables(b->prec);
+			const int mp = match_kernel_precision(b->prec);
+			const LutMatchTables* tab = b->trk->PrepareMatchTables(mp);
 			const int nb = b->trk->cfg.zlut_radialsteps;
 			const size_t bytes = QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, nb, tab->beads);
 			const bool two = repeats > 1 && !b->diag;
@@ -388,7 +389,7 @@ QTRK_API int QTrkBatchRun(QTrkBatch* b, int n, int repeats)
 			if (two) HIP_CHECK(hipStreamWaitEvent(b->stream2, b->e0, 0));
 			for (int r = 0; r < repeats; r++) {
 				const int k = two ? (r & 1) : 0;
-				b->trk->LocalizeSplit(b->rois.p, b->pdt, b->jobs.as<DevJob>(), b->results.as<DevResult>(), n, b->prec,
+				b->trk->LocalizeSplit(b->rois.p, b->pdt, b->jobs.as<DevJob>(), b->results.as<DevResult>(), n, mp,
 				                      tab, sc[k], b->diag ? b->best.as<int>() : nullptr, k ? b->stream2 : b->stream,
 				                      nullptr, 0, mpb == 0 ? 1 : mpb);
 			}
@@ -418,12 +419,12 @@ QTRK_API int QTrkBatchSetMatchPrecision(QTrkBatch* b, int precision)
 {
 	bool ok = GUARD({
 		if (!b) fail("null batch");
-		if (precision < kMatchFP32 || precision > kMatchFP16) fail("unknown match precision");
+		if (precision < kMatchFP32 || precision > kMatchFP32Grouped) fail("unknown match precision");
 		HIP_CHECK(hipSetDevice(b->trk->Device()));
 		if (precision != kMatchFP32) {
 			const std::string why = b->trk->SplitMatchUsable();
 			if (!why.empty()) fail("the grouped Z-LUT match cannot run: " + why);
-			const LutMatchTables* tab = b->trk->PrepareMatchTables(precision);
+			const LutMatchTables* tab = b->trk->PrepareMatchTables(match_kernel_precision(precision));
 			b->split.ensure(QueuedHIPTracker::SplitScratch::Bytes(b->maxROIs, b->trk->cfg.zlut_radialsteps, tab->beads));
 		}
 		b->prec = precision;

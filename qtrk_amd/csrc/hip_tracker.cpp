@@ -487,14 +487,16 @@ void QueuedHIPTracker::Launch(Slot* s)
 	}
 	HIP_CHECK(hipEventRecord(s->ev_k0, s->stream));
 	if (matchPrec != kMatchFP32 && SplitMatchUsable().empty()) {
-		// match_precision bf16/fp16: profile kernel + grouped match (modes it cannot take run fused)
-		const LutMatchTables* tab = MatchTablesLocked(matchPrec, s->stream);
+		// match_precision bf16/fp16/fp32_grouped: profile kernel + grouped match (modes it cannot
+		// take run fused)
+		const int mp = match_kernel_precision(matchPrec);
+		const LutMatchTables* tab = MatchTablesLocked(mp, s->stream);
 		const int nb = cfg.zlut_radialsteps;
 		s->d_split.ensure(SplitScratch::Bytes(batchSize, nb, tab->beads));
 		const SplitScratch sc = SplitScratch::Carve(s->d_split.p, batchSize, nb, tab->beads);
 		const int mpb = MaxPerBead(s->h_jobs.as<DevJob>(), s->n, tab->beads);
 		if (mpb > 0)
-			LocalizeSplit(P.src, P.pdt, P.jobs, P.results, s->n, matchPrec, tab, sc, nullptr, s->stream, P.roi_offset,
+			LocalizeSplit(P.src, P.pdt, P.jobs, P.results, s->n, mp, tab, sc, nullptr, s->stream, P.roi_offset,
 			              P.pitch, mpb);
 		else
 			HIP_CHECK(launch_localize(P, s->stream));  // no ROI has a bead in the LUT: nothing to match
@@ -1234,8 +1236,12 @@ void QueuedHIPTracker::SetConfigValue(std::string name, std::string value)
 		else numSlots = v;
 		AllocSlots();
 	} else if (name == "match_precision") {
-		int v = value == "fp32" ? kMatchFP32 : value == "bf16" ? kMatchBF16 : value == "fp16" ? kMatchFP16 : -1;
-		if (v < 0) fail("match_precision must be fp32, bf16 or fp16");
+		int v = value == "fp32"           ? kMatchFP32
+		        : value == "bf16"         ? kMatchBF16
+		        : value == "fp16"         ? kMatchFP16
+		        : value == "fp32_grouped" ? kMatchFP32Grouped
+		                                  : -1;
+		if (v < 0) fail("match_precision must be fp32, bf16, fp16 or fp32_grouped");
 		std::unique_lock<std::mutex> lk(mtx);
 		HIP_CHECK(hipSetDevice(dev));
 		if (filling && filling->n > 0) {  // ROIs queued before go with the old setting
@@ -1265,7 +1271,10 @@ QueuedTracker::ConfigValueMap QueuedHIPTracker::GetConfigValues()
 	m["num_streams"] = std::to_string(numSlots);
 	m["frame_copy_threads"] = std::to_string(frameCopyThreads);
 	m["queue_spin_us"] = std::to_string((long long)spinWait.count());
-	m["match_precision"] = matchPrec == kMatchBF16 ? "bf16" : matchPrec == kMatchFP16 ? "fp16" : "fp32";
+	m["match_precision"] = matchPrec == kMatchBF16          ? "bf16"
+	                       : matchPrec == kMatchFP16        ? "fp16"
+	                       : matchPrec == kMatchFP32Grouped ? "fp32_grouped"
+	                                                        : "fp32";
 	return m;
 }
 

@@ -32,6 +32,11 @@
 namespace qtrk {
 
 enum : int { kMatchFP32 = 0, kMatchBF16 = 1, kMatchFP16 = 2 };
+// A device batch's fourth form: the profile kernel and the grouped match in kMatchFP32 (the
+// grouped match itself knows only the three precisions above).
+constexpr int kMatchFP32Grouped = 3;
+// the grouped match's precision for a batch form
+inline int match_kernel_precision(int form) { return form == kMatchFP32Grouped ? kMatchFP32 : form; }
 constexpr int kMatchTopK = 8;
 constexpr int kMatchMaxRes = 256;     // radial steps supported by the matrix-core path
 constexpr int kMatchExactTile = 16;   // profiles per workgroup, exact path

@@ -43,7 +43,7 @@ QTRK_API int QTrkLocalizeDeviceNoAlloc(QueuedTracker* qtrk, const void* rois, QT
 		if (!t) fail("not a MI355X tracker");
 		if (n < 0) fail("n must be >= 0");
 		if ((int)pdt < 0 || (int)pdt > 2) fail("invalid pixel data type");
-		if (precision < kMatchFP32 || precision > kMatchFP16) fail("unknown match precision");
+		if (precision < kMatchFP32 || precision > kMatchFP32Grouped) fail("unknown match precision");
 		if (n > 0 && (!rois || !jobs || !results)) fail("null buffer");
 		HIP_CHECK(hipSetDevice(t->Device()));
 		const hipStream_t s = (hipStream_t)stream;
@@ -60,7 +60,8 @@ QTRK_API int QTrkLocalizeDeviceNoAlloc(QueuedTracker* qtrk, const void* rois, QT
 		}
 		const std::string why = t->SplitMatchUsable();
 		if (!why.empty()) fail("the grouped Z-LUT match cannot run: " + why);
-		const LutMatchTables* tab = t->CurrentMatchTables(precision);
+		const int mp = match_kernel_precision(precision);
+		const LutMatchTables* tab = t->CurrentMatchTables(mp);
 		if (!tab)
 			fail("the tracker's LUT is not converted for this match precision (or changed since): call "
 			     "QTrkBatchSetMatchPrecision on a batch of this tracker first");
@@ -70,7 +71,7 @@ QTRK_API int QTrkLocalizeDeviceNoAlloc(QueuedTracker* qtrk, const void* rois, QT
 			     std::to_string(kNoAllocScratchBytes) + " (" + std::to_string(QTRK_NOALLOC_SCRATCH_MB) + " MiB)");
 		const QueuedHIPTracker::SplitScratch sc =
 		    QueuedHIPTracker::SplitScratch::Carve(base, n, t->cfg.zlut_radialsteps, tab->beads);
-		t->LocalizeSplit(rois, (int)pdt, dj, dr, n, precision, tab, sc, nullptr, s);
+		t->LocalizeSplit(rois, (int)pdt, dj, dr, n, mp, tab, sc, nullptr, s);
 	});
 	return ok ? 0 : -1;
 }

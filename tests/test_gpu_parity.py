@@ -127,12 +127,15 @@ def test_batching_tracker_sizes_parity(gpu, oracle, spec):
     _assert_parity(got, ref, best, rbest)
 
 
-@pytest.mark.parametrize("prec", [api.MATCH_BF16, api.MATCH_FP16], ids=["bf16", "fp16"])
+@pytest.mark.parametrize("prec", [api.MATCH_BF16, api.MATCH_FP16, api.MATCH_FP32_GROUPED],
+                         ids=["bf16", "fp16", "fp32_grouped"])
 def test_batch_grouped_match_parity(gpu, oracle, prec):
     """A device batch with the grouped Z-LUT match (QTrkBatchSetMatchPrecision): the localization
     kernel in profile-only form, the ROIs grouped by bead on the GPU, each bead's profiles matched
-    together on the matrix cores with the 8 best planes and the fit window rescored in the
-    reference arithmetic.  Against the oracle's ProcessJob: indexing and best plane bit-exact, x/y/z
+    together -- on the matrix cores with the 8 best planes and the fit window rescored in the
+    reference arithmetic (bf16, fp16), or every plane in the reference arithmetic (fp32_grouped,
+    whose results must also equal the fused FP32 form's byte for byte).  Against the oracle's
+    ProcessJob: indexing and best plane bit-exact, x/y/z
     within the bars, profiles equal, with one bead holding 200 of the 1,024 ROIs (its workgroups
     loop over several profile tiles) and ROIs of a bead outside the LUT (z 0, best plane -1, as in
     the per-ROI form); after SetRadialWeights the next run matches with the new weights."""
@@ -157,22 +160,33 @@ def test_batch_grouped_match_parity(gpu, oracle, prec):
         _assert_parity(got, ref, best, rbest)
         np.testing.assert_allclose(prof, rprof, atol=1e-5)
         assert np.all(got["z"][200:210] == 0) and np.all(best[200:210] == -1)
+        if prec == api.MATCH_FP32_GROUPED:
+            b.set_match_precision(api.MATCH_FP32)
+            b.run(1024, 2)
+            b.sync()
+            fused = b.download(1024)
+            fbest, _ = b.diagnostics(1024)
+            b.set_match_precision(prec)
+            assert np.array_equal(got.view(np.uint8), fused.view(np.uint8))
+            assert np.array_equal(best, fbest)
     b.free()
     trk.free()
 
 
-def test_queue_match_precision_bf16(gpu, oracle):
-    """match_precision = bf16: queued batches, from ScheduleLocalization and from ScheduleFrame's
-    pitched frame slots, run the profile kernel and the grouped matrix-core match; results equal
-    the oracle's ProcessJob, also after the radial weights change between batches."""
+@pytest.mark.parametrize("mp", ["bf16", "fp32_grouped"])
+def test_queue_match_precision_grouped(gpu, oracle, mp):
+    """match_precision = bf16 / fp32_grouped: queued batches, from ScheduleLocalization and from
+    ScheduleFrame's pitched frame slots, run the profile kernel and the grouped match (matrix-core
+    or reference arithmetic); results equal the oracle's ProcessJob, also after the radial weights
+    change between batches."""
     w = synth.make_workload("realtime100", n=300)
     trk = _tracker(w)
     trk.set_config_value("batch_size", "128")
-    trk.set_config_value("match_precision", "bf16")
-    assert trk.get_config_values()["match_precision"] == "bf16"
+    trk.set_config_value("match_precision", mp)
+    assert trk.get_config_values()["match_precision"] == mp
     got = trk.localize_all(w["rois"], w["jobs"])
     ref, _, _ = _oracle(oracle, w)
-    _report("queue bf16", got, ref)
+    _report(f"queue {mp}", got, ref)
     _assert_parity(got, ref)
     wts = np.linspace(0.3, 1.7, trk.cfg.zlut_radialsteps).astype(np.float32)
     trk.set_radial_weights(wts)
@@ -188,7 +202,7 @@ def test_queue_match_precision_bf16(gpu, oracle):
     r = _by_index(trk.fetch_results(20))
     trk.free()
     ref = _oracle_frames(oracle, w, w["luts"], [(frame, pos, job)], zcmp=wts)
-    _report("queue bf16 frames weighted", r, ref)
+    _report(f"queue {mp} frames weighted", r, ref)
     _assert_parity(r, ref)
     with pytest.raises(api.QTrkError, match="match_precision"):
         t2 = _tracker(w)

@@ -31,7 +31,8 @@ def _batch_results(b, n):
     return b.download(n)
 
 
-@pytest.mark.parametrize("prec", [api.MATCH_BF16, api.MATCH_FP32], ids=["bf16", "fp32"])
+@pytest.mark.parametrize("prec", [api.MATCH_BF16, api.MATCH_FP32, api.MATCH_FP32_GROUPED],
+                         ids=["bf16", "fp32", "fp32_grouped"])
 def test_noalloc_equals_batch_run(gpu, prec):
     import torch
 
