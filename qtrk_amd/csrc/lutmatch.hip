@@ -24,6 +24,7 @@
 #include "peakfit.hpp"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace qtrk {
 
@@ -184,15 +185,15 @@ __host__ __device__ inline MfmaLds mfma_lds(int FT, int R, int KS, int ptiles)
 struct ExactLds {
 	int prof, w, sc, row, total;
 };
-__host__ __device__ inline ExactLds exact_lds(int R, int planes)
+__host__ __device__ inline ExactLds exact_lds(int R, int planes, int tile = kMatchExactTile)
 {
 	ExactLds L;
 	const int R4 = (R + 3) & ~3;
 	int o = 0;
-	L.prof = o;  o += a16(kMatchExactTile * R4 * 4);
+	L.prof = o;  o += a16(tile * R4 * 4);
 	L.w = o;     o += a16(R4 * 4);
-	L.sc = o;    o += a16(kMatchExactTile * planes * 8);
-	L.row = o;   o += a16(kMatchExactTile * 4);
+	L.sc = o;    o += a16(tile * planes * 8);
+	L.row = o;   o += a16(tile * 4);
 	L.total = o;
 	return L;
 }
@@ -483,27 +484,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void l
 
 // Up to 16 profiles of one bead per workgroup; lane per plane, all 16 scores
 // accumulated together so each LUT row is read once per 16 profiles.
-template <bool WEIGHTED>
+template <bool WEIGHTED, int TILE>
 __device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, int chunk, unsigned char* smem)
 {
-	const int s0 = P.start[bead] + chunk * kMatchExactTile;
+	const int s0 = P.start[bead] + chunk * TILE;
 	const int rem = P.start[bead + 1] - s0;
-	const int nf = rem < kMatchExactTile ? rem : kMatchExactTile;
+	const int nf = rem < TILE ? rem : TILE;
 	if (nf <= 0) return;
 
 	const int R = P.res, R4 = (R + 3) & ~3, NP = P.planes;
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-	const ExactLds L = exact_lds(R, NP);
+	const ExactLds L = exact_lds(R, NP, TILE);
 	float* prof_s = (float*)(smem + L.prof);
 	float* w_s = (float*)(smem + L.w);
 	double* sc_s = (double*)(smem + L.sc);
 	int* row_s = (int*)(smem + L.row);
 
-	if (tid < kMatchExactTile) row_s[tid] = tid < nf ? (P.rowidx ? P.rowidx[s0 + tid] : s0 + tid) : -1;
+	if (tid < TILE) row_s[tid] = tid < nf ? (P.rowidx ? P.rowidx[s0 + tid] : s0 + tid) : -1;
 	if (WEIGHTED)
 		for (int r = tid; r < R4; r += 256) w_s[r] = r < R ? P.w[r] : 0.0f;
 	__syncthreads();
-	for (int e = tid; e < kMatchExactTile * R4; e += 256) {
+	for (int e = tid; e < TILE * R4; e += 256) {
 		const int f = e / R4, r = e - f * R4;
 		prof_s[e] = (f < nf && r < R) ? P.prof[(size_t)row_s[f] * R + r] : 0.0f;
 	}
@@ -512,15 +513,15 @@ __device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, i
 	const float* Lb = P.lut + (size_t)bead * NP * R;
 	for (int p = tid; p < NP; p += 256) {
 		const float* Lr = Lb + (size_t)p * R;
-		double acc[kMatchExactTile];
+		double acc[TILE];
 #pragma unroll
-		for (int f = 0; f < kMatchExactTile; f++) acc[f] = 0.0;
+		for (int f = 0; f < TILE; f++) acc[f] = 0.0;
 		// radius r for all 16 profiles: float difference, weight, negated square, double sum
 		// (unweighted: one exact fused multiply-add, as in score_step)
 		auto step = [&](int r, float l) {
 			const double wr = WEIGHTED ? (double)w_s[r] : 1.0;
 #pragma unroll
-			for (int f = 0; f < kMatchExactTile; f++) {
+			for (int f = 0; f < TILE; f++) {
 				double d = (double)(prof_s[f * R4 + r] - l);
 				if (WEIGHTED) {
 					d *= wr;
@@ -552,7 +553,7 @@ __device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, i
 		}
 		for (; r < R; r++) step(r, Lr[r]);
 #pragma unroll
-		for (int f = 0; f < kMatchExactTile; f++)
+		for (int f = 0; f < TILE; f++)
 			if (f < nf) sc_s[f * NP + p] = acc[f];
 	}
 	__syncthreads();
@@ -583,7 +584,7 @@ __device__ __forceinline__ void exact_chunk(const LutMatchParams& P, int bead, i
 	}
 }
 
-template <bool WEIGHTED, bool LOOP>
+template <bool WEIGHTED, bool LOOP, int TILE>
 __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 {
 	extern __shared__ __align__(16) unsigned char smem[];
@@ -592,11 +593,11 @@ __global__ __launch_bounds__(256) void lutmatch_exact_kernel(LutMatchParams P)
 	if (bead >= P.beads) return;
 	const int count = P.start[bead + 1] - P.start[bead];
 	if (!LOOP) {
-		if (chunk * kMatchExactTile < count) exact_chunk<WEIGHTED>(P, bead, chunk, smem);
+		if (chunk * TILE < count) exact_chunk<WEIGHTED, TILE>(P, bead, chunk, smem);
 		return;
 	}
-	for (; chunk * kMatchExactTile < count; chunk += P.chunks) {
-		exact_chunk<WEIGHTED>(P, bead, chunk, smem);
+	for (; chunk * TILE < count; chunk += P.chunks) {
+		exact_chunk<WEIGHTED, TILE>(P, bead, chunk, smem);
 		__syncthreads();
 	}
 }
@@ -766,13 +767,13 @@ static hipError_t launch_mfma_prec(const LutMatchParams& q, int FT, int grid, hi
 	return loop ? launch_mfma_loop<PREC, true>(q, FT, grid, stream) : launch_mfma_loop<PREC, false>(q, FT, grid, stream);
 }
 
-template <bool WEIGHTED, bool LOOP>
+template <bool WEIGHTED, bool LOOP, int TILE>
 static hipError_t launch_exact(const LutMatchParams& q, int grid, size_t lds, hipStream_t stream)
 {
-	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<WEIGHTED, LOOP>,
+	hipError_t e = hipFuncSetAttribute((const void*)lutmatch_exact_kernel<WEIGHTED, LOOP, TILE>,
 	                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
 	if (e != hipSuccess) return e;
-	lutmatch_exact_kernel<WEIGHTED, LOOP><<<grid, 256, lds, stream>>>(q);
+	lutmatch_exact_kernel<WEIGHTED, LOOP, TILE><<<grid, 256, lds, stream>>>(q);
 	return hipGetLastError();
 }
 
@@ -783,13 +784,25 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 	LutMatchParams q = p;
 	if (q.zstride <= 0) q.zstride = 1;
 	if (prec == kMatchFP32) {
-		q.chunks = (max_per_bead + kMatchExactTile - 1) / kMatchExactTile;
+		// 16 profiles per workgroup (each LUT row read once per 16), or 8 or 4 when 16 would leave
+		// fewer than two workgroups per CU (a few profiles per bead, as in one camera batch)
+		int tile = kMatchExactTile;
+		while (tile > 4 && (long long)p.beads * ((max_per_bead + tile - 1) / tile) < 2 * kMatchCUs) tile /= 2;
+		q.chunks = (max_per_bead + tile - 1) / tile;
 		const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
-		const size_t lds = (size_t)exact_lds(p.res, p.planes).total;
+		const size_t lds = (size_t)exact_lds(p.res, p.planes, tile).total;
 		if (lds > 160 * 1024) return hipErrorInvalidValue;
-		if (p.w)
-			return loop_tiles ? launch_exact<true, true>(q, grid, lds, stream) : launch_exact<true, false>(q, grid, lds, stream);
-		return loop_tiles ? launch_exact<false, true>(q, grid, lds, stream) : launch_exact<false, false>(q, grid, lds, stream);
+		auto go = [&](auto T) -> hipError_t {
+			constexpr int TL = decltype(T)::value;
+			if (p.w)
+				return loop_tiles ? launch_exact<true, true, TL>(q, grid, lds, stream)
+				                  : launch_exact<true, false, TL>(q, grid, lds, stream);
+			return loop_tiles ? launch_exact<false, true, TL>(q, grid, lds, stream)
+			                  : launch_exact<false, false, TL>(q, grid, lds, stream);
+		};
+		if (tile == 16) return go(std::integral_constant<int, 16>());
+		if (tile == 8) return go(std::integral_constant<int, 8>());
+		return go(std::integral_constant<int, 4>());
 	}
 	if (p.res > kMatchMaxRes) return hipErrorInvalidValue;
 	if (frame_tile != 0 && frame_tile != 32 && frame_tile != 64) return hipErrorInvalidValue;

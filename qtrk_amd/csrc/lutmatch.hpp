@@ -39,7 +39,8 @@ constexpr int kMatchFP32Grouped = 3;
 inline int match_kernel_precision(int form) { return form == kMatchFP32Grouped ? kMatchFP32 : form; }
 constexpr int kMatchTopK = 8;
 constexpr int kMatchMaxRes = 256;     // radial steps supported by the matrix-core path
-constexpr int kMatchExactTile = 16;   // profiles per workgroup, exact path
+constexpr int kMatchExactTile = 16;   // profiles per workgroup, exact path (8 or 4 for few per bead)
+constexpr int kMatchCUs = 256;        // compute units of an MI355X
 
 struct LutMatchParams {
 	int beads, planes, res;
