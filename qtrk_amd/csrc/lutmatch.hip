@@ -777,6 +777,18 @@ static hipError_t launch_exact(const LutMatchParams& q, int grid, size_t lds, hi
 	return hipGetLastError();
 }
 
+// Profiles per workgroup of the exact path: 16 (each LUT row read once per 16), or 8 or 4 when 16
+// would leave fewer than two workgroups per CU (a few profiles per bead, as in one camera batch)
+// or when the per-profile score rows of 16 would not fit the LDS (many planes).
+static int exact_tile(int beads, int max_per_bead, int res, int planes)
+{
+	int tile = kMatchExactTile;
+	while (tile > 4 && ((long long)beads * ((max_per_bead + tile - 1) / tile) < 2 * kMatchCUs ||
+	                    exact_lds(res, planes, tile).total > 160 * 1024))
+		tile /= 2;
+	return tile;
+}
+
 hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, hipStream_t stream, int frame_tile,
                            bool loop_tiles)
 {
@@ -784,10 +796,7 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 	LutMatchParams q = p;
 	if (q.zstride <= 0) q.zstride = 1;
 	if (prec == kMatchFP32) {
-		// 16 profiles per workgroup (each LUT row read once per 16), or 8 or 4 when 16 would leave
-		// fewer than two workgroups per CU (a few profiles per bead, as in one camera batch)
-		int tile = kMatchExactTile;
-		while (tile > 4 && (long long)p.beads * ((max_per_bead + tile - 1) / tile) < 2 * kMatchCUs) tile /= 2;
+		const int tile = exact_tile(p.beads, max_per_bead, p.res, p.planes);
 		q.chunks = (max_per_bead + tile - 1) / tile;
 		const int grid = (p.beads + 7) / 8 * 8 * q.chunks;
 		const size_t lds = (size_t)exact_lds(p.res, p.planes, tile).total;
@@ -814,9 +823,9 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 	return hipErrorInvalidValue;
 }
 
-size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile)
+size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile, int beads)
 {
-	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes).total;
+	if (prec == kMatchFP32) return (size_t)exact_lds(res, planes, exact_tile(beads, max_per_bead, res, planes)).total;
 	const int FT = frame_tile ? frame_tile / 32 : 1;
 	return (size_t)mfma_lds(FT, res, ksteps, (planes + 31) / 32).total;
 }

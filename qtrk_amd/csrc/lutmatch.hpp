@@ -84,7 +84,8 @@ hipError_t launch_lutmatch(const LutMatchParams& p, int prec, int max_per_bead, 
 // every bead's count unless loop_tiles, where a workgroup also takes the bead's later tiles (every
 // ceil(max_per_bead / tile)-th): then any max_per_bead is correct, a smaller one only slower.
 // Dynamic LDS bytes of that launch.
-size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile = 0);
+size_t lutmatch_lds_bytes(int prec, int res, int planes, int ksteps, int max_per_bead, int frame_tile = 0,
+                          int beads = 1);
 
 // Groups rows 0..n-1 by bead (bead of row i at bead_idx[i * stride]; rows of other indices are left
 // out): counts [beads] (scratch, ends as fill cursors), start [beads + 1], rowidx [n].

@@ -179,8 +179,8 @@ QTRK_API int QTrkLUTMatchRun(QTrkLUTMatch* m, int precision, int repeats)
 		HIP_CHECK(hipSetDevice(m->dev));
 		const LutMatchTables& t = *m->tab;
 		if (precision == QTRK_MATCH_FP32) {
-			if (lutmatch_lds_bytes(precision, t.res, t.planes, t.ksteps, m->frames) > 160 * 1024)
-				fail("too many planes for the FP32 match (at most about 1,200)");
+			if (lutmatch_lds_bytes(precision, t.res, t.planes, t.ksteps, m->frames, 0, t.beads) > 160 * 1024)
+				fail("too many planes for the FP32 match (at most about 4,900)");
 		} else {
 			m->tab->prepare(precision, m->stream);
 		}

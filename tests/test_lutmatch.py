@@ -15,16 +15,18 @@ from qtrk_amd import api, synth
 PRECS = {"fp32": api.MATCH_FP32, "bf16": api.MATCH_BF16, "fp16": api.MATCH_FP16}
 
 
-def make_case(beads, planes, res, frames, seed, noise=0.1):
+def make_case(beads, planes, res, frames, seed, noise=0.1, span=None, zmax=None):
     """Per-bead analytic LUTs (normalised rows) and profiles interpolated between
-    planes at random z with Gaussian noise, normalised like measured profiles."""
+    planes at random z (below zmax if given) with Gaussian noise, normalised like measured
+    profiles.  The LUT family's ring period and width change by as much over `span` planes
+    (default: all of them) as realtime100's over its 200."""
     rng = np.random.default_rng(seed)
-    scale = 200.0 / max(planes, 2)
+    scale = 200.0 / max(span or planes, 2)
     luts = np.stack([synth.normalize_profiles(synth.analytic_lut(planes, res, 1.0, 50.0, rng.uniform(7, 10),
                                                                  dperiod=0.04 * scale, dwidth=0.05 * scale))
                      for _ in range(beads)])
     lo = min(2.0, (planes - 1) / 2)
-    z = rng.uniform(lo, planes - 1 - lo, size=(beads, frames))
+    z = rng.uniform(lo, (zmax or planes) - 1 - lo, size=(beads, frames))
     i = np.clip(np.floor(z).astype(int), 0, planes - 2) if planes > 1 else np.zeros_like(z, int)
     t = (z - i)[..., None] if planes > 1 else np.zeros(z.shape + (1,))
     rows = luts[np.arange(beads)[:, None], i]
@@ -50,6 +52,13 @@ SHAPES = [
     (3, 37, 50, 33),     # ragged: planes, radial steps and profiles off every tile size
     (2, 130, 100, 150),  # more than one 64-profile tile per bead
     (2, 5, 16, 7),       # fewer planes than candidates
+    (2, 700, 160, 40),   # 22 plane tiles (two tile-mean tiles), 160 radial steps (16 k-steps)
+    # 66 plane tiles: five tile-mean jobs, one wavefront takes two.  Neighbouring planes as far
+    # apart as at 500 planes (the family spread over 2,100 differs less per plane than BF16
+    # resolves) and beads below plane 1,000 (the rings of the upper planes grow past the radial
+    # range and their profiles flatten); every candidate must then lie near the best plane, which
+    # a wrong tile-mean term in the later tiles would break.
+    (1, 2100, 48, 20, 500, 1000),
 ]
 
 
@@ -57,8 +66,10 @@ SHAPES = [
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("weighted", [False, True], ids=["unweighted", "weighted"])
 def test_match_parity(oracle, oracle64, gpu, shape, weighted):
-    beads, planes, res, frames = shape
-    luts, prof, ztrue = make_case(beads, planes, res, frames, seed=sum(shape) + weighted)
+    beads, planes, res, frames = shape[:4]
+    local = len(shape) > 4
+    luts, prof, ztrue = make_case(beads, planes, res, frames, seed=sum(shape) + weighted,
+                                  span=shape[4] if local else None, zmax=shape[5] if local else None)
     w = np.linspace(0.5, 1.5, res).astype(np.float32) if weighted else None
     zo, bo = oracle_match(oracle, luts, prof, w)
     z64, _ = oracle_match(oracle64, luts, prof, w)
@@ -76,6 +87,8 @@ def test_match_parity(oracle, oracle64, gpu, shape, weighted):
                 srt = np.sort(np.where(valid, cand, -1 - np.arange(8)), axis=-1)
                 assert np.all(np.diff(srt, axis=-1) != 0), "duplicate candidates"
                 assert np.all(np.any(cand == bo[..., None], axis=-1))
+                if local:
+                    assert np.all(np.abs(cand - bo[..., None]) <= 16), "candidates far from the best plane"
             print(f"  {name}: |z - FP64 oracle| max {np.max(np.abs(z - z64)):.3g} planes, "
                   f"rms vs truth {np.sqrt(np.mean((z - ztrue) ** 2)):.3g}")
     finally:
