@@ -7,7 +7,7 @@
 // followed by the 7-point FP64 fit around the best plane.  Here the profiles
 // of one bead are matched together:
 //   * exact path: every plane scored in exactly that arithmetic, with each
-//     LUT row read once per 16 profiles;
+//     LUT row read once per 8 profiles;
 //   * matrix-core path: -score = |p|^2_w - 2 p.L_w + |L|^2_w, and the cross
 //     term for all (profile, plane) pairs is computed on the matrix cores.
 //     Each tile of 32 planes is stored as its mean row M_t plus BF16/FP16
@@ -39,7 +39,7 @@ constexpr int kMatchFP32Grouped = 3;
 inline int match_kernel_precision(int form) { return form == kMatchFP32Grouped ? kMatchFP32 : form; }
 constexpr int kMatchTopK = 8;
 constexpr int kMatchMaxRes = 256;     // radial steps supported by the matrix-core path
-constexpr int kMatchExactTile = 16;   // profiles per workgroup, exact path (8 or 4 for few per bead)
+constexpr int kMatchExactTile = 8;    // profiles per workgroup, exact path (4 for few per bead)
 constexpr int kMatchCUs = 256;        // compute units of an MI355X
 
 struct LutMatchParams {
