@@ -190,6 +190,25 @@ void QueuedHIPTracker::BuildTables()
 		qd[2 * j + 1] = sinf(ang);
 	}
 	upload(d_qi_dirs, qd.data(), qd.size() * 4);
+	// each quadrant's sample offsets from the centre, (mx*dx)*r and (my*dy)*r in float as the
+	// reference forms them (cputrack/cpu_tracker.cpp:649-659): the sign flips are exact, so the
+	// products are the kernel's own; one table for every ROI, read instead of formed per sample
+	{
+		const float minr = cfg.qi_minradius, maxr = cfg.qi_maxradius;
+		const float rstep = (maxr - minr) / (float)nr;
+		std::vector<float> offs(8 * (size_t)A * nr);
+		for (int q = 0; q < 4; q++) {
+			const float mx = (q == 0 || q == 3) ? 1.0f : -1.0f, my = q < 2 ? 1.0f : -1.0f;
+			for (int j = 0; j < A; j++)
+				for (int i = 0; i < nr; i++) {
+					const float r = minr + rstep * (float)i;
+					float* o = &offs[2 * (((size_t)q * A + j) * nr + i)];
+					o[0] = (mx * qd[2 * j]) * r;
+					o[1] = (my * qd[2 * j + 1]) * r;
+				}
+		}
+		upload(d_qi_offs, offs.data(), offs.size() * 4);
+	}
 
 	// Angular samples and table stride per iteration (cputrack/cpu_tracker.cpp:314,
 	// 356, 642-657): the float count is truncated, clamped to >= 4, and the
@@ -316,6 +335,7 @@ void QueuedHIPTracker::FillParams(LocalizeParams& P, int pdt, int n, const void*
 	P.qi_iter_steps = d_qi_steps.as<int>();
 	P.qi_iter_stride = d_qi_stride.as<int>();
 	P.qi_dirs = d_qi_dirs.as<float2>();
+	P.qi_offs = d_qi_offs.as<float2>();
 	P.qi_window = d_qi_window.as<float>();
 	P.qi_fft = qi_fft.plan();
 	P.xc_len = cfg.xc1_profileLength;

@@ -179,7 +179,7 @@ private:
 	hipStream_t aux_stream() const { return slots[0]->stream; }
 
 	// per-config device tables
-	DeviceBuffer d_qi_steps, d_qi_stride, d_qi_dirs, d_qi_window, d_z_dirs;
+	DeviceBuffer d_qi_steps, d_qi_stride, d_qi_dirs, d_qi_offs, d_qi_window, d_z_dirs;
 	HostFFT qi_fft;   // length 2*qi_radialsteps
 	HostFFT xc_fft;   // length xc1_profileLength (LT_XCor1D), when 2..kMaxFFTLen
 	HostFFT qa_fft;   // length 2*zlut_radialsteps (LT_ZLUTAlign), when <= kMaxFFTLen

@@ -101,7 +101,7 @@ __host__ __device__ inline LdsLayout lds_layout(const LocalizeParams& P, bool ex
 	const int u = o;
 	// QI phase: directions, quadrant profiles, 4 FFT buffers, twiddles, leaf order
 	int q = u;
-	L.qdir = q;   q += align16(4 * P.qi_A * 8);  // per quadrant, signs applied
+	L.qdir = q;   q += align16(4 * P.qi_A * 8);  // per quadrant, signs applied (instances without the offset table)
 	L.qprof = q;  q += align16(4 * P.qi_nr * 4);
 	L.fft = q;    q += align16(4 * N * 8);
 	L.tw = q;     q += align16(2 * N * 8);
@@ -205,6 +205,19 @@ __device__ __forceinline__ int mad_u24(int a, int b, int c)
 	int r;
 	asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(b), "v"(c));
 	return r;
+}
+
+// A float2 of a table at a per-lane byte offset plus a wavefront-uniform one, as one buffer
+// load: the uniform part is the scalar offset (a global load would form a 64-bit address per
+// lane and load); reads past the table's end return zeros.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const void* base, int bytes)
+{
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 qi_offset(__amdgpu_buffer_rsrc_t table, unsigned lane_off, int soff)
+{
+	const auto v = __builtin_amdgcn_raw_buffer_load_b64(table, lane_off, __builtin_amdgcn_readfirstlane(soff), 0);
+	return make_float2(__uint_as_float(v[0]), __uint_as_float(v[1]));
 }
 
 // Bilinear samples with the reference's "outside" rule (cputrack/utils.h:39-58):
@@ -1073,6 +1086,11 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	float* st = (float*)(lds + L.st);
 
 	const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+	// instances for ROIs of 96 px and more read the QI sample offsets from P.qi_offs (buffer loads,
+	// prefetched a group of angles ahead) instead of forming them from LDS directions: fewer VALU and
+	// LDS instructions per sample for 10 more VGPRs, which these instances have to spare (their LDS
+	// already holds a CU to three ROIs), while smaller ones would lose a wavefront per SIMD
+	constexpr bool kQiTable = CW >= 96;
 	const int roi = blockIdx.x;
 	const int W = CW ? CW : P.width, H = CH ? CH : P.height, npx = W * H;
 	const bool build = P.build != 0;
@@ -1156,24 +1174,28 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			tw[N + i] = P.qi_fft.tw_inv[i];
 			perm[i] = P.qi_fft.perm[i];
 		}
-		// direction table angle-major, the 4 quadrants' entries of an angle adjacent (a wavefront's
-		// 4 quadrants read 4 neighbouring words, in different banks), each with the quadrant's
-		// signs applied: (mx*dx, my*dy), exact sign flips of the reference's mx*quadrantDirs[i].x
-		// (cputrack/cpu_tracker.cpp:658-659)
-		for (int j = tid; j < 4 * P.qi_A; j += kBlock) {
-			const int a = j >> 2, q = j & 3;
-			const float2 d = P.qi_dirs[a];
-			const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
-			const float sy = (q < 2) ? 1.0f : -1.0f;
-			qdir[j] = make_float2(sx * d.x, sy * d.y);
+		const int A = P.qi_A;
+		const __amdgpu_buffer_rsrc_t qtab = table_rsrc(P.qi_offs, 4 * A * nr * 8);
+		if constexpr (!kQiTable) {
+			// direction table angle-major, the 4 quadrants' entries of an angle adjacent (a wavefront's
+			// 4 quadrants read 4 neighbouring words, in different banks), each with the quadrant's
+			// signs applied: (mx*dx, my*dy), exact sign flips of the reference's mx*quadrantDirs[i].x
+			// (cputrack/cpu_tracker.cpp:658-659)
+			for (int j = tid; j < 4 * P.qi_A; j += kBlock) {
+				const int a = j >> 2, q = j & 3;
+				const float2 d = P.qi_dirs[a];
+				const float sx = (q == 0 || q == 3) ? 1.0f : -1.0f;
+				const float sy = (q < 2) ? 1.0f : -1.0f;
+				qdir[j] = make_float2(sx * d.x, sy * d.y);
+			}
 		}
 		__syncthreads();
 		for (int k = 0; k < P.qi_iters; k++) {
 			boundary = (cx + maxr >= (float)W) || (cx - maxr < 0.0f) || (cy + maxr >= (float)H) || (cy - maxr < 0.0f);
 			const int steps = P.qi_iter_steps[k], stride = P.qi_iter_stride[k];
 			const bool ok = !isnan(cx) && !isnan(cy);
-			// uniform loop over (quadrant, radius) slots: the angle index is the same in
-			// every lane, so the direction reads from LDS are broadcasts
+			// uniform loop over (quadrant, radius) slots: the angle index is the same in every lane
+			// (with the offset table, a half-wavefront's offsets are 256 contiguous bytes of it).
 			// (quadrant, radius) slots in segments of 32 consecutive radii of one quadrant: each
 			// half-wavefront samples along one ray (the LDS serves a half-wavefront at a time, and
 			// neighbouring radii read neighbouring pixels), and a wavefront's two segments share
@@ -1195,43 +1217,109 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 				const bool clear = !valid || (ok && circle_inside(cx, cy, r, Wf, Hf));
 				const bool fast = __ballot(!clear) == 0;  // uniform over the wavefront
 				if (valid) {
-					const float2* qd = qdir + q;
-					const int st4 = 4 * stride;
 					float sum = 0.0f;
 					int cnt = 0;
-					int a = 0;
-					if (fast) {
-						for (; a + 4 <= steps; a += 4) {
-							float x[4], y[4];
+					if constexpr (kQiTable) {
+						// this lane's column of the offset table (angle j at element j * nr); the uniform
+						// part of each address advances in scalar registers
+						const int jstep8 = stride * nr * 8;
+						const unsigned lane_off = (unsigned)(((q * A) * nr + ri) * 8);
+						// one vector offset per angle of a group: the group's start is the only scalar
+						const unsigned voff[4] = { lane_off, lane_off + jstep8, lane_off + 2 * jstep8,
+						                           lane_off + 3 * jstep8 };
+						int a = 0;
+						const int n4 = steps & ~3;
+						// groups of 4 angles; the next group's offsets load while the current group is
+						// sampled (two register sets in turn, the scheduling barriers keep each group's
+						// loads ahead of the previous group's samples; the loop's last load repeats a
+						// group instead of reading past the table)
+						float2 o0[4], o1[4];
+						auto load4 = [&](float2 (&o)[4], int a0) {
+#pragma unroll
+							for (int u = 0; u < 4; u++) o[u] = qi_offset(qtab, voff[u], a0 * jstep8);
+						};
+						auto groups = [&](auto&& sample4) {
+							if (n4 == 0) return;
+							load4(o0, 0);
+							int g = 0;
+							for (; g + 8 <= n4; g += 8) {
+								load4(o1, g + 4);
+								__builtin_amdgcn_sched_barrier(0);
+								sample4(o0);
+								load4(o0, g + 8 < n4 ? g + 8 : g + 4);
+								__builtin_amdgcn_sched_barrier(0);
+								sample4(o1);
+							}
+							if (g < n4) sample4(o0);  // an odd number of groups: the last one
+						};
+						auto xy4 = [&](const float2 (&o)[4], float (&x)[4], float (&y)[4]) {
 #pragma unroll
 							for (int u = 0; u < 4; u++) {
-								const float2 d = qd[st4 * (a + u)];
-								x[u] = cx + d.x * r;
-								y[u] = cy + d.y * r;
+								x[u] = cx + o[u].x;
+								y[u] = cy + o[u].y;
 							}
-							sample_n_inside<SPix, float, 4>(img, W, x, y, sum);
+						};
+						if (fast) {
+							groups([&](const float2 (&o)[4]) {
+								float x[4], y[4];
+								xy4(o, x, y);
+								sample_n_inside<SPix, float, 4>(img, W, x, y, sum);
+							});
+							for (a = n4; a < steps; a++) {
+								const float2 d = qi_offset(qtab, lane_off, a * jstep8);
+								const float x[1] = { cx + d.x }, y[1] = { cy + d.y };
+								sample_n_inside<SPix, float, 1>(img, W, x, y, sum);
+							}
+							cnt = steps;
+						} else {
+							groups([&](const float2 (&o)[4]) {
+								float x[4], y[4];
+								xy4(o, x, y);
+								sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+							});
+							for (a = n4; a < steps; a++) {
+								const float2 d = qi_offset(qtab, lane_off, a * jstep8);
+								const float x[1] = { cx + d.x }, y[1] = { cy + d.y };
+								sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+							}
 						}
-						for (; a < steps; a++) {
-							const float2 d = qd[st4 * a];
-							const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
-							sample_n_inside<SPix, float, 1>(img, W, x, y, sum);
-						}
-						cnt = steps;
 					} else {
-						for (; a + 4 <= steps; a += 4) {
-							float x[4], y[4];
+						const float2* qd = qdir + q;
+						const int st4 = 4 * stride;
+						int a = 0;
+						if (fast) {
+							for (; a + 4 <= steps; a += 4) {
+								float x[4], y[4];
 #pragma unroll
-							for (int u = 0; u < 4; u++) {
-								const float2 d = qd[st4 * (a + u)];
-								x[u] = cx + d.x * r;
-								y[u] = cy + d.y * r;
+								for (int u = 0; u < 4; u++) {
+									const float2 d = qd[st4 * (a + u)];
+									x[u] = cx + d.x * r;
+									y[u] = cy + d.y * r;
+								}
+								sample_n_inside<SPix, float, 4>(img, W, x, y, sum);
 							}
-							sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
-						}
-						for (; a < steps; a++) {
-							const float2 d = qd[st4 * a];
-							const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
-							sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+							for (; a < steps; a++) {
+								const float2 d = qd[st4 * a];
+								const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+								sample_n_inside<SPix, float, 1>(img, W, x, y, sum);
+							}
+							cnt = steps;
+						} else {
+							for (; a + 4 <= steps; a += 4) {
+								float x[4], y[4];
+#pragma unroll
+								for (int u = 0; u < 4; u++) {
+									const float2 d = qd[st4 * (a + u)];
+									x[u] = cx + d.x * r;
+									y[u] = cy + d.y * r;
+								}
+								sample_n_acc<SPix, float, 4>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+							}
+							for (; a < steps; a++) {
+								const float2 d = qd[st4 * a];
+								const float x[1] = { cx + d.x * r }, y[1] = { cy + d.y * r };
+								sample_n_acc<SPix, float, 1>(img, W, Wf, Hf, Wm2, Hm2, ok, x, y, sum, cnt);
+							}
 						}
 					}
 					float val = cnt >= 4 ? sum / (float)cnt : mean;

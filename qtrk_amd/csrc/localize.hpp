@@ -88,6 +88,10 @@ struct LocalizeParams {
 	const int* qi_iter_steps;   // [qi_iters] angular samples per quadrant
 	const int* qi_iter_stride;  // [qi_iters] step through the direction table
 	const float2* qi_dirs;      // [qi_A]
+	// [4][qi_A][qi_nr]: sample offsets (fl(mx*dx_j * r_i), fl(my*dy_j * r_i)) of quadrant q, angle j,
+	// radius r_i = minr + ((maxr - minr) / nr) * i, the float products the reference adds to the
+	// centre (cputrack/cpu_tracker.cpp:658-659); the same for every ROI
+	const float2* qi_offs;
 	const float* qi_window;     // [qi_nr] or null
 
 	FFTPlan qi_fft;             // length 2*qi_nr (a power of two)
