@@ -566,8 +566,9 @@ class DeviceBatch:
         return out
 
     def set_match_precision(self, precision):
-        """MATCH_FP32 (per-ROI compare in the localization kernel) or MATCH_BF16 / MATCH_FP16
-        (profile kernel + grouped matrix-core match)."""
+        """MATCH_FP32 (per-ROI compare in the localization kernel), MATCH_BF16 / MATCH_FP16
+        (profile kernel + grouped matrix-core match) or MATCH_FP32_GROUPED (profile kernel + grouped
+        match in the reference arithmetic)."""
         if lib().QTrkBatchSetMatchPrecision(self.h, int(precision)) != 0:
             check()
 

@@ -173,6 +173,39 @@ def test_batch_grouped_match_parity(gpu, oracle, prec):
     trk.free()
 
 
+def test_batch_grouped_match_many_rois(gpu):
+    """The grouped forms on a batch of 10,240 ROIs (more than the grouping kernel's 8,192 rows per
+    pass, the last pass partial), beads of 40 to 200 ROIs and ROIs of a bead outside the LUT: every
+    record byte-equal to the fused FP32 form's."""
+    w = synth.make_workload("realtime100", n=1024)
+    rois = np.tile(w["rois"], (10, 1, 1))
+    jobs = np.tile(w["jobs"], 10)
+    jobs["frame"] = np.arange(len(jobs))
+    jobs["zlutIndex"][::7] = 11
+    jobs["zlutIndex"][5::301] = 9999
+    trk = _tracker(w)
+    b = api.DeviceBatch(trk, len(jobs), api.QTrkU16)
+    b.enable_diagnostics(True)
+    b.upload(rois, jobs)
+    try:
+        out = {}
+        for prec in (api.MATCH_FP32, api.MATCH_BF16, api.MATCH_FP32_GROUPED):
+            print(f"grouped match, {len(jobs)} ROIs, precision {prec}", flush=True)
+            b.set_match_precision(prec)
+            b.run(len(jobs), 1)
+            b.sync()
+            out[prec] = (b.download(len(jobs)), b.diagnostics(len(jobs))[0])
+        ref, rbest = out[api.MATCH_FP32]
+        assert np.all(rbest[5::301] == -1)
+        for prec in (api.MATCH_BF16, api.MATCH_FP32_GROUPED):
+            got, best = out[prec]
+            assert np.array_equal(got.view(np.uint8), ref.view(np.uint8)), prec
+            assert np.array_equal(best, rbest), prec
+    finally:
+        b.free()
+        trk.free()
+
+
 @pytest.mark.parametrize("mp", ["bf16", "fp32_grouped"])
 def test_queue_match_precision_grouped(gpu, oracle, mp):
     """match_precision = bf16 / fp32_grouped: queued batches, from ScheduleLocalization and from
