@@ -1091,6 +1091,14 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	// LDS instructions per sample for 10 more VGPRs, which these instances have to spare (their LDS
 	// already holds a CU to three ROIs), while smaller ones would lose a wavefront per SIMD
 	constexpr bool kQiTable = CW >= 96;
+	// In the same instances the wavefronts issue at priority 1 except in the two sampling loops (QI
+	// and radial profile), which run at 0: when a SIMD holds a wavefront of one ROI's dependent
+	// stretch (transforms, peak fit, reductions, LUT scores) and one of another ROI's sampling, the
+	// first goes first, since the sampling of the other ROI has work to spare and its own ROI waits
+	// for it only at the end of the loop.  (Smaller instances, five ROIs per CU, measured slower for
+	// consecutive batches with it.)
+	constexpr bool kChainPrio = CW >= 96;
+	if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);
 	const int roi = blockIdx.x;
 	const int W = CW ? CW : P.width, H = CH ? CH : P.height, npx = W * H;
 	const bool build = P.build != 0;
@@ -1201,6 +1209,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			// neighbouring radii read neighbouring pixels), and a wavefront's two segments share
 			// their radius band, so wavefronts of inner radii, whose circles lie inside the ROI,
 			// take the unmasked loop
+			if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(0);
 			for (int base = 0; base < 4 * nr; base += kBlock) {
 				const int p = base + tid;
 				int q, ri;
@@ -1327,6 +1336,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					qprof[q * nr + ri] = val;
 				}
 			}
+			if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);
 			__syncthreads();
 			// wavefront w: the leaves of transform w (X, reversed X, Y, reversed Y) into buffer
 			// w, then its forward transform
@@ -1531,6 +1541,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			__syncthreads();
 			// slots bin-major (lane = nch chunks x bins), so that wavefronts of inner bins take
 			// the unmasked loop
+			if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(0);
 			for (int base = 0; base < nch * nb; base += kBlock) {
 				const int p = base + tid;
 				const int bin = p / nch, ch = p - bin * nch;
@@ -1581,6 +1592,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					zcnt[ch * nb + bin] = c;
 				}
 			}
+			if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);
 			__syncthreads();
 			for (int bin = tid; bin < nb; bin += kBlock) {
 				double s = zpart[bin];
