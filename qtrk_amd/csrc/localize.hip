@@ -1095,9 +1095,11 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	// and radial profile), which run at 0: when a SIMD holds a wavefront of one ROI's dependent
 	// stretch (transforms, peak fit, reductions, LUT scores) and one of another ROI's sampling, the
 	// first goes first, since the sampling of the other ROI has work to spare and its own ROI waits
-	// for it only at the end of the loop.  (Smaller instances, five ROIs per CU, measured slower for
-	// consecutive batches with it.)
+	// for it only at the end of the loop.  The other instances compiled for a size (five ROIs per CU)
+	// do so only between the QI sampling loops and run at 0 elsewhere: priority 1 outside the QI
+	// iterations too measured slower for their consecutive batches.
 	constexpr bool kChainPrio = CW >= 96;
+	constexpr bool kQiPrio = CW > 0;
 	if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);
 	const int roi = blockIdx.x;
 	const int W = CW ? CW : P.width, H = CH ? CH : P.height, npx = W * H;
@@ -1209,7 +1211,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			// neighbouring radii read neighbouring pixels), and a wavefront's two segments share
 			// their radius band, so wavefronts of inner radii, whose circles lie inside the ROI,
 			// take the unmasked loop
-			if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(0);
+			if constexpr (kQiPrio) __builtin_amdgcn_s_setprio(0);
 			for (int base = 0; base < 4 * nr; base += kBlock) {
 				const int p = base + tid;
 				int q, ri;
@@ -1336,7 +1338,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 					qprof[q * nr + ri] = val;
 				}
 			}
-			if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);
+			if constexpr (kQiPrio) __builtin_amdgcn_s_setprio(1);
 			__syncthreads();
 			// wavefront w: the leaves of transform w (X, reversed X, Y, reversed Y) into buffer
 			// w, then its forward transform
@@ -1398,6 +1400,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 			cy = st[1];
 			__syncthreads();
 		}
+		if constexpr (kQiPrio && !kChainPrio) __builtin_amdgcn_s_setprio(0);
 	}
 
 	// ---- 2D Gaussian MLE when QI is off (cputrack/cpu_tracker.cpp:514-589, selected at
