@@ -109,6 +109,24 @@ def test_workload_parity_angstep_factor(gpu, oracle):
     np.testing.assert_allclose(prof, rprof, atol=1e-5)
 
 
+@pytest.mark.parametrize("cov,factor", [(0.5, 1.0), (2.0, 1.3)], ids=["16_radial", "128_radial_factor1.3"])
+def test_qi_radial_steps_parity(gpu, oracle, cov, factor):
+    """100x100 ROIs (the instance reading the QI sample offsets from the tracker's table) with 16
+    QI radial steps (64 (quadrant, radius) slots: slots not in 32-radius segments, three wavefronts
+    without one) and with 128 (512 slots: two passes over the workgroup) at qi_angstep_factor 1.3;
+    bit-exact indexing and best plane, x/y/z within the bars, profiles equal."""
+    w = synth.make_workload("realtime100", n=512, qi_radial_coverage=cov, qi_angstep_factor=factor)
+    nr = api.computed_config(w["settings"]).qi_radialsteps
+    assert nr == (16 if cov < 1 else 128)
+    ref, rbest, rprof = _oracle(oracle, w)
+    trk = _tracker(w)
+    got, best, prof = _batch(trk, w["rois"], w["jobs"])
+    trk.free()
+    _report(f"realtime100 {nr} QI radial steps", got, ref)
+    _assert_parity(got, ref, best, rbest)
+    np.testing.assert_allclose(prof, rprof, atol=1e-5)
+
+
 @pytest.mark.parametrize("spec", synth.BATCHING, ids=[f"{s[0]}px" for s in synth.BATCHING])
 def test_batching_tracker_sizes_parity(gpu, oracle, spec):
     """The batching workload's four trackers (ROI 32/64/96/128 px, LUTs of 100/150/200/300 planes x
