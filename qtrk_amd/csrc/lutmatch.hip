@@ -460,8 +460,8 @@ __device__ __forceinline__ void mfma_chunk(const LutMatchParams& P, int bead, in
 
 // Workgroup (bead, chunk) takes the bead's profile tile `chunk` (the grid sized for the largest
 // bead) or, with LOOP, the tiles chunk, chunk + chunks, ... (a grid sized without knowing the
-// largest bead).  The loop is a separate instantiation: around the tile it keeps 34 more VGPRs
-// live (143 against 109), which costs a wavefront per SIMD.
+// largest bead).  The loop is a separate instantiation: around the tile it keeps 39 more VGPRs
+// live (159 against 120), which costs a wavefront per SIMD.
 template <int PREC, int FT, int KSMAX, bool LOOP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void lutmatch_mfma_kernel(LutMatchParams P)
 {
