@@ -1091,8 +1091,9 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	// LDS instructions per sample for 10 more VGPRs, which these instances have to spare (their LDS
 	// already holds a CU to three ROIs), while smaller ones would lose a wavefront per SIMD
 	constexpr bool kQiTable = CW >= 96;
-	// In the same instances the wavefronts issue at priority 1 except in the two sampling loops (QI
-	// and radial profile), which run at 0: when a SIMD holds a wavefront of one ROI's dependent
+	// In the same instances the wavefronts issue at priority 1 from the end of the COM on, except in
+	// the two sampling loops (QI and radial profile); the ROI staging, the COM and the sampling loops
+	// run at 0.  When a SIMD holds a wavefront of one ROI's dependent
 	// stretch (transforms, peak fit, reductions, LUT scores) and one of another ROI's sampling, the
 	// first goes first, since the sampling of the other ROI has work to spare and its own ROI waits
 	// for it only at the end of the loop.  The other instances compiled for a size (five ROIs per CU)
@@ -1100,7 +1101,6 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 	// iterations too measured slower for their consecutive batches.
 	constexpr bool kChainPrio = CW >= 96;
 	constexpr bool kQiPrio = CW > 0;
-	if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);
 	const int roi = blockIdx.x;
 	const int W = CW ? CW : P.width, H = CH ? CH : P.height, npx = W * H;
 	const bool build = P.build != 0;
@@ -1164,6 +1164,7 @@ __global__ void __launch_bounds__(kBlock) localize_kernel(LocalizeParams P)
 		}
 	}
 
+	if constexpr (kChainPrio) __builtin_amdgcn_s_setprio(1);  // staging and COM ran at 0
 	// ---- XCor1D, which takes precedence over QI (cputrack/QueuedCPUTracker.cpp:299-303)
 	float cx = comx, cy = comy;
 	int boundary = 0;
